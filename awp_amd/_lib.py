@@ -1,0 +1,77 @@
+"""ctypes binding of libawp_hip.so (the C ABI declared in include/awp_hip.h).
+
+The library is loaded from the package directory.  There is no fallback:
+if the shared library is missing, or no GPU is visible when a launcher is
+called, the call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libawp_hip.so"
+
+_fp = C.c_void_p  # device pointers travel as plain addresses
+_i = C.c_int
+_f = C.c_float
+_s = C.c_void_p  # hipStream_t
+
+# name -> argtypes, in header order
+SIGNATURES: dict[str, list] = {
+    "awp_last_error": [],
+    "awp_last_error_string": [],
+    "awp_clear_error": [],
+    "awp_set_tuning": [C.c_char_p, _i],
+    "SetDeviceConstValue": [C.POINTER(_f), _f, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i,
+                            _f, _f, C.POINTER(_f), C.POINTER(_f)],
+    "dvelcx_H_opt": [_fp] * 13 + [_i, _i, _s, _i, _i, _i, _i],
+    "dstrqc_H": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 10,
+    "dstrqc_H_new": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 10,
+    "fstr_H": [_fp] * 3 + [_s, _i, _i, _i, _i],
+    "addsrc_H": [_i, _i, _i, _fp, _i, _s] + [_fp] * 12 + [_i],
+    "velbuffer_H": [_fp] * 8 + [_i] * 13 + [_s, _i, _fp, _i],
+    "awp_receivers_gather": [_fp] * 4 + [_i, _fp, _i, _s, _i],
+}
+RESTYPES = {
+    "awp_last_error": _i,
+    "awp_last_error_string": C.c_char_p,
+    "awp_set_tuning": _i,
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m awp_amd.build` "
+            "(the HIP path has no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = RESTYPES.get(name, None)
+    _lib = lib
+    return lib
+
+
+class AwpError(RuntimeError):
+    pass
+
+
+def check() -> None:
+    """Raise if the library recorded an error since the last check."""
+    lib = load()
+    code = lib.awp_last_error()
+    if code:
+        msg = lib.awp_last_error_string().decode()
+        lib.awp_clear_error()
+        raise AwpError(f"awp_hip error {code}: {msg}")
+
+
+def set_tuning(key: str, value: int) -> None:
+    if load().awp_set_tuning(key.encode(), int(value)) != 0:
+        raise ValueError(f"unknown tuning key or bad value: {key}={value}")

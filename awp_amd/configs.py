@@ -1,0 +1,182 @@
+"""Synthetic workloads: media, sources and receivers for the benchmark
+configurations of the time-stepping path.
+
+  elastic_halfspace_64   64^3, h=100 m, dt=5 ms, vp 6000, vs 3464, rho 2700,
+                         elastic (Q^-1 = 0), isotropic moment-tensor Ricker
+                         source (M0 = 1e15 N m, f0 = 2 Hz) at the centre,
+                         200 steps, 8 seeded surface receivers.
+  anelastic_512          512x512x256, h=50 m, dt=2.5 ms, 1000 steps, layered
+                         vs 1500->4500 m/s, vp = 1.73 vs, Gardner rho, 5 %
+                         von Karman perturbation (a = 500 m, Hurst 0.5),
+                         Qs = 50 vs[km/s], Qp = 2 Qs, double-couple Ricker
+                         source f0 = 1 Hz.
+Every random draw is seeded.  Sizes can be scaled down (``scale``) for parity
+tests; the physics parameters stay the same.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .grid import Dims
+from .media import Medium, build_medium
+
+
+def ricker(t: np.ndarray, f0: float, t0: float | None = None) -> np.ndarray:
+    """Ricker wavelet with unit peak, centred at t0 (default 1.2/f0)."""
+    if t0 is None:
+        t0 = 1.2 / f0
+    a = (np.pi * f0 * (t - t0)) ** 2
+    return (1.0 - 2.0 * a) * np.exp(-a)
+
+
+@dataclass
+class PointSources:
+    """Point moment-rate sources in the reference's input convention
+    (src/awp/source.c:160-200): cells are 1-based (x, y) and z counted 1 at
+    the surface; rates are (npsrc, nst, 6) columns xx, yy, zz, xz, yz, xy."""
+    cells: np.ndarray          # (npsrc, 3) int, model (x, y, z) 1-based
+    rates: np.ndarray          # (npsrc, nst, 6) float64 [N m / s]
+
+    @property
+    def npsrc(self) -> int:
+        return int(self.cells.shape[0])
+
+    @property
+    def nst(self) -> int:
+        return int(self.rates.shape[1])
+
+    def psrc(self, dims: Dims) -> np.ndarray:
+        """tpsrc as pmcl3d stores it: (x, y, nzt+1-z) (source.c:179)."""
+        p = self.cells.astype(np.int32).copy()
+        p[:, 2] = dims.nzt + 1 - self.cells[:, 2]
+        return p
+
+    def columns(self) -> list[np.ndarray]:
+        """axx, ayy, azz, axz, ayz, axy as flat [j*READ_STEP + n] arrays."""
+        return [np.ascontiguousarray(self.rates[:, :, c]).reshape(-1) for c in range(6)]
+
+
+@dataclass
+class Receivers:
+    """Surface receivers: model cells (x, y) 1-based; sampled with the
+    velbuffer rule (u, v at the surface k, w one cell below)."""
+    xy: np.ndarray  # (n, 2)
+
+    def array_index(self, dims: Dims) -> np.ndarray:
+        n = self.xy.shape[0]
+        idx = np.empty((n, 3), dtype=np.int32)
+        idx[:, 0] = self.xy[:, 0] + 1 + 8
+        idx[:, 1] = self.xy[:, 1] + 1 + 8
+        idx[:, 2] = dims.ktop
+        return idx
+
+
+@dataclass
+class Workload:
+    name: str
+    dims: Dims
+    DH: float
+    DT: float
+    nt: int
+    medium_fn: object = None
+    sources: PointSources | None = None
+    receivers: Receivers | None = None
+    params: dict = field(default_factory=dict)
+
+    def medium(self) -> Medium:
+        return self.medium_fn()
+
+
+def seeded_surface_receivers(dims: Dims, n: int, seed: int, margin: int) -> Receivers:
+    rng = np.random.default_rng(seed)
+    lo, hx, hy = 1 + margin, dims.nxt - margin, dims.nyt - margin
+    if hx < lo or hy < lo:
+        lo, hx, hy = 1, dims.nxt, dims.nyt
+    xs = rng.integers(lo, hx + 1, size=n)
+    ys = rng.integers(lo, hy + 1, size=n)
+    return Receivers(np.stack([xs, ys], axis=1))
+
+
+def elastic_halfspace(n: int = 64, nt: int = 200, seed: int = 1234, nrec: int = 8,
+                      sponge: bool = True) -> Workload:
+    dims = Dims(n, n, n)
+    DH, DT = 100.0, 0.005
+    vp, vs, rho = 6000.0, 3464.0, 2700.0
+    f0, M0 = 2.0, 1.0e15
+    t = np.arange(nt) * DT
+    rate = M0 * ricker(t, f0)
+    rates = np.zeros((1, nt, 6))
+    rates[0, :, 0] = rates[0, :, 1] = rates[0, :, 2] = rate
+    c = n // 2
+    src = PointSources(np.array([[c, c, c]]), rates)
+    rec = seeded_surface_receivers(dims, nrec, seed, margin=min(20, n // 4))
+
+    def mk():
+        return build_medium(dims, DH, DT, vp, vs, rho, Qp=0.0, Qs=0.0, sponge=sponge)
+
+    return Workload("elastic_halfspace_%d" % n, dims, DH, DT, nt, mk, src, rec,
+                    dict(vp=vp, vs=vs, rho=rho, f0=f0, M0=M0))
+
+
+def layered_vs(depth: np.ndarray, zmax: float, nlayers: int = 6, vmin: float = 1500.0,
+               vmax: float = 4500.0) -> np.ndarray:
+    """Piecewise-constant vs rising from vmin at the surface to vmax in
+    nlayers equal-thickness layers."""
+    layer = np.minimum((depth / zmax * nlayers).astype(int), nlayers - 1)
+    return vmin + (vmax - vmin) * layer / (nlayers - 1)
+
+
+def von_karman(shape: tuple[int, int, int], h: float, a: float, nu: float, std: float,
+               seed: int) -> np.ndarray:
+    """Seeded 3-D von Karman random field (PSD ~ (1 + k^2 a^2)^-(nu+3/2)) with
+    the given standard deviation, float32."""
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal(shape, dtype=np.float32)
+    F = np.fft.rfftn(white)
+    kx = np.fft.fftfreq(shape[0], d=h) * 2 * np.pi
+    ky = np.fft.fftfreq(shape[1], d=h) * 2 * np.pi
+    kz = np.fft.rfftfreq(shape[2], d=h) * 2 * np.pi
+    k2 = (kx[:, None, None] ** 2 + ky[None, :, None] ** 2 + kz[None, None, :] ** 2)
+    amp = (1.0 + k2 * a * a) ** (-(nu + 1.5) / 2.0)
+    F *= amp.astype(np.float32)
+    del amp, k2
+    f = np.fft.irfftn(F, s=shape).astype(np.float32)
+    f -= f.mean()
+    f *= std / f.std()
+    return f
+
+
+def layered_random_model(dims: Dims, DH: float, seed: int = 7, std: float = 0.05,
+                         corr: float = 500.0, nu: float = 0.5):
+    """vp, vs, rho, Qp, Qs of the layered + von Karman medium (model order x, y,
+    z-down).  Returns float64 arrays."""
+    z = (np.arange(dims.nzt) + 0.5) * DH
+    vs1d = layered_vs(z, dims.nzt * DH)
+    pert = von_karman((dims.nxt, dims.nyt, dims.nzt), DH, corr, nu, std, seed)
+    vs = vs1d[None, None, :] * (1.0 + pert.astype(np.float64))
+    del pert
+    vp = 1.73 * vs
+    rho = 310.0 * vp ** 0.25
+    Qs = 50.0 * vs / 1000.0
+    Qp = 2.0 * Qs
+    return vp, vs, rho, Qp, Qs
+
+
+def anelastic(nx: int = 512, ny: int = 512, nz: int = 256, nt: int = 1000, DH: float = 50.0,
+              DT: float = 0.0025, seed: int = 7, nrec: int = 8) -> Workload:
+    dims = Dims(nx, ny, nz)
+    f0, M0 = 1.0, 1.0e15
+    t = np.arange(nt) * DT
+    rates = np.zeros((1, nt, 6))
+    rates[0, :, 5] = M0 * ricker(t, f0)  # xy: vertical strike-slip double couple
+    src = PointSources(np.array([[nx // 2, ny // 2, nz // 2]]), rates)
+    rec = seeded_surface_receivers(dims, nrec, seed + 1, margin=min(20, nx // 4))
+
+    def mk():
+        vp, vs, rho, Qp, Qs = layered_random_model(dims, DH, seed)
+        return build_medium(dims, DH, DT, vp, vs, rho, Qp, Qs)
+
+    return Workload("anelastic_%dx%dx%d" % (nx, ny, nz), dims, DH, DT, nt, mk, src, rec,
+                    dict(f0=f0, M0=M0, seed=seed))

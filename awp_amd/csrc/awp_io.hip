@@ -1,0 +1,209 @@
+// Point-source injection (reference addsrc_cu, src/awp/kernel.cu:1780-1806),
+// sub-volume output sampling (velbuffer, :3927-4011), receiver gathers and
+// the launch-shape knobs.
+#include <string.h>
+
+#include "awp_common.h"
+#include "awp_tuning.h"
+#include "../../include/awp_hip.h"
+
+static AwpTuning g_tuning = {4, 4, 2048, 0, 0, 1};
+AwpTuning& awp_tuning() { return g_tuning; }
+
+namespace {
+
+struct SrcLaunch {
+  int sample, read_step, dim, npsrc, yline;
+  long long slice;
+  float vtst;
+};
+
+// One lane per source, as the reference.
+__global__ __launch_bounds__(256) void addsrc_kernel(const int* __restrict__ psrc,
+                                                     const float* __restrict__ axx,
+                                                     const float* __restrict__ ayy,
+                                                     const float* __restrict__ azz,
+                                                     const float* __restrict__ axz,
+                                                     const float* __restrict__ ayz,
+                                                     const float* __restrict__ axy, float* xx,
+                                                     float* yy, float* zz, float* xy, float* yz,
+                                                     float* xz, const SrcLaunch A) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= A.npsrc) return;
+  const int idx = psrc[j * A.dim] + 1 + AWP_NGSL;
+  const int idy = psrc[j * A.dim + 1] + 1 + AWP_NGSL;
+  const int idz = psrc[j * A.dim + 2] + AWP_ALIGN - 1;
+  const long long p = (long long)idx * A.slice + (long long)idy * A.yline + idz;
+  const long long s = (long long)j * A.read_step + A.sample;
+  const float v = A.vtst;
+  xx[p] = xx[p] - v * axx[s];
+  yy[p] = yy[p] - v * ayy[s];
+  zz[p] = zz[p] - v * azz[s];
+  xz[p] = xz[p] - v * axz[s];
+  yz[p] = yz[p] - v * ayz[s];
+  xy[p] = xy[p] - v * axy[s];
+}
+
+struct VbLaunch {
+  int NVE, nbgx, nedx, nskpx, nbgy, nedy, nskpy, nbgz, nedz, nskpz, rec_nxt, rec_nyt, rec_nzt, follow;
+  int d_i, nzt, yline, bathy_ld;
+  long long slice;
+};
+
+__global__ __launch_bounds__(256) void velbuffer_kernel(
+    const float* __restrict__ u1, const float* __restrict__ v1, const float* __restrict__ w1,
+    const float* __restrict__ neta, float* __restrict__ Bufx, float* __restrict__ Bufy,
+    float* __restrict__ Bufz, float* __restrict__ Bufeta, const int* __restrict__ bathy,
+    const VbLaunch V) {
+  const int ti = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int tj = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int tk = (int)(blockIdx.z * blockDim.z + threadIdx.z);
+  if (ti >= V.rec_nxt || tj >= V.rec_nyt || tk >= V.rec_nzt) return;
+  const int i = 2 + AWP_NGSL + V.nbgx + ti * V.nskpx;
+  const int j = 2 + AWP_NGSL + V.nbgy + tj * V.nskpy;
+  const int k = V.nbgz + tk * V.nskpz;
+  if (i > 2 + AWP_NGSL + V.nedx || j > 2 + AWP_NGSL + V.nedy || k > V.nedz) return;
+  int ko, koz;
+  if (V.d_i > 0) {
+    ko = V.nzt + AWP_ALIGN - 1 - k;
+    koz = ko;
+  } else if (V.follow == 0) {
+    ko = V.nzt + AWP_ALIGN - 1 - k;
+    koz = ko - 1;
+  } else {
+    const int b = bathy[(long long)j * V.bathy_ld + i];
+    ko = b - k;
+    koz = (b == V.nzt + AWP_ALIGN - 1) ? ko - 1 : ko;
+  }
+  const long long base = (long long)i * V.slice + (long long)j * V.yline;
+  const long long t = (long long)((k - V.nbgz) / V.nskpz) * V.rec_nxt * V.rec_nyt +
+                      (long long)((j - 2 - AWP_NGSL - V.nbgy) / V.nskpy) * V.rec_nxt +
+                      (i - 2 - AWP_NGSL - V.nbgx) / V.nskpx;
+  Bufx[t] = u1[base + ko];
+  Bufy[t] = v1[base + ko];
+  Bufz[t] = w1[base + koz];
+  if (V.NVE == 3) Bufeta[t] = neta[base + ko];
+}
+
+struct RecLaunch {
+  int npts, slot, zshift, yline;
+  long long slice;
+};
+
+__global__ __launch_bounds__(256) void receivers_kernel(const float* __restrict__ u1,
+                                                        const float* __restrict__ v1,
+                                                        const float* __restrict__ w1,
+                                                        const int* __restrict__ idx,
+                                                        float* __restrict__ out,
+                                                        const RecLaunch R) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= R.npts) return;
+  const long long p =
+      (long long)idx[3 * n] * R.slice + (long long)idx[3 * n + 1] * R.yline + idx[3 * n + 2];
+  float* o = out + (long long)R.slot * 3 * R.npts + 3 * n;
+  o[0] = u1[p];
+  o[1] = v1[p];
+  o[2] = w1[p - R.zshift];
+}
+
+}  // namespace
+
+extern "C" {
+
+// Launch-shape knobs (no reference counterpart; the reference fixes its
+// block shapes at compile time, include/awp/pmcl3d_cons.h:2-4).
+int awp_set_tuning(const char* key, int value) {
+  AwpTuning& t = g_tuning;
+  if (!strcmp(key, "vel_bj") || !strcmp(key, "str_bj")) {
+    if (value != 1 && value != 2 && value != 4 && value != 8) return -1;
+    (!strcmp(key, "vel_bj") ? t.vel_bj : t.str_bj) = value;
+  } else if (!strcmp(key, "target_wgs")) {
+    if (value < 1) return -1;
+    t.target_wgs = value;
+  } else if (!strcmp(key, "vel_ichunk")) {
+    t.vel_ichunk = value < 0 ? 0 : value;
+  } else if (!strcmp(key, "str_ichunk")) {
+    t.str_ichunk = value < 0 ? 0 : value;
+  } else if (!strcmp(key, "xcd_remap")) {
+    t.xcd_remap = value ? 1 : 0;
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+void addsrc_H(int i, int READ_STEP, int dim, int* psrc, int npsrc, hipStream_t St,
+              float* axx, float* ayy, float* azz, float* axz, float* ayz, float* axy,
+              float* xx, float* yy, float* zz, float* xy, float* yz, float* xz, int d_i) {
+  const char* W = "addsrc_H";
+  if (!awp_grid_ok(d_i, W)) return;
+  if (npsrc <= 0) return;
+  if (i < 1 || i > READ_STEP || dim < 3) {
+    awp_set_error(AWP_ERR_ARG, "%s: sample %d outside [1, READ_STEP=%d] or dim=%d < 3", W, i,
+                  READ_STEP, dim);
+    return;
+  }
+  const AwpGrid& g = awp_state().grid[d_i];
+  SrcLaunch A;
+  A.sample = i - 1;
+  A.read_step = READ_STEP;
+  A.dim = dim;
+  A.npsrc = npsrc;
+  A.yline = g.yline;
+  A.slice = g.slice;
+  A.vtst = g.DT / (g.DH * g.DH * g.DH);  // kernel.cu:1790, (_prec)d_DT/(DH*DH*DH)
+  const int block = npsrc < 256 ? npsrc : 256;
+  hipLaunchKernelGGL(addsrc_kernel, dim3(awp_cdiv(npsrc, block)), dim3(block), 0, St, psrc, axx,
+                     ayy, azz, axz, ayz, axy, xx, yy, zz, xy, yz, xz, A);
+  awp_check_launch(W);
+}
+
+void velbuffer_H(const float* u1, const float* v1, const float* w1, const float* neta,
+                 float* Bufx, float* Bufy, float* Bufz, float* Bufeta, int NVE, int nbgx,
+                 int nedx, int nskpx, int nbgy, int nedy, int nskpy, int nbgz, int nedz,
+                 int nskpz, int rec_nxt, int rec_nyt, int rec_nzt, hipStream_t St,
+                 int FOLLOWBATHY, const int* bathy, int d_i) {
+  const char* W = "velbuffer_H";
+  if (!awp_grid_ok(d_i, W)) return;
+  const AwpGrid& g = awp_state().grid[d_i];
+  if (nskpx < 1 || nskpy < 1 || nskpz < 1 || rec_nxt < 1 || rec_nyt < 1 || rec_nzt < 1) return;
+  if (nbgx < 0 || nedx >= g.nxt + 2 + AWP_NGSL || nbgy < 0 || nedy >= g.nyt + 2 + AWP_NGSL ||
+      nbgz < 0 || (FOLLOWBATHY == 0 && nedz > g.nzt + AWP_ALIGN - 2)) {
+    awp_set_error(AWP_ERR_ARG, "%s: sampling window leaves the padded grid", W);
+    return;
+  }
+  VbLaunch V;
+  V.NVE = NVE; V.nbgx = nbgx; V.nedx = nedx; V.nskpx = nskpx;
+  V.nbgy = nbgy; V.nedy = nedy; V.nskpy = nskpy;
+  V.nbgz = nbgz; V.nedz = nedz; V.nskpz = nskpz;
+  V.rec_nxt = rec_nxt; V.rec_nyt = rec_nyt; V.rec_nzt = rec_nzt; V.follow = FOLLOWBATHY; V.d_i = d_i;
+  V.nzt = g.nzt; V.yline = g.yline; V.slice = g.slice; V.bathy_ld = g.nxt + 4 + AWP_NGSL2;
+  dim3 block(16, 4, 4);
+  dim3 grid(awp_cdiv(rec_nxt, 16), awp_cdiv(rec_nyt, 4), awp_cdiv(rec_nzt, 4));
+  hipLaunchKernelGGL(velbuffer_kernel, grid, block, 0, St, u1, v1, w1, neta, Bufx, Bufy, Bufz,
+                     Bufeta, bathy, V);
+  awp_check_launch(W);
+}
+
+void awp_receivers_gather(const float* u1, const float* v1, const float* w1, const int* idx,
+                          int npts, float* out, int step_slot, hipStream_t St, int d_i) {
+  const char* W = "awp_receivers_gather";
+  if (!awp_grid_ok(d_i, W)) return;
+  if (npts <= 0) return;
+  if (step_slot < 0) {
+    awp_set_error(AWP_ERR_ARG, "%s: negative step_slot %d", W, step_slot);
+    return;
+  }
+  const AwpGrid& g = awp_state().grid[d_i];
+  RecLaunch R;
+  R.npts = npts;
+  R.slot = step_slot;
+  R.zshift = (d_i == 0) ? 1 : 0;
+  R.yline = g.yline;
+  R.slice = g.slice;
+  hipLaunchKernelGGL(receivers_kernel, dim3(awp_cdiv(npts, 256)), dim3(256), 0, St, u1, v1, w1,
+                     idx, out, R);
+  awp_check_launch(W);
+}
+
+}  // extern "C"

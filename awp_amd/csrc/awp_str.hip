@@ -1,0 +1,443 @@
+// Stress + anelastic memory-variable update (reference dstrqc,
+// src/awp/kernel.cu:880-1197, and dstrqc_new :343-859), free-surface
+// velocity ghosts (:1087-1112) and stress imaging fstr (:1616-1647), gfx950.
+//
+// Mapping as in the velocity kernel: wave64 along k, BJ waves along j, an
+// i-march with register pipelines.  All four 8-cell material means
+// (lam, mu, qp, qs) keep their plane-(i+1) corners in registers, so every
+// per-cell array is fetched once per march; u/v/w keep a 4-plane x window.
+// The 16 Q coefficients sit in LDS and are indexed per lane by ww.
+//
+// Operation order and literal types follow the reference (double-literal
+// sub-expressions are evaluated in double), built with -ffp-contract=off.
+#include "awp_common.h"
+#include "awp_tuning.h"
+#include "../../include/awp_hip.h"
+
+namespace {
+
+struct StrLaunch {
+  int k0, k1, j0, j1, i0;
+  int nkt, njt, ichunk, ni, nblocks, remap;
+  int yline;
+  long long slice;
+  float c1, c2, dth, dh1, DT;
+};
+
+// Production Q weight (kernel.cu:978-1045).
+__device__ __forceinline__ float qw_prod(float q, float ca, float cb, float wwo) {
+  float w;
+  const double q2 = (double)q * 2.0;
+  if (1. / q2 <= 200.0)
+    w = (float)((double)ca * (2. * (double)q) * (2. * (double)q) + (double)cb * (2. * (double)q));
+  else
+    w = 2.0f * wwo * q;
+  return w / wwo;
+}
+
+// dstrqc_new's Q weights (kernel.cu:505-604): float test; qpa keeps a double
+// quadratic, the others are float.
+__device__ __forceinline__ float qw_new(float q, float ca, float cb, float wwo, bool dbl) {
+  float w;
+  if (1.0f / (q * 2.0f) <= 200.0f) {
+    if (dbl)
+      w = (float)((double)ca * (2. * (double)q) * (2. * (double)q) + (double)cb * (2. * (double)q));
+    else
+      w = ca * (2.0f * q) * (2.0f * q) + cb * (2.0f * q);
+  } else {
+    w = 2.0f * wwo * q;
+  }
+  return w / wwo;
+}
+
+template <int BJ, bool NEWLIT>
+__global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
+    float* __restrict__ xx, float* __restrict__ yy, float* __restrict__ zz,
+    float* __restrict__ xy, float* __restrict__ xz, float* __restrict__ yz,
+    float* __restrict__ r1, float* __restrict__ r2, float* __restrict__ r3,
+    float* __restrict__ r4, float* __restrict__ r5, float* __restrict__ r6,
+    const float* __restrict__ u1, const float* __restrict__ v1, const float* __restrict__ w1,
+    const float* __restrict__ lam, const float* __restrict__ mu, const float* __restrict__ qp,
+    const float* __restrict__ coeff, const float* __restrict__ qs,
+    const float* __restrict__ dcrjx, const float* __restrict__ dcrjy,
+    const float* __restrict__ dcrjz, const float* __restrict__ vx1,
+    const float* __restrict__ vx2, const int* __restrict__ ww, const float* __restrict__ wwo,
+    const StrLaunch L) {
+  __shared__ float s_coeff[16];
+  const int tid = threadIdx.y * 64 + threadIdx.x;
+  if (tid < 16) s_coeff[tid] = coeff[tid];
+  __syncthreads();
+
+  const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
+  const int kt = b % L.nkt;
+  const int jt = (b / L.nkt) % L.njt;
+  const int ic = b / (L.nkt * L.njt);
+  const int k = L.k0 + kt * 64 + (int)threadIdx.x;
+  const int j = L.j0 + jt * BJ + (int)threadIdx.y;
+  if (k > L.k1 || j > L.j1) return;
+  const int ia = L.i0 + ic * L.ichunk;
+  int nsteps = L.ni - ic * L.ichunk;
+  if (nsteps > L.ichunk) nsteps = L.ichunk;
+
+  const int Y = L.yline;
+  const long long S = L.slice;
+  const int o = j * Y + k;
+  const float c1 = L.c1, c2 = L.c2, dth = L.dth, dh1 = L.dh1, DT = L.DT;
+  const float dy = dcrjy[j];
+  const float dz = dcrjz[k];
+
+  const long long p0 = (long long)ia * S + o;
+  // x windows: u needs i-1..i+2, v and w need i-2..i+1
+  float u_m1 = u1[p0 - S], u_0 = u1[p0], u_p1 = u1[p0 + S];
+  float v_m2 = v1[p0 - 2 * S], v_m1 = v1[p0 - S], v_0 = v1[p0];
+  float w_m2 = w1[p0 - 2 * S], w_m1 = w1[p0 - S], w_0 = w1[p0];
+  // material corners at plane i: (j,k), (j-1,k), (j,k-1), (j-1,k-1)
+  float lc = lam[p0], ljm = lam[p0 - Y], lkm = lam[p0 - 1], ljk = lam[p0 - Y - 1];
+  float mc = mu[p0], mjm = mu[p0 - Y], mkm = mu[p0 - 1], mjk = mu[p0 - Y - 1];
+  float pc = qp[p0], pjm = qp[p0 - Y], pkm = qp[p0 - 1], pjk = qp[p0 - Y - 1];
+  float sc = qs[p0], sjm = qs[p0 - Y], skm = qs[p0 - 1], sjk = qs[p0 - Y - 1];
+
+  for (int s = 0; s < nsteps; ++s) {
+    const int i = ia + s;
+    const long long pl = (long long)i * S;
+    const float* up = u1 + pl;
+    const float* vp = v1 + pl;
+    const float* wp = w1 + pl;
+    const long long pn = pl + S;  // plane i+1
+
+    // leads
+    const float u_p2 = up[2 * S + o];
+    const float v_p1 = vp[S + o];
+    const float w_p1 = wp[S + o];
+    const float Lc = lam[pn + o], Ljm = lam[pn + o - Y], Lkm = lam[pn + o - 1], Ljk = lam[pn + o - Y - 1];
+    const float Mc = mu[pn + o], Mjm = mu[pn + o - Y], Mkm = mu[pn + o - 1], Mjk = mu[pn + o - Y - 1];
+    const float Pc = qp[pn + o], Pjm = qp[pn + o - Y], Pkm = qp[pn + o - 1], Pjk = qp[pn + o - Y - 1];
+    const float Sc = qs[pn + o], Sjm = qs[pn + o - Y], Skm = qs[pn + o - 1], Sjk = qs[pn + o - Y - 1];
+
+    // in-plane stencils
+    const float u_jp1 = up[o + Y], u_jp2 = up[o + 2 * Y], u_jm1 = up[o - Y];
+    const float u_kp1 = up[o + 1], u_kp2 = up[o + 2], u_km1 = up[o - 1];
+    const float v_jm1 = vp[o - Y], v_jp1 = vp[o + Y], v_jm2 = vp[o - 2 * Y];
+    const float v_kp1 = vp[o + 1], v_kp2 = vp[o + 2], v_km1 = vp[o - 1];
+    const float w_km1 = wp[o - 1], w_kp1 = wp[o + 1], w_km2 = wp[o - 2];
+    const float w_jp1 = wp[o + Y], w_jp2 = wp[o + 2 * Y], w_jm1 = wp[o - Y];
+
+    const long long p = pl + o;
+    const float f_vx1 = vx1[p], f_vx2 = vx2[p], f_wwo = wwo[p];
+    const int f_ww = ww[p];
+    const float ca = s_coeff[(f_ww * 2 - 2) & 15], cb = s_coeff[(f_ww * 2 - 1) & 15];
+    const float f_dcrj = dcrjx[i] * dy * dz;
+
+    const float sl8 = lc + Lc + ljm + Ljm + lkm + Lkm + ljk + Ljk;
+    const float sm8 = mc + Mc + mjm + Mjm + mkm + Mkm + mjk + Mjk;
+    const float sp8 = pc + Pc + pjm + Pjm + pkm + Pkm + pjk + Pjk;
+    const float ss8 = sc + Sc + sjm + Sjm + skm + Skm + sjk + Sjk;
+    float xl, xm, xmu1, xmu2, xmu3, qpa, h, h1, h2, h3, qpaw, hw, h1w, h2w, h3w;
+    if (!NEWLIT) {
+      xl = (float)(8.0 / (double)sl8);
+      xm = (float)(16.0 / (double)sm8);
+      xmu1 = (float)(2.0 / (double)(mc + mkm));
+      xmu2 = (float)(2.0 / (double)(mc + mjm));
+      xmu3 = (float)(2.0 / (double)(mc + Mc));
+      xl = xl + xm;
+      qpa = 0.0625f * sp8;
+      qpaw = qw_prod(qpa, ca, cb, f_wwo);
+      h = 0.0625f * ss8;
+      hw = qw_prod(h, ca, cb, f_wwo);
+      h1 = 0.250f * (sc + skm);
+      h1w = qw_prod(h1, ca, cb, f_wwo);
+      h2 = 0.250f * (sc + sjm);
+      h2w = qw_prod(h2, ca, cb, f_wwo);
+      h3 = 0.250f * (sc + Sc);
+      h3w = qw_prod(h3, ca, cb, f_wwo);
+    } else {
+      xl = 8.0f / sl8;
+      xm = 16.0f / sm8;
+      xmu1 = 2.0f / (mc + mkm);
+      xmu2 = (float)(2.0 / (double)(mc + mjm));
+      xmu3 = (float)(2.0 / (double)(mc + Mc));
+      xl = xl + xm;
+      qpa = 0.0625f * sp8;
+      qpaw = qw_new(qpa, ca, cb, f_wwo, true);
+      h = 0.0625f * ss8;
+      hw = qw_new(h, ca, cb, f_wwo, false);
+      h1 = 0.250f * (sc + skm);
+      h1w = qw_new(h1, ca, cb, f_wwo, false);
+      h2 = 0.250f * (sc + sjm);
+      h2w = qw_new(h2, ca, cb, f_wwo, false);
+      h3 = 0.250f * (sc + Sc);
+      h3w = qw_new(h3, ca, cb, f_wwo, false);
+    }
+    h = -xm * hw * dh1;
+    h1 = -xmu1 * h1w * dh1;
+    h2 = -xmu2 * h2w * dh1;
+    h3 = -xmu3 * h3w * dh1;
+    qpa = -qpaw * xl * dh1;
+    xm = xm * dth;
+    xmu1 = xmu1 * dth;
+    xmu2 = xmu2 * dth;
+    xmu3 = xmu3 * dth;
+    xl = xl * dth;
+    h = h * f_vx1;
+    h1 = h1 * f_vx1;
+    h2 = h2 * f_vx1;
+    h3 = h3 * f_vx1;
+    qpa = qpa * f_vx1;
+    xm = xm + DT * h;
+    xmu1 = xmu1 + DT * h1;
+    xmu2 = xmu2 + DT * h2;
+    xmu3 = xmu3 + DT * h3;
+    const float vx1p = DT * (1 + f_vx2 * f_vx1);
+
+    float vs1 = c1 * (u_p1 - u_0) + c2 * (u_p2 - u_m1);
+    float vs2 = c1 * (v_0 - v_jm1) + c2 * (v_jp1 - v_jm2);
+    float vs3 = c1 * (w_0 - w_km1) + c2 * (w_kp1 - w_km2);
+    float tmp = xl * (vs1 + vs2 + vs3);
+    const float a1 = qpa * (vs1 + vs2 + vs3);
+    tmp = tmp + DT * a1;
+
+    float f_r, f_rtmp, t;
+    const float one_m_vx1 = 1 - f_vx1;
+    const float wwo_m1 = f_wwo - 1;
+
+    f_r = r1[p];
+    f_rtmp = -h * (vs2 + vs3) + a1;
+    t = xx[p] + tmp - xm * (vs2 + vs3) + vx1p * f_r;
+    r1[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
+    xx[p] = (t + DT * f_rtmp) * f_dcrj;
+
+    f_r = r2[p];
+    f_rtmp = -h * (vs1 + vs3) + a1;
+    t = (yy[p] + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
+    r2[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
+    yy[p] = (t + DT * f_rtmp) * f_dcrj;
+
+    f_r = r3[p];
+    f_rtmp = -h * (vs1 + vs2) + a1;
+    t = (zz[p] + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
+    r3[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
+    zz[p] = (t + DT * f_rtmp) * f_dcrj;
+
+    vs1 = c1 * (u_jp1 - u_0) + c2 * (u_jp2 - u_jm1);
+    vs2 = c1 * (v_0 - v_m1) + c2 * (v_p1 - v_m2);
+    f_r = r4[p];
+    f_rtmp = h1 * (vs1 + vs2);
+    t = xy[p] + xmu1 * (vs1 + vs2) + vx1p * f_r;
+    r4[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
+    xy[p] = (t + DT * f_rtmp) * f_dcrj;
+
+    vs1 = c1 * (u_kp1 - u_0) + c2 * (u_kp2 - u_km1);
+    vs2 = c1 * (w_0 - w_m1) + c2 * (w_p1 - w_m2);
+    f_r = r5[p];
+    f_rtmp = h2 * (vs1 + vs2);
+    t = xz[p] + xmu2 * (vs1 + vs2) + vx1p * f_r;
+    r5[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
+    xz[p] = (t + DT * f_rtmp) * f_dcrj;
+
+    vs1 = c1 * (v_kp1 - v_0) + c2 * (v_kp2 - v_km1);
+    vs2 = c1 * (w_jp1 - w_0) + c2 * (w_jp2 - w_jm1);
+    f_r = r6[p];
+    f_rtmp = h3 * (vs1 + vs2);
+    t = yz[p] + xmu3 * (vs1 + vs2) + vx1p * f_r;
+    r6[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
+    yz[p] = (t + DT * f_rtmp) * f_dcrj;
+
+    // advance x windows and material corners
+    u_m1 = u_0; u_0 = u_p1; u_p1 = u_p2;
+    v_m2 = v_m1; v_m1 = v_0; v_0 = v_p1;
+    w_m2 = w_m1; w_m1 = w_0; w_0 = w_p1;
+    lc = Lc; ljm = Ljm; lkm = Lkm; ljk = Ljk;
+    mc = Mc; mjm = Mjm; mkm = Mkm; mjk = Mjk;
+    pc = Pc; pjm = Pjm; pkm = Pkm; pjk = Pjk;
+    sc = Sc; sjm = Sjm; skm = Skm; sjk = Sjk;
+  }
+}
+
+// Free-surface velocity ghosts at k = top+1 (kernel.cu:1087-1112 for dstrqc,
+// 660-687 for dstrqc_new).  In the reference the k=top lane of the stress
+// kernel writes these and reads them back itself, and the k=top-1 lane
+// rewrites u1, v1 at top+1 with the same formula; every value depends only
+// on interior (k <= top) velocities, so computing them in a pass of their
+// own before the stress sweep leaves the same memory state and the same
+// stresses.  One lane per (i, j) column, lanes along j.
+struct FsLaunch {
+  int i0, i1, j0, j1, ktop, yline, lam_mu_ld, nxt, nyt, NX, NPC, rankx, ranky;
+  long long slice;
+};
+
+template <bool NEWLIT>
+__global__ __launch_bounds__(256) void fs_ghost_kernel(float* __restrict__ u1,
+                                                       float* __restrict__ v1,
+                                                       float* __restrict__ w1,
+                                                       const float* __restrict__ lam_mu,
+                                                       const FsLaunch F) {
+  const int j = F.j0 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int i = F.i0 + (int)(blockIdx.y * 4 + threadIdx.y);
+  if (j > F.j1 || i > F.i1) return;
+  const long long S = F.slice;
+  const int Y = F.yline;
+  const long long p = (long long)i * S + (long long)j * Y + F.ktop;
+  const float uc = u1[p], vc = v1[p], wc = w1[p];
+  const float ug = uc - (wc - w1[p - S]);
+  const float vg = vc - (w1[p + Y] - wc);
+  u1[p + 1] = ug;
+  v1[p + 1] = vg;
+  float vs1, vs2;
+  int gi = F.nxt * F.rankx + i - AWP_NGSL - 1;
+  if (gi < F.NX || (NEWLIT && F.NPC == 2))
+    vs1 = u1[p + S] - (w1[p + S] - wc);
+  else
+    vs1 = 0.0f;
+  gi = F.nyt * F.ranky + j - AWP_NGSL - 1;
+  if (gi > 1 || F.NPC == 2)
+    vs2 = v1[p - Y] - (wc - w1[p - Y]);
+  else
+    vs2 = 0.0f;
+  w1[p + 1] = w1[p - 1] - lam_mu[(long long)i * F.lam_mu_ld + j] *
+                              ((vs1 - ug) + (u1[p + S] - uc) + (vg - vs2) + (vc - v1[p - Y]));
+}
+
+template <bool NEWLIT>
+void dstrqc_common(const char* W, float* xx, float* yy, float* zz, float* xy, float* xz,
+                   float* yz, float* r1, float* r2, float* r3, float* r4, float* r5, float* r6,
+                   float* u1, float* v1, float* w1, float* lam, float* mu, float* qp,
+                   float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
+                   int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
+                   float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i, int s_j,
+                   int e_j, int d_i) {
+  if (!awp_grid_ok(d_i, W)) return;
+  const AwpGrid& g = awp_state().grid[d_i];
+  if (nyt != g.nyt || nzt != g.nzt) {
+    awp_set_error(AWP_ERR_ARG, "%s: nyt=%d nzt=%d disagree with grid %d (%d, %d)", W, nyt, nzt,
+                  d_i, g.nyt, g.nzt);
+    return;
+  }
+  if (s_i > e_i || s_j > e_j) return;
+  if (s_i < 2 || e_i + 2 > g.nxt + 3 + AWP_NGSL2 || s_j < 2 || e_j + 2 > g.nyt + 3 + AWP_NGSL2) {
+    awp_set_error(AWP_ERR_ARG, "%s: range i[%d,%d] j[%d,%d] leaves the padded grid (%d x %d)", W,
+                  s_i, e_i, s_j, e_j, g.nxt, g.nyt);
+    return;
+  }
+  if (d_i == 0) {
+    FsLaunch F;
+    F.i0 = s_i; F.i1 = e_i; F.j0 = s_j; F.j1 = e_j;
+    F.ktop = nzt + AWP_ALIGN - 1;
+    F.yline = g.yline; F.slice = g.slice;
+    F.lam_mu_ld = nyt + 4 + AWP_NGSL2;
+    F.nxt = g.nxt; F.nyt = g.nyt; F.NX = NX; F.NPC = NPC; F.rankx = rankx; F.ranky = ranky;
+    dim3 grid(awp_cdiv(e_j - s_j + 1, 64), awp_cdiv(e_i - s_i + 1, 4));
+    hipLaunchKernelGGL(fs_ghost_kernel<NEWLIT>, grid, dim3(64, 4), 0, St, u1, v1, w1, lam_mu, F);
+    if (!awp_check_launch(W)) return;
+  }
+  StrLaunch L;
+  L.k0 = AWP_ALIGN + 3;  // kernel.cu:901 (mink)
+  L.k1 = (d_i == 0) ? nzt + AWP_ALIGN - 1 : nzt + AWP_ALIGN - 3;
+  L.j0 = s_j;
+  L.j1 = e_j;
+  L.i0 = s_i;
+  L.ni = e_i - s_i + 1;
+  L.yline = g.yline;
+  L.slice = g.slice;
+  L.c1 = g.c1; L.c2 = g.c2; L.dth = g.dth; L.dh1 = g.dh1; L.DT = g.DT;
+  L.remap = awp_tuning().xcd_remap;
+  if (L.k1 < L.k0) return;
+  const int bj = awp_tuning().str_bj;
+  L.nkt = awp_cdiv(L.k1 - L.k0 + 1, 64);
+  L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
+  int nic, ichunk;
+  awp_split_i(L.ni, L.nkt * L.njt, awp_tuning().str_ichunk, awp_tuning().target_wgs, &nic, &ichunk);
+  L.ichunk = ichunk;
+  L.nblocks = L.nkt * L.njt * nic;
+#define AWP_STR_ARGS                                                                          \
+  xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, coeff, qs, dcrjx, \
+      dcrjy, dcrjz, vx1, vx2, ww, wwo, L
+  switch (bj) {
+    case 1: hipLaunchKernelGGL((dstrqc_kernel<1, NEWLIT>), dim3(L.nblocks), dim3(64, 1), 0, St, AWP_STR_ARGS); break;
+    case 2: hipLaunchKernelGGL((dstrqc_kernel<2, NEWLIT>), dim3(L.nblocks), dim3(64, 2), 0, St, AWP_STR_ARGS); break;
+    case 8: hipLaunchKernelGGL((dstrqc_kernel<8, NEWLIT>), dim3(L.nblocks), dim3(64, 8), 0, St, AWP_STR_ARGS); break;
+    default: hipLaunchKernelGGL((dstrqc_kernel<4, NEWLIT>), dim3(L.nblocks), dim3(64, 4), 0, St, AWP_STR_ARGS); break;
+  }
+#undef AWP_STR_ARGS
+  awp_check_launch(W);
+}
+
+// fstr (kernel.cu:1616-1647): one lane per (i, j) column of grid 0.
+struct FstrLaunch {
+  int i0, i1, j0, j1, ktop, yline;
+  long long slice;
+};
+
+__global__ __launch_bounds__(256) void fstr_kernel(float* __restrict__ zz, float* __restrict__ xz,
+                                                   float* __restrict__ yz, const FstrLaunch F) {
+  const int j = F.j0 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int i = F.i0 + (int)(blockIdx.y * 4 + threadIdx.y);
+  if (j > F.j1 || i > F.i1) return;
+  const long long p = (long long)i * F.slice + (long long)j * F.yline + F.ktop;
+  const float zc = zz[p], zm1 = zz[p - 1];
+  const float xm1 = xz[p - 1], xm2 = xz[p - 2];
+  const float ym1 = yz[p - 1], ym2 = yz[p - 2];
+  zz[p + 1] = -zc;
+  zz[p + 2] = -zm1;
+  xz[p + 1] = -xm1;
+  xz[p + 2] = -xm2;
+  yz[p + 1] = -ym1;
+  yz[p + 2] = -ym2;
+  xz[p] = 0.0f;
+  yz[p] = 0.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+void dstrqc_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
+              float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
+              float* w1, float* lam, float* mu, float* qp, float* coeff, float* qs,
+              float* dcrjx, float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St,
+              float* lam_mu, float* vx1, float* vx2, int* ww, float* wwo, int NX, int NPC,
+              int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i) {
+  dstrqc_common<false>("dstrqc_H", xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1,
+                       lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt, St, lam_mu, vx1,
+                       vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j, d_i);
+}
+
+void dstrqc_H_new(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                  float* r1, float* r2, float* r3, float* r4, float* r5, float* r6,
+                  float* u1, float* v1, float* w1, float* lam, float* mu, float* qp,
+                  float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
+                  int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
+                  float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i,
+                  int s_j, int e_j, int d_i) {
+  dstrqc_common<true>("dstrqc_H_new", xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1,
+                      w1, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt, St, lam_mu,
+                      vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j, d_i);
+}
+
+void fstr_H(float* zz, float* xz, float* yz, hipStream_t St, int s_i, int e_i, int s_j,
+            int e_j) {
+  const char* W = "fstr_H";
+  if (!awp_grid_ok(0, W)) return;
+  const AwpGrid& g = awp_state().grid[0];
+  if (s_i > e_i || s_j > e_j) return;
+  if (s_i < 0 || e_i >= g.nxt + 4 + AWP_NGSL2 || s_j < 0 || e_j >= g.nyt + 4 + AWP_NGSL2) {
+    awp_set_error(AWP_ERR_ARG, "%s: range i[%d,%d] j[%d,%d] leaves the padded grid", W, s_i, e_i,
+                  s_j, e_j);
+    return;
+  }
+  FstrLaunch F;
+  F.i0 = s_i; F.i1 = e_i; F.j0 = s_j; F.j1 = e_j;
+  F.ktop = g.nzt + AWP_ALIGN - 1;
+  F.yline = g.yline;
+  F.slice = g.slice;
+  dim3 grid(awp_cdiv(e_j - s_j + 1, 64), awp_cdiv(e_i - s_i + 1, 4));
+  hipLaunchKernelGGL(fstr_kernel, grid, dim3(64, 4), 0, St, zz, xz, yz, F);
+  awp_check_launch(W);
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Launch-shape knobs shared by the launchers (set with awp_set_tuning).
+#pragma once
+
+struct AwpTuning {
+  int vel_bj;       // j rows per velocity workgroup (64 k-lanes each)
+  int str_bj;       // j rows per stress workgroup
+  int target_wgs;   // workgroups to aim for when splitting the i march
+  int vel_ichunk;   // >0 forces the velocity i-chunk length
+  int str_ichunk;   // >0 forces the stress i-chunk length
+  int xcd_remap;    // 1: XCD-aware tile order, 0: plain order
+};
+
+AwpTuning& awp_tuning();
+
+// Number of i-chunks and chunk length for a march over ni planes when the
+// (k, j) tiling gives nkj workgroups per chunk.
+static inline void awp_split_i(int ni, int nkj, int forced, int target, int* nic, int* ichunk) {
+  int c;
+  if (forced > 0) {
+    c = forced;
+  } else {
+    int want = (target + nkj - 1) / nkj;
+    if (want < 1) want = 1;
+    if (want > ni) want = ni;
+    c = (ni + want - 1) / want;
+  }
+  if (c < 1) c = 1;
+  *ichunk = c;
+  *nic = (ni + c - 1) / c;
+}

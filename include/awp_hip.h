@@ -1,0 +1,110 @@
+/* C ABI of the gfx950 (MI355X) AWP-ODC time-stepping library, libawp_hip.so.
+ *
+ * Entry points keep the names, argument order and meaning of the reference's
+ * kernel launchers in include/awp/kernel_launch.h (SCECcode/awp), with
+ * hipStream_t where the reference has cudaStream_t.  The caller allocates and
+ * owns every device array; launchers never allocate fields, mutate them in
+ * place and run asynchronously on the given stream.
+ *
+ * Layout of every 3-D field (reference src/awp/kernel.cu:87-94): one flat
+ * float array of (nxt+4+16) * (nyt+4+16) * (nzt+64) elements with k (z)
+ * contiguous, index i*slice + j*yline + k, yline = nzt+64,
+ * slice = (nyt+20)*yline.  k = nzt+31 is the free surface.
+ *
+ * Errors: the launchers return void like the reference's.  An invalid
+ * argument (index range outside the padded arrays, grid index without
+ * SetDeviceConstValue, ...) launches nothing, prints a message to stderr and
+ * records an error code readable with awp_last_error(); launch failures are
+ * recorded the same way.  The reference prints or exits instead
+ * (include/awp/pmcl3d_cons.h:43-53, src/awp/kernel.cu:1233-1235).
+ */
+#ifndef AWP_HIP_H
+#define AWP_HIP_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AWP_ERR_ARG 1   /* invalid argument: nothing was launched */
+#define AWP_ERR_STATE 2 /* SetDeviceConstValue missing or inconsistent */
+#define AWP_ERR_HIP 3   /* a HIP runtime call or launch failed */
+
+/* Error channel (no reference counterpart; the reference prints or exits). */
+int awp_last_error(void);
+const char* awp_last_error_string(void);
+void awp_clear_error(void);
+
+/* Replaces SetDeviceConstValue, include/awp/kernel_launch.h:24-26
+ * (src/awp/kernel.cu:68-117).  Stores per-grid DT/DH-derived constants and
+ * strides, plus the plasticity rotation (fmajor, fminor, Rz, RzT). */
+void SetDeviceConstValue(float* DH, float DT, int* nxt, int* nyt, int* nzt, int ngrids,
+                         float fmajor, float fminor, float* Rz, float* RzT);
+
+/* Replaces dvelcx_H_opt, include/awp/kernel_launch.h:31-34
+ * (src/awp/kernel.cu:298-316, kernel dvelcx_opt :125-268).  4th-order
+ * staggered velocity update of u1, v1, w1 for i in [s_i, e_i], all interior
+ * j, and k in [align, nzt+align-1] (narrowed by the DM guards for d_i). */
+void dvelcx_H_opt(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz,
+                  float* xy, float* xz, float* yz, float* dcrjx, float* dcrjy,
+                  float* dcrjz, float* d_1, int nyt, int nzt, hipStream_t St, int s_i,
+                  int e_i, int d_i, int ngrids);
+
+/* Replaces dstrqc_H, include/awp/kernel_launch.h:51-58 (src/awp/kernel.cu:
+ * 861-878, kernel dstrqc :880-1197).  Stress + coarse-grained Q memory
+ * variables for i in [s_i, e_i], j in [s_j, e_j], k in [align+3, top];
+ * on grid 0 it also writes the free-surface velocity ghosts at k = top+1. */
+void dstrqc_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
+              float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
+              float* w1, float* lam, float* mu, float* qp, float* coeff, float* qs,
+              float* dcrjx, float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St,
+              float* lam_mu, float* vx1, float* vx2, int* ww, float* wwo, int NX, int NPC,
+              int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i);
+
+/* Replaces dstrqc_H_new, include/awp/kernel_launch.h:42-49 (src/awp/kernel.cu:
+ * 1201-1239, kernel dstrqc_new :343-859): the same update with that kernel's
+ * float literals and its periodic (NPC==2) rule on the x edge. */
+void dstrqc_H_new(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                  float* r1, float* r2, float* r3, float* r4, float* r5, float* r6,
+                  float* u1, float* v1, float* w1, float* lam, float* mu, float* qp,
+                  float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
+                  int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
+                  float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i,
+                  int s_j, int e_j, int d_i);
+
+/* Replaces fstr_H, include/awp/kernel_launch.h:59-60 (src/awp/kernel.cu:
+ * 1243-1250, kernel fstr :1616-1647).  Antisymmetric stress imaging above the
+ * free surface of grid 0 and zero xz, yz on it. */
+void fstr_H(float* zz, float* xz, float* yz, hipStream_t St, int s_i, int e_i, int s_j,
+            int e_j);
+
+/* Replaces addsrc_H, include/awp/kernel_launch.h:73-76 (src/awp/kernel.cu:
+ * 1465-1491, kernel addsrc_cu :1780-1806).  sigma -= DT/DH^3 * a[j*READ_STEP
+ * + i-1] at each source cell (x+1+ngsl, y+1+ngsl, z+align-1).  Two sources on
+ * one cell race, as in the reference. */
+void addsrc_H(int i, int READ_STEP, int dim, int* psrc, int npsrc, hipStream_t St,
+              float* axx, float* ayy, float* azz, float* axz, float* ayz, float* axy,
+              float* xx, float* yy, float* zz, float* xy, float* yz, float* xz, int d_i);
+
+/* Replaces velbuffer_H, include/awp/kernel_launch.h:181-186
+ * (src/awp/kernel.cu:3927-4011).  Strided sub-volume sampling: u1, v1 at
+ * ko = nzt+align-1-k and w1 at ko-1 on grid 0 (FOLLOWBATHY = 0), ko on finer
+ * levels; neta when NVE == 3.  FOLLOWBATHY = 1 uses bathy[j*(nxt+20)+i]. */
+void velbuffer_H(const float* u1, const float* v1, const float* w1, const float* neta,
+                 float* Bufx, float* Bufy, float* Bufz, float* Bufeta, int NVE, int nbgx,
+                 int nedx, int nskpx, int nbgy, int nedy, int nskpy, int nbgz, int nedz,
+                 int nskpz, int rec_nxt, int rec_nyt, int rec_nzt, hipStream_t St,
+                 int FOLLOWBATHY, const int* bathy, int d_i);
+
+/* No reference counterpart: gathers u1, v1 at cell (i, j, k) and w1 at
+ * (i, j, k-1) on grid 0 (k-1 -> k on d_i > 0), velbuffer's sampling rule, for
+ * an arbitrary list of npts receiver cells idx[3*n + {0,1,2}] = (i, j, k) in
+ * array coordinates, writing out[step_slot*3*npts + 3*n + c]. */
+void awp_receivers_gather(const float* u1, const float* v1, const float* w1, const int* idx,
+                          int npts, float* out, int step_slot, hipStream_t St, int d_i);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AWP_HIP_H */

@@ -1,0 +1,436 @@
+// AWP-ODC time-stepping oracle: a CPU restatement of the reference kernels.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library in
+// awp_amd/csrc; only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it.  Nothing in the product path links it.
+//
+// Every function follows a reference kernel in operation order and literal
+// type, so that for T=float the arithmetic is the reference's single-precision
+// arithmetic (sub-expressions the reference writes with double literals are
+// evaluated in double here too), and for T=double it is the FP64 oracle.
+// Build with -ffp-contract=off: the reference's expression trees are restated
+// without fused multiply-adds so that results are reproducible bit for bit.
+//
+// Memory layout (reference src/awp/kernel.cu:87-94, src/awp/grid.c:14-38):
+//   one flat array per field, index p = i*slice + j*yline + k,
+//   slice = (nyt+4+2*ngsl)*(nzt+2*align), yline = nzt+2*align, k fastest.
+//   k = nzt+align-1 is the free surface; depth grows as k decreases.
+//
+// Pinning (what checks each function; see tests/test_oracle_*.py):
+//   dvelcx  : cubic-exactness of the 4th-order staggered derivative at the
+//             staggered points, constant-field zero update, 4-point density
+//             means against closed forms, sponge factor application.
+//   dstrqc  : elastic closed forms (harmonic 8-cell lambda/mu means, pair means)
+//             for Q^-1=0, Q-branch weights against a brute-force evaluation of
+//             the coarse-grained memory-variable recursion, plane-wave Q.
+//   fs/fstr : closed-form ghost values on hand-built columns; antisymmetry.
+//   addsrc  : closed form.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <omp.h>
+#include <xmmintrin.h>
+
+namespace {
+
+// The reference is built with -use_fast_math (CMakeLists.txt:9-14), which
+// flushes single-precision denormals to zero on input and output.  MXCSR
+// FTZ|DAZ does the same on the host; MXCSR is per thread, so every parallel
+// loop body sets it.
+inline void ftz_daz() { _mm_setcsr(_mm_getcsr() | 0x8040); }
+
+constexpr int NGSL = 8;    // include/awp/pmcl3d_cons.h:22
+constexpr int NGSL2 = 16;  // include/awp/pmcl3d_cons.h:23
+constexpr int ALIGN = 32;  // include/awp/pmcl3d_cons.h:16
+
+// Per-grid constants as SetDeviceConstValue derives them
+// (src/awp/kernel.cu:68-117): dth = DT/DH in _prec, dh1 = 1.0/DH in double
+// then stored as _prec, c1 = 9/8, c2 = -1/24.
+template <typename T>
+struct Grid {
+  T c1, c2, dth, dh1, DT, DH;
+  long slice, yline;
+  int nxt, nyt, nzt;
+};
+
+template <typename T>
+Grid<T> make_grid(double DH, double DT, int nxt, int nyt, int nzt) {
+  Grid<T> g;
+  const T dh = (T)DH, dt = (T)DT;
+  g.c1 = (T)(9.0 / 8.0);
+  g.c2 = (T)(-1.0 / 24.0);
+  g.dth = dt / dh;
+  g.dh1 = (T)(1.0 / (double)dh);
+  g.DT = dt;
+  g.DH = dh;
+  g.yline = nzt + 2 * ALIGN;
+  g.slice = (long)(nyt + 4 + NGSL2) * g.yline;
+  g.nxt = nxt;
+  g.nyt = nyt;
+  g.nzt = nzt;
+  return g;
+}
+
+// dvelcx_opt, src/awp/kernel.cu:125-268 (register pipelines unrolled into
+// direct indexing; offsets per the pipeline contents at iteration i).
+template <typename T>
+void dvelcx(T* u1, T* v1, T* w1, const T* xx, const T* yy, const T* zz,
+            const T* xy, const T* xz, const T* yz, const T* dcrjx,
+            const T* dcrjy, const T* dcrjz, const T* d1, const Grid<T>& g,
+            int s_i, int e_i, int d_i, int ngrids) {
+  const long sl = g.slice, yl = g.yline;
+  const int j0 = 2 + NGSL, j1 = g.nyt + NGSL + 1;
+  int k0 = ALIGN, k1 = g.nzt + ALIGN - 1;
+  if (d_i > 0) k1 = g.nzt + ALIGN - 3;             // kernel.cu:146
+  if (d_i < ngrids - 1) k0 = ALIGN + 3;            // kernel.cu:147
+#pragma omp parallel for schedule(static)
+  for (int i = s_i; i <= e_i; ++i) {
+    ftz_daz();
+    for (int j = j0; j <= j1; ++j) {
+      for (int k = k0; k <= k1; ++k) {
+        const long p = i * sl + j * yl + k;
+        const T dcrj = dcrjx[i] * dcrjy[j] * dcrjz[k];
+        // kernel.cu:214-220: 0.25 is a double literal (exact scaling).
+        T d_1 = (T)(0.25 * (double)(d1[p] + d1[p - yl] + d1[p - 1] + d1[p - yl - 1]));
+        T d_2 = (T)(0.25 * (double)(d1[p] + d1[p + sl] + d1[p - 1] + d1[p + sl - 1]));
+        T d_3 = (T)(0.25 * (double)(d1[p] + d1[p + sl] + d1[p - yl] + d1[p + sl - yl]));
+        d_1 = g.dth / d_1;
+        d_2 = g.dth / d_2;
+        d_3 = g.dth / d_3;
+        const T c1 = g.c1, c2 = g.c2;
+        u1[p] = (u1[p] + d_1 * (c1 * (xx[p] - xx[p - sl]) + c2 * (xx[p + sl] - xx[p - 2 * sl])
+                              + c1 * (xy[p] - xy[p - yl]) + c2 * (xy[p + yl] - xy[p - 2 * yl])
+                              + c1 * (xz[p] - xz[p - 1]) + c2 * (xz[p + 1] - xz[p - 2]))) * dcrj;
+        v1[p] = (v1[p] + d_2 * (c1 * (xy[p + sl] - xy[p]) + c2 * (xy[p + 2 * sl] - xy[p - sl])
+                              + c1 * (yy[p + yl] - yy[p]) + c2 * (yy[p + 2 * yl] - yy[p - yl])
+                              + c1 * (yz[p] - yz[p - 1]) + c2 * (yz[p + 1] - yz[p - 2]))) * dcrj;
+        w1[p] = (w1[p] + d_3 * (c1 * (xz[p + sl] - xz[p]) + c2 * (xz[p + 2 * sl] - xz[p - sl])
+                              + c1 * (yz[p] - yz[p - yl]) + c2 * (yz[p + yl] - yz[p - 2 * yl])
+                              + c1 * (zz[p + 1] - zz[p]) + c2 * (zz[p + 2] - zz[p - 1]))) * dcrj;
+      }
+    }
+  }
+}
+
+// Coarse-grained Q weight, production dstrqc (kernel.cu:978-1045): the branch
+// test and the quadratic are double expressions, the fallback is float.
+template <typename T>
+inline T qweight_prod(T q, const T* coeff, int ww, T wwo) {
+  T w;
+  if (1. / ((double)q * 2.0) <= 200.0)
+    w = (T)((double)coeff[ww * 2 - 2] * (2. * (double)q) * (2. * (double)q) +
+            (double)coeff[ww * 2 - 1] * (2. * (double)q));
+  else
+    w = (T)2.0f * wwo * q;
+  return w / wwo;
+}
+
+// dstrqc_new's variants (kernel.cu:505-604): float test; qpa's quadratic stays
+// double, the four shear-side quadratics are float.
+template <typename T>
+inline T qweight_new(T q, const T* coeff, int ww, T wwo, bool dbl_poly) {
+  T w;
+  if (1.0f / (q * 2.0f) <= 200.0f) {
+    if (dbl_poly)
+      w = (T)((double)coeff[ww * 2 - 2] * (2. * (double)q) * (2. * (double)q) +
+              (double)coeff[ww * 2 - 1] * (2. * (double)q));
+    else
+      w = coeff[ww * 2 - 2] * (2.0f * q) * (2.0f * q) + coeff[ww * 2 - 1] * (2.0f * q);
+  } else {
+    w = 2.0f * wwo * q;
+  }
+  return w / wwo;
+}
+
+// Free-surface velocity ghosts written inside the stress kernel
+// (kernel.cu:1087-1112 for dstrqc, 660-687 for dstrqc_new).  The k=top-1
+// thread's writes of u1/v1 at top+1 repeat the same formula on the same
+// interior values, so one write per column reproduces them.
+template <typename T, bool NEWLIT>
+void fs_ghosts(T* u1, T* v1, T* w1, const T* lam_mu, const Grid<T>& g, int NX,
+               int NPC, int rankx, int ranky, int s_i, int e_i, int s_j, int e_j) {
+  const long sl = g.slice, yl = g.yline;
+  const int k = g.nzt + ALIGN - 1;
+#pragma omp parallel for schedule(static)
+  for (int i = s_i; i <= e_i; ++i) {
+    ftz_daz();
+    for (int j = s_j; j <= e_j; ++j) {
+      const long p = i * sl + j * yl + k;
+      u1[p + 1] = u1[p] - (w1[p] - w1[p - sl]);
+      v1[p + 1] = v1[p] - (w1[p + yl] - w1[p]);
+      T vs1, vs2;
+      int gi = g.nxt * rankx + i - NGSL - 1;
+      if (gi < NX || (NEWLIT && NPC == 2))
+        vs1 = u1[p + sl] - (w1[p + sl] - w1[p]);
+      else
+        vs1 = (T)0.0;
+      gi = g.nyt * ranky + j - NGSL - 1;
+      if (gi > 1 || NPC == 2)
+        vs2 = v1[p - yl] - (w1[p] - w1[p - yl]);
+      else
+        vs2 = (T)0.0;
+      w1[p + 1] = w1[p - 1] - lam_mu[(long)i * (g.nyt + 4 + NGSL2) + j] *
+                                  ((vs1 - u1[p + 1]) + (u1[p + sl] - u1[p]) +
+                                   (v1[p + 1] - vs2) + (v1[p] - v1[p - yl]));
+    }
+  }
+}
+
+// dstrqc (kernel.cu:880-1197) when NEWLIT=false; dstrqc_new (343-859, the
+// non-ELA, non-SMEM build) when NEWLIT=true.
+template <typename T, bool NEWLIT>
+void dstrqc(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz, T* r1, T* r2, T* r3,
+            T* r4, T* r5, T* r6, T* u1, T* v1, T* w1, const T* lam, const T* mu,
+            const T* qp, const T* coeff, const T* qs, const T* dcrjx,
+            const T* dcrjy, const T* dcrjz, const T* lam_mu, const T* vx1,
+            const T* vx2, const int* ww, const T* wwo, int NX, int NPC,
+            int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i,
+            const Grid<T>& g) {
+  const long sl = g.slice, yl = g.yline;
+  const int mink = ALIGN + 3;
+  const int maxk = (d_i == 0) ? g.nzt + ALIGN - 1 : g.nzt + ALIGN - 3;
+  if (d_i == 0)
+    fs_ghosts<T, NEWLIT>(u1, v1, w1, lam_mu, g, NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j);
+  const T c1 = g.c1, c2 = g.c2, DT = g.DT, dth = g.dth, dh1 = g.dh1;
+#pragma omp parallel for schedule(static)
+  for (int i = s_i; i <= e_i; ++i) {
+    ftz_daz();
+    for (int j = s_j; j <= e_j; ++j) {
+      for (int k = mink; k <= maxk; ++k) {
+        const long p = i * sl + j * yl + k;
+        const long ip1 = p + sl, jm1 = p - yl, ijk = p + sl - yl, km1 = p - 1,
+                   ik1 = p + sl - 1, jk1 = p - yl - 1, ijk1 = p + sl - yl - 1;
+        const T f_vx1 = vx1[p], f_vx2 = vx2[p], f_wwo = wwo[p];
+        const int f_ww = ww[p];
+        const T f_dcrj = dcrjx[i] * dcrjy[j] * dcrjz[k];
+        const T sl8 = lam[p] + lam[ip1] + lam[jm1] + lam[ijk] + lam[km1] + lam[ik1] + lam[jk1] + lam[ijk1];
+        const T sm8 = mu[p] + mu[ip1] + mu[jm1] + mu[ijk] + mu[km1] + mu[ik1] + mu[jk1] + mu[ijk1];
+        const T sp8 = qp[p] + qp[ip1] + qp[jm1] + qp[ijk] + qp[km1] + qp[ik1] + qp[jk1] + qp[ijk1];
+        const T ss8 = qs[p] + qs[ip1] + qs[jm1] + qs[ijk] + qs[km1] + qs[ik1] + qs[jk1] + qs[ijk1];
+        T xl, xm, xmu1, xmu2, xmu3, qpa, h, h1, h2, h3, qpaw, hw, h1w, h2w, h3w;
+        if (!NEWLIT) {
+          xl = (T)(8.0 / (double)sl8);
+          xm = (T)(16.0 / (double)sm8);
+          xmu1 = (T)(2.0 / (double)(mu[p] + mu[km1]));
+          xmu2 = (T)(2.0 / (double)(mu[p] + mu[jm1]));
+          xmu3 = (T)(2.0 / (double)(mu[p] + mu[ip1]));
+          xl = xl + xm;
+          qpa = (T)(0.0625 * (double)sp8);
+          qpaw = qweight_prod(qpa, coeff, f_ww, f_wwo);
+          h = (T)(0.0625 * (double)ss8);
+          hw = qweight_prod(h, coeff, f_ww, f_wwo);
+          h1 = (T)(0.250 * (double)(qs[p] + qs[km1]));
+          h1w = qweight_prod(h1, coeff, f_ww, f_wwo);
+          h2 = (T)(0.250 * (double)(qs[p] + qs[jm1]));
+          h2w = qweight_prod(h2, coeff, f_ww, f_wwo);
+          h3 = (T)(0.250 * (double)(qs[p] + qs[ip1]));
+          h3w = qweight_prod(h3, coeff, f_ww, f_wwo);
+        } else {
+          xl = 8.0f / sl8;
+          xm = 16.0f / sm8;
+          xmu1 = 2.0f / (mu[p] + mu[km1]);
+          xmu2 = (T)(2.0 / (double)(mu[p] + mu[jm1]));
+          xmu3 = (T)(2.0 / (double)(mu[p] + mu[ip1]));
+          xl = xl + xm;
+          qpa = 0.0625f * sp8;
+          qpaw = qweight_new(qpa, coeff, f_ww, f_wwo, true);
+          h = 0.0625f * ss8;
+          hw = qweight_new(h, coeff, f_ww, f_wwo, false);
+          h1 = 0.250f * (qs[p] + qs[km1]);
+          h1w = qweight_new(h1, coeff, f_ww, f_wwo, false);
+          h2 = 0.250f * (qs[p] + qs[jm1]);
+          h2w = qweight_new(h2, coeff, f_ww, f_wwo, false);
+          h3 = 0.250f * (qs[p] + qs[ip1]);
+          h3w = qweight_new(h3, coeff, f_ww, f_wwo, false);
+        }
+        h = -xm * hw * dh1;
+        h1 = -xmu1 * h1w * dh1;
+        h2 = -xmu2 * h2w * dh1;
+        h3 = -xmu3 * h3w * dh1;
+        qpa = -qpaw * xl * dh1;
+        xm = xm * dth;
+        xmu1 = xmu1 * dth;
+        xmu2 = xmu2 * dth;
+        xmu3 = xmu3 * dth;
+        xl = xl * dth;
+        h = h * f_vx1;
+        h1 = h1 * f_vx1;
+        h2 = h2 * f_vx1;
+        h3 = h3 * f_vx1;
+        qpa = qpa * f_vx1;
+        xm = xm + DT * h;
+        xmu1 = xmu1 + DT * h1;
+        xmu2 = xmu2 + DT * h2;
+        xmu3 = xmu3 + DT * h3;
+        const T vx1p = DT * (1 + f_vx2 * f_vx1);
+
+        T vs1 = c1 * (u1[p + sl] - u1[p]) + c2 * (u1[p + 2 * sl] - u1[p - sl]);
+        T vs2 = c1 * (v1[p] - v1[p - yl]) + c2 * (v1[p + yl] - v1[p - 2 * yl]);
+        T vs3 = c1 * (w1[p] - w1[p - 1]) + c2 * (w1[p + 1] - w1[p - 2]);
+        T tmp = xl * (vs1 + vs2 + vs3);
+        const T a1 = qpa * (vs1 + vs2 + vs3);
+        tmp = tmp + DT * a1;
+
+        T f_r, f_rtmp, s;
+        // xx: sponge applied once.
+        f_r = r1[p];
+        f_rtmp = -h * (vs2 + vs3) + a1;
+        s = xx[p] + tmp - xm * (vs2 + vs3) + vx1p * f_r;
+        r1[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+        f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+        xx[p] = (s + DT * f_rtmp) * f_dcrj;
+        // yy, zz: sponge applied twice (kernel.cu:1124-1142).
+        f_r = r2[p];
+        f_rtmp = -h * (vs1 + vs3) + a1;
+        s = (yy[p] + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
+        r2[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+        f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+        yy[p] = (s + DT * f_rtmp) * f_dcrj;
+        f_r = r3[p];
+        f_rtmp = -h * (vs1 + vs2) + a1;
+        s = (zz[p] + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
+        r3[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+        f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+        zz[p] = (s + DT * f_rtmp) * f_dcrj;
+        // xy
+        vs1 = c1 * (u1[p + yl] - u1[p]) + c2 * (u1[p + 2 * yl] - u1[p - yl]);
+        vs2 = c1 * (v1[p] - v1[p - sl]) + c2 * (v1[p + sl] - v1[p - 2 * sl]);
+        f_r = r4[p];
+        f_rtmp = h1 * (vs1 + vs2);
+        s = xy[p] + xmu1 * (vs1 + vs2) + vx1p * f_r;
+        r4[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+        f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+        xy[p] = (s + DT * f_rtmp) * f_dcrj;
+        // xz
+        vs1 = c1 * (u1[p + 1] - u1[p]) + c2 * (u1[p + 2] - u1[p - 1]);
+        vs2 = c1 * (w1[p] - w1[p - sl]) + c2 * (w1[p + sl] - w1[p - 2 * sl]);
+        f_r = r5[p];
+        f_rtmp = h2 * (vs1 + vs2);
+        s = xz[p] + xmu2 * (vs1 + vs2) + vx1p * f_r;
+        r5[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+        f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+        xz[p] = (s + DT * f_rtmp) * f_dcrj;
+        // yz
+        vs1 = c1 * (v1[p + 1] - v1[p]) + c2 * (v1[p + 2] - v1[p - 1]);
+        vs2 = c1 * (w1[p + yl] - w1[p]) + c2 * (w1[p + 2 * yl] - w1[p - yl]);
+        f_r = r6[p];
+        f_rtmp = h3 * (vs1 + vs2);
+        s = yz[p] + xmu3 * (vs1 + vs2) + vx1p * f_r;
+        r6[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+        f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+        yz[p] = (s + DT * f_rtmp) * f_dcrj;
+      }
+    }
+  }
+}
+
+// fstr (kernel.cu:1616-1647): antisymmetric stress imaging above the free
+// surface, zero horizontal shear on it.
+template <typename T>
+void fstr(T* zz, T* xz, T* yz, const Grid<T>& g, int s_i, int e_i, int s_j, int e_j) {
+  const long sl = g.slice, yl = g.yline;
+  const int k = g.nzt + ALIGN - 1;
+  for (int i = s_i; i <= e_i; ++i) {
+    for (int j = s_j; j <= e_j; ++j) {
+      const long p = i * sl + j * yl + k;
+      zz[p + 1] = -zz[p];
+      zz[p + 2] = -zz[p - 1];
+      xz[p + 1] = -xz[p - 1];
+      xz[p + 2] = -xz[p - 2];
+      yz[p + 1] = -yz[p - 1];
+      yz[p + 2] = -yz[p - 2];
+      xz[p] = (T)0.0;
+      yz[p] = (T)0.0;
+    }
+  }
+}
+
+// addsrc_cu (kernel.cu:1780-1806) / host addsrc (src/awp/source.c:271-308):
+// sigma -= DT/DH^3 * Mdot at sample (i-1); sources applied in list order.
+template <typename T>
+void addsrc(int i, int READ_STEP, int dim, const int* psrc, int npsrc,
+            const T* axx, const T* ayy, const T* azz, const T* axz,
+            const T* ayz, const T* axy, T* xx, T* yy, T* zz, T* xy, T* yz,
+            T* xz, const Grid<T>& g) {
+  const T vtst = g.DT / (g.DH * g.DH * g.DH);
+  i = i - 1;
+  for (int j = 0; j < npsrc; ++j) {
+    const int idx = psrc[j * dim] + 1 + NGSL;
+    const int idy = psrc[j * dim + 1] + 1 + NGSL;
+    const int idz = psrc[j * dim + 2] + ALIGN - 1;
+    const long p = idx * g.slice + idy * g.yline + idz;
+    const long s = (long)j * READ_STEP + i;
+    xx[p] = xx[p] - vtst * axx[s];
+    yy[p] = yy[p] - vtst * ayy[s];
+    zz[p] = zz[p] - vtst * azz[s];
+    xz[p] = xz[p] - vtst * axz[s];
+    yz[p] = yz[p] - vtst * ayz[s];
+    xy[p] = xy[p] - vtst * axy[s];
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void awp_oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int awp_oracle_get_threads(void) { return omp_get_max_threads(); }
+
+#define AWP_ORACLE_DVELCX(SUF, T)                                                   \
+  void awp_oracle_dvelcx_##SUF(T* u1, T* v1, T* w1, const T* xx, const T* yy,       \
+                               const T* zz, const T* xy, const T* xz, const T* yz,  \
+                               const T* dcrjx, const T* dcrjy, const T* dcrjz,      \
+                               const T* d1, int nxt, int nyt, int nzt, double DH,   \
+                               double DT, int s_i, int e_i, int d_i, int ngrids) {  \
+    ftz_daz();                                                                      \
+    const Grid<T> g = make_grid<T>(DH, DT, nxt, nyt, nzt);                          \
+    dvelcx<T>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d1, g, s_i,  \
+              e_i, d_i, ngrids);                                                    \
+  }
+AWP_ORACLE_DVELCX(f32, float)
+AWP_ORACLE_DVELCX(f64, double)
+
+#define AWP_ORACLE_DSTRQC(NAME, T, NEWLIT)                                           \
+  void NAME(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz, T* r1, T* r2, T* r3, T* r4,   \
+            T* r5, T* r6, T* u1, T* v1, T* w1, const T* lam, const T* mu,           \
+            const T* qp, const T* coeff, const T* qs, const T* dcrjx,               \
+            const T* dcrjy, const T* dcrjz, const T* lam_mu, const T* vx1,          \
+            const T* vx2, const int* ww, const T* wwo, int nxt, int nyt, int nzt,   \
+            double DH, double DT, int NX, int NPC, int rankx, int ranky, int s_i,   \
+            int e_i, int s_j, int e_j, int d_i) {                                   \
+    ftz_daz();                                                                      \
+    const Grid<T> g = make_grid<T>(DH, DT, nxt, nyt, nzt);                          \
+    dstrqc<T, NEWLIT>(xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1,   \
+                      lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, lam_mu, vx1,     \
+                      vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j, d_i, \
+                      g);                                                           \
+  }
+AWP_ORACLE_DSTRQC(awp_oracle_dstrqc_f32, float, false)
+AWP_ORACLE_DSTRQC(awp_oracle_dstrqc_f64, double, false)
+AWP_ORACLE_DSTRQC(awp_oracle_dstrqc_new_f32, float, true)
+
+#define AWP_ORACLE_FSTR(SUF, T)                                                     \
+  void awp_oracle_fstr_##SUF(T* zz, T* xz, T* yz, int nxt, int nyt, int nzt,        \
+                             int s_i, int e_i, int s_j, int e_j) {                  \
+    ftz_daz();                                                                      \
+    const Grid<T> g = make_grid<T>(1.0, 1.0, nxt, nyt, nzt);                        \
+    fstr<T>(zz, xz, yz, g, s_i, e_i, s_j, e_j);                                     \
+  }
+AWP_ORACLE_FSTR(f32, float)
+AWP_ORACLE_FSTR(f64, double)
+
+#define AWP_ORACLE_ADDSRC(SUF, T)                                                    \
+  void awp_oracle_addsrc_##SUF(int i, int READ_STEP, int dim, const int* psrc,       \
+                               int npsrc, const T* axx, const T* ayy, const T* azz,  \
+                               const T* axz, const T* ayz, const T* axy, T* xx,      \
+                               T* yy, T* zz, T* xy, T* yz, T* xz, int nxt, int nyt,  \
+                               int nzt, double DH, double DT) {                      \
+    ftz_daz();                                                                       \
+    const Grid<T> g = make_grid<T>(DH, DT, nxt, nyt, nzt);                           \
+    addsrc<T>(i, READ_STEP, dim, psrc, npsrc, axx, ayy, azz, axz, ayz, axy, xx, yy,  \
+              zz, xy, yz, xz, g);                                                    \
+  }
+AWP_ORACLE_ADDSRC(f32, float)
+AWP_ORACLE_ADDSRC(f64, double)
+
+}  // extern "C"

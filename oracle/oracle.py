@@ -1,0 +1,159 @@
+"""Python driver of the CPU oracle (oracle/awp_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+
+OracleSolver mirrors awp_amd.solver.Solver on host numpy arrays in float32
+(the reference's single-precision arithmetic) or float64 (the FP64 oracle),
+in the pmcl3d per-step order (reference src/awp/pmcl3d.c:1628-2487).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libawp_oracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = C.CDLL(str(LIB_PATH))
+        P, I, D = C.c_void_p, C.c_int, C.c_double
+        for suf in ("f32", "f64"):
+            getattr(_lib, f"awp_oracle_dvelcx_{suf}").argtypes = [P] * 13 + [I, I, I, D, D, I, I, I, I]
+            getattr(_lib, f"awp_oracle_dstrqc_{suf}").argtypes = (
+                [P] * 28 + [I, I, I, D, D] + [I] * 9)
+            getattr(_lib, f"awp_oracle_fstr_{suf}").argtypes = [P] * 3 + [I] * 7
+            getattr(_lib, f"awp_oracle_addsrc_{suf}").argtypes = (
+                [I, I, I, P, I] + [P] * 12 + [I, I, I, D, D])
+        _lib.awp_oracle_dstrqc_new_f32.argtypes = [P] * 28 + [I, I, I, D, D] + [I] * 9
+        _lib.awp_oracle_set_threads.argtypes = [I]
+        _lib.awp_oracle_get_threads.restype = I
+    return _lib
+
+
+def set_threads(n: int) -> None:
+    load().awp_oracle_set_threads(int(n))
+
+
+def _p(a: np.ndarray) -> int:
+    assert a.flags.c_contiguous
+    return a.ctypes.data
+
+
+FIELDS = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
+          "r1", "r2", "r3", "r4", "r5", "r6")
+
+
+class OracleSolver:
+    def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
+                 dtype=np.float32, variant: str = "dstrqc", NPC: int = 0):
+        self.lib = load()
+        self.dtype = np.dtype(dtype)
+        self.suf = "f32" if self.dtype == np.float32 else "f64"
+        if variant != "dstrqc" and self.suf != "f32":
+            raise ValueError("dstrqc_new literal variant exists in float32 only")
+        self.variant = variant
+        self.NPC = NPC
+        self.medium = medium
+        self.dims = d = medium.dims
+        self.f = {n: np.zeros(d.shape, dtype=self.dtype) for n in FIELDS}
+        self.m = {}
+        for n in ("d1", "lam", "mu", "qp", "qs", "vx1", "vx2", "wwo", "coeff", "lam_mu",
+                  "dcrjx", "dcrjy", "dcrjz"):
+            self.m[n] = np.ascontiguousarray(getattr(medium, n), dtype=self.dtype)
+        self.m["ww"] = np.ascontiguousarray(medium.ww, dtype=np.int32)
+        self.DH, self.DT = float(medium.DH), float(medium.DT)
+        self.src = None
+        if sources is not None and sources.npsrc > 0:
+            self.src = dict(psrc=np.ascontiguousarray(sources.psrc(d).reshape(-1), dtype=np.int32),
+                            cols=[np.ascontiguousarray(c, dtype=self.dtype) for c in sources.columns()],
+                            npsrc=sources.npsrc, nst=sources.nst)
+        self.rec = None
+        if receivers is not None and len(receivers.xy) > 0:
+            self.rec = dict(idx=receivers.array_index(d), out=np.zeros((max(nt_record, 1), len(receivers.xy), 3), dtype=self.dtype))
+        self.cur_step = 0
+
+    def velocity(self) -> None:
+        d, f, m = self.dims, self.f, self.m
+        s_i, e_i = d.i_range
+        getattr(self.lib, f"awp_oracle_dvelcx_{self.suf}")(
+            *(_p(f[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")),
+            _p(m["dcrjx"]), _p(m["dcrjy"]), _p(m["dcrjz"]), _p(m["d1"]), d.nxt, d.nyt, d.nzt,
+            self.DH, self.DT, s_i, e_i, 0, 1)
+
+    def stress(self) -> None:
+        d, f, m = self.dims, self.f, self.m
+        s_i, e_i = d.i_range
+        s_j, e_j = d.j_range
+        name = (f"awp_oracle_dstrqc_{self.suf}" if self.variant == "dstrqc"
+                else "awp_oracle_dstrqc_new_f32")
+        getattr(self.lib, name)(
+            *(_p(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                 "r5", "r6", "u1", "v1", "w1")),
+            _p(m["lam"]), _p(m["mu"]), _p(m["qp"]), _p(m["coeff"]), _p(m["qs"]),
+            _p(m["dcrjx"]), _p(m["dcrjy"]), _p(m["dcrjz"]), _p(m["lam_mu"]), _p(m["vx1"]),
+            _p(m["vx2"]), _p(m["ww"]), _p(m["wwo"]), d.nxt, d.nyt, d.nzt, self.DH, self.DT,
+            d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, 0)
+
+    def add_source(self, sample_step: int) -> None:
+        if self.src is None:
+            return
+        f, s, d = self.f, self.src, self.dims
+        getattr(self.lib, f"awp_oracle_addsrc_{self.suf}")(
+            sample_step, s["nst"], 3, _p(s["psrc"]), s["npsrc"], *(_p(c) for c in s["cols"]),
+            _p(f["xx"]), _p(f["yy"]), _p(f["zz"]), _p(f["xy"]), _p(f["yz"]), _p(f["xz"]),
+            d.nxt, d.nyt, d.nzt, self.DH, self.DT)
+
+    def free_surface(self) -> None:
+        d, f = self.dims, self.f
+        s_i, e_i = d.i_range
+        s_j, e_j = d.j_range
+        getattr(self.lib, f"awp_oracle_fstr_{self.suf}")(
+            _p(f["zz"]), _p(f["xz"]), _p(f["yz"]), d.nxt, d.nyt, d.nzt, s_i, e_i, s_j, e_j)
+
+    def record(self, slot: int) -> None:
+        if self.rec is None or slot >= self.rec["out"].shape[0]:
+            return
+        idx = self.rec["idx"]
+        f = self.f
+        i, j, k = idx[:, 0], idx[:, 1], idx[:, 2]
+        out = self.rec["out"][slot]
+        out[:, 0] = f["u1"][i, j, k]
+        out[:, 1] = f["v1"][i, j, k]
+        out[:, 2] = f["w1"][i, j, k - 1]
+
+    def start(self) -> None:
+        self.add_source(1)
+
+    def step(self) -> None:
+        self.cur_step += 1
+        self.velocity()
+        self.stress()
+        if self.src is not None and self.cur_step < self.src["nst"]:
+            self.add_source(self.cur_step + 1)
+        self.free_surface()
+        self.record(self.cur_step - 1)
+
+    def run(self, nt: int) -> None:
+        if self.cur_step == 0:
+            self.start()
+        for _ in range(nt):
+            self.step()
+
+    def traces(self) -> np.ndarray:
+        return self.rec["out"] if self.rec is not None else np.zeros((0, 0, 3))

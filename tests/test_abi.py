@@ -1,0 +1,70 @@
+"""The C ABI: libawp_hip.so loads (no GPU needed) and exports every symbol
+include/awp_hip.h declares."""
+import ctypes
+import re
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "awp_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(?:void|int|const char\s*\*)\s+(\w+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_the_reference_launchers():
+    names = declared_symbols()
+    for n in ("SetDeviceConstValue", "dvelcx_H_opt", "dstrqc_H", "dstrqc_H_new", "fstr_H",
+              "addsrc_H", "velbuffer_H"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    from awp_amd import _lib
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_bindings_cover_header():
+    from awp_amd import _lib
+    assert set(declared_symbols()) <= set(_lib.SIGNATURES)
+
+
+def test_error_channel_without_gpu():
+    """Argument validation runs before any HIP call: an unset grid is an
+    error, not a launch."""
+    from awp_amd import _lib
+    lib = _lib.load()
+    F, I = ctypes.c_float, ctypes.c_int
+    z9 = (F * 9)()
+    lib.SetDeviceConstValue((F * 1)(100.0), F(0.005), (I * 1)(8), (I * 1)(8), (I * 1)(8), 1,
+                            0.0, 0.0, z9, z9)
+    lib.awp_clear_error()
+    lib.dvelcx_H_opt(*([None] * 13), 8, 8, None, 10, 17, 9, 1)
+    assert lib.awp_last_error() == 1
+    assert b"d_i=9" in lib.awp_last_error_string()
+    lib.awp_clear_error()
+    assert lib.awp_last_error() == 0
+
+
+def test_range_validation_without_gpu():
+    """An i range that would read outside the padded arrays launches nothing."""
+    from awp_amd import _lib
+    lib = _lib.load()
+    F, I = ctypes.c_float, ctypes.c_int
+    z9 = (F * 9)()
+    lib.SetDeviceConstValue((F * 1)(100.0), F(0.005), (I * 1)(8), (I * 1)(8), (I * 1)(8), 1,
+                            0.0, 0.0, z9, z9)
+    lib.awp_clear_error()
+    lib.dvelcx_H_opt(*([None] * 13), 8, 8, None, 1, 17, 0, 1)   # s_i - 2 < 0
+    assert lib.awp_last_error() == 1
+    lib.awp_clear_error()
+    lib.dvelcx_H_opt(*([None] * 13), 8, 8, None, 10, 26, 0, 1)  # e_i + 2 >= nxt + 20
+    assert lib.awp_last_error() == 1
+    lib.awp_clear_error()
+    lib.dvelcx_H_opt(*([None] * 13), 9, 8, None, 10, 17, 0, 1)  # nyt disagrees
+    assert lib.awp_last_error() == 1
+    lib.awp_clear_error()
