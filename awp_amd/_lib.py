@@ -25,8 +25,8 @@ SIGNATURES: dict[str, list] = {
     "SetDeviceConstValue": [C.POINTER(_f), _f, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), _i,
                             _f, _f, C.POINTER(_f), C.POINTER(_f)],
     "dvelcx_H_opt": [_fp] * 13 + [_i, _i, _s, _i, _i, _i, _i],
-    "dstrqc_H": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 10,
-    "dstrqc_H_new": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 10,
+    "dstrqc_H": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,
+    "dstrqc_H_new": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,
     "fstr_H": [_fp] * 3 + [_s, _i, _i, _i, _i],
     "addsrc_H": [_i, _i, _i, _fp, _i, _s] + [_fp] * 12 + [_i],
     "velbuffer_H": [_fp] * 8 + [_i] * 13 + [_s, _i, _fp, _i],

@@ -68,3 +68,19 @@ def test_range_validation_without_gpu():
     lib.dvelcx_H_opt(*([None] * 13), 9, 8, None, 10, 17, 0, 1)  # nyt disagrees
     assert lib.awp_last_error() == 1
     lib.awp_clear_error()
+
+
+def _header_arg_counts():
+    text = (ROOT / "include" / "awp_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(?:void|int|const char\s*\*)\s+(\w+)\s*\(([^)]*)\)", text):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_binding_arity_matches_header():
+    from awp_amd import _lib
+    for name, n in _header_arg_counts().items():
+        assert len(_lib.SIGNATURES[name]) == n, (name, len(_lib.SIGNATURES[name]), n)
