@@ -62,3 +62,19 @@ __device__ __forceinline__ int awp_xcd_remap(int b, int nblocks) {
 }
 
 static inline int awp_cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Loads and stores through a wave-uniform row pointer and a per-lane 32-bit
+// byte offset.  With one wave per j row the row pointer lives in scalar
+// registers and every access of a row lowers to
+// global_load/store v, v_koff, s[base:base+1]: one lane-offset VGPR shared
+// by all fields instead of a 64-bit address pair per access.
+__device__ __forceinline__ float awp_ld(const float* row, unsigned kbyte) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(row) + kbyte);
+}
+__device__ __forceinline__ int awp_ldi(const int* row, unsigned kbyte) {
+  return *reinterpret_cast<const int*>(reinterpret_cast<const char*>(row) + kbyte);
+}
+__device__ __forceinline__ void awp_st(float* row, unsigned kbyte, float v) {
+  *reinterpret_cast<float*>(reinterpret_cast<char*>(row) + kbyte) = v;
+}
+

@@ -7,7 +7,9 @@
 #include "awp_tuning.h"
 #include "../../include/awp_hip.h"
 
-static AwpTuning g_tuning = {4, 4, 2048, 0, 0, 1};
+// Defaults from sweeps on MI355X at 512x512x256: velocity 4 rows x 2 i-chunks
+// (all workgroups resident), stress 16 rows x ~32-plane i-chunks.
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -115,17 +117,20 @@ extern "C" {
 int awp_set_tuning(const char* key, int value) {
   AwpTuning& t = g_tuning;
   if (!strcmp(key, "vel_bj") || !strcmp(key, "str_bj")) {
-    if (value != 1 && value != 2 && value != 4 && value != 8) return -1;
+    if (value != 1 && value != 2 && value != 4 && value != 8 && value != 16) return -1;
     (!strcmp(key, "vel_bj") ? t.vel_bj : t.str_bj) = value;
-  } else if (!strcmp(key, "target_wgs")) {
+  } else if (!strcmp(key, "vel_target_wgs") || !strcmp(key, "str_target_wgs")) {
     if (value < 1) return -1;
-    t.target_wgs = value;
+    (!strcmp(key, "vel_target_wgs") ? t.vel_target_wgs : t.str_target_wgs) = value;
   } else if (!strcmp(key, "vel_ichunk")) {
     t.vel_ichunk = value < 0 ? 0 : value;
   } else if (!strcmp(key, "str_ichunk")) {
     t.str_ichunk = value < 0 ? 0 : value;
   } else if (!strcmp(key, "xcd_remap")) {
     t.xcd_remap = value ? 1 : 0;
+  } else if (!strcmp(key, "kalign")) {
+    t.kalign = value ? 1 : 0;
+
   } else {
     return -1;
   }

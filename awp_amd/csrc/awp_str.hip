@@ -17,7 +17,7 @@
 namespace {
 
 struct StrLaunch {
-  int k0, k1, j0, j1, i0;
+  int kb, k0, k1, j0, j1, i0;  // kb: k of lane 0 in tile 0; [k0, k1] active
   int nkt, njt, ichunk, ni, nblocks, remap;
   int yline;
   long long slice;
@@ -72,9 +72,9 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
   const int kt = b % L.nkt;
   const int jt = (b / L.nkt) % L.njt;
   const int ic = b / (L.nkt * L.njt);
-  const int k = L.k0 + kt * 64 + (int)threadIdx.x;
+  const int k = L.kb + kt * 64 + (int)threadIdx.x;
   const int j = L.j0 + jt * BJ + (int)threadIdx.y;
-  if (k > L.k1 || j > L.j1) return;
+  if (k < L.k0 || k > L.k1 || j > L.j1) return;
   const int ia = L.i0 + ic * L.ichunk;
   int nsteps = L.ni - ic * L.ichunk;
   if (nsteps > L.ichunk) nsteps = L.ichunk;
@@ -337,6 +337,9 @@ void dstrqc_common(const char* W, float* xx, float* yy, float* zz, float* xy, fl
   }
   StrLaunch L;
   L.k0 = AWP_ALIGN + 3;  // kernel.cu:901 (mink)
+  // Tiles start on a 128-byte line (k = 32): whole-line loads and stores;
+  // the three lanes below mink idle.
+  L.kb = awp_tuning().kalign ? (L.k0 & ~31) : L.k0;
   L.k1 = (d_i == 0) ? nzt + AWP_ALIGN - 1 : nzt + AWP_ALIGN - 3;
   L.j0 = s_j;
   L.j1 = e_j;
@@ -348,10 +351,10 @@ void dstrqc_common(const char* W, float* xx, float* yy, float* zz, float* xy, fl
   L.remap = awp_tuning().xcd_remap;
   if (L.k1 < L.k0) return;
   const int bj = awp_tuning().str_bj;
-  L.nkt = awp_cdiv(L.k1 - L.k0 + 1, 64);
+  L.nkt = awp_cdiv(L.k1 - L.kb + 1, 64);
   L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
   int nic, ichunk;
-  awp_split_i(L.ni, L.nkt * L.njt, awp_tuning().str_ichunk, awp_tuning().target_wgs, &nic, &ichunk);
+  awp_split_i(L.ni, L.nkt * L.njt, awp_tuning().str_ichunk, awp_tuning().str_target_wgs, &nic, &ichunk);
   L.ichunk = ichunk;
   L.nblocks = L.nkt * L.njt * nic;
 #define AWP_STR_ARGS                                                                          \
@@ -360,8 +363,9 @@ void dstrqc_common(const char* W, float* xx, float* yy, float* zz, float* xy, fl
   switch (bj) {
     case 1: hipLaunchKernelGGL((dstrqc_kernel<1, NEWLIT>), dim3(L.nblocks), dim3(64, 1), 0, St, AWP_STR_ARGS); break;
     case 2: hipLaunchKernelGGL((dstrqc_kernel<2, NEWLIT>), dim3(L.nblocks), dim3(64, 2), 0, St, AWP_STR_ARGS); break;
+    case 4: hipLaunchKernelGGL((dstrqc_kernel<4, NEWLIT>), dim3(L.nblocks), dim3(64, 4), 0, St, AWP_STR_ARGS); break;
     case 8: hipLaunchKernelGGL((dstrqc_kernel<8, NEWLIT>), dim3(L.nblocks), dim3(64, 8), 0, St, AWP_STR_ARGS); break;
-    default: hipLaunchKernelGGL((dstrqc_kernel<4, NEWLIT>), dim3(L.nblocks), dim3(64, 4), 0, St, AWP_STR_ARGS); break;
+    default: hipLaunchKernelGGL((dstrqc_kernel<16, NEWLIT>), dim3(L.nblocks), dim3(64, 16), 0, St, AWP_STR_ARGS); break;
   }
 #undef AWP_STR_ARGS
   awp_check_launch(W);

@@ -4,10 +4,12 @@
 struct AwpTuning {
   int vel_bj;       // j rows per velocity workgroup (64 k-lanes each)
   int str_bj;       // j rows per stress workgroup
-  int target_wgs;   // workgroups to aim for when splitting the i march
+  int vel_target_wgs;  // velocity: workgroups to aim for when splitting the i march
+  int str_target_wgs;  // stress: same
   int vel_ichunk;   // >0 forces the velocity i-chunk length
   int str_ichunk;   // >0 forces the stress i-chunk length
   int xcd_remap;    // 1: XCD-aware tile order, 0: plain order
+  int kalign;       // 1: start k tiles on a 32-float (128 B) line boundary
 };
 
 AwpTuning& awp_tuning();

@@ -17,7 +17,7 @@
 namespace {
 
 struct VelLaunch {
-  int k0, k1, j0, j1, i0;
+  int kb, k0, k1, j0, j1, i0;  // kb: k of lane 0 in tile 0; [k0, k1] active
   int nkt, njt, ichunk, ni, nblocks, remap;
   int yline;
   long long slice;
@@ -35,57 +35,71 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
   const int kt = b % L.nkt;
   const int jt = (b / L.nkt) % L.njt;
   const int ic = b / (L.nkt * L.njt);
-  const int k = L.k0 + kt * 64 + (int)threadIdx.x;
-  const int j = L.j0 + jt * BJ + (int)threadIdx.y;
-  if (k > L.k1 || j > L.j1) return;
+  // one wave per j row: j is wave-uniform, k is the lane
+  const int j = L.j0 + jt * BJ + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const int k = L.kb + kt * 64 + (int)threadIdx.x;
+  if (j > L.j1) return;
+  if (k < L.k0 || k > L.k1) return;
   const int ia = L.i0 + ic * L.ichunk;
   int nsteps = L.ni - ic * L.ichunk;
   if (nsteps > L.ichunk) nsteps = L.ichunk;
 
-  const int Y = L.yline;
+  const long long Y = L.yline;
   const long long S = L.slice;
-  const int o = j * Y + k;  // in-plane offset
+  const long long J = (long long)j * Y;
+  const unsigned kb = (unsigned)k * 4u;
   const float c1 = L.c1, c2 = L.c2, dth = L.dth;
-  const float dyz = dcrjy[j];
+  const float dy = dcrjy[j];
   const float dz = dcrjz[k];
+// row pointer of field A at plane ii, row j+dj, shifted by dk in k (uniform)
+#define RP(A, ii, dj, dk) ((A) + (long long)(ii) * S + J + (long long)(dj) * Y + (dk))
 
   // Prime the x pipelines at i = ia.
-  const long long p0 = (long long)ia * S + o;
-  float xx_m2 = xx[p0 - 2 * S], xx_m1 = xx[p0 - S], xx_0 = xx[p0];
-  float xy_m1 = xy[p0 - S], xy_0 = xy[p0], xy_p1 = xy[p0 + S];
-  float xz_m1 = xz[p0 - S], xz_0 = xz[p0], xz_p1 = xz[p0 + S];
+  float xx_m2 = awp_ld(RP(xx, ia - 2, 0, 0), kb), xx_m1 = awp_ld(RP(xx, ia - 1, 0, 0), kb),
+        xx_0 = awp_ld(RP(xx, ia, 0, 0), kb);
+  float xy_m1 = awp_ld(RP(xy, ia - 1, 0, 0), kb), xy_0 = awp_ld(RP(xy, ia, 0, 0), kb),
+        xy_p1 = awp_ld(RP(xy, ia + 1, 0, 0), kb);
+  float xz_m1 = awp_ld(RP(xz, ia - 1, 0, 0), kb), xz_0 = awp_ld(RP(xz, ia, 0, 0), kb),
+        xz_p1 = awp_ld(RP(xz, ia + 1, 0, 0), kb);
   // density at plane i: (j,k), (j-1,k), (j,k-1), (j-1,k-1)
-  float dc = d1[p0], djm = d1[p0 - Y], dkm = d1[p0 - 1], djk = d1[p0 - Y - 1];
+  float dc = awp_ld(RP(d1, ia, 0, 0), kb), djm = awp_ld(RP(d1, ia, -1, 0), kb),
+        dkm = awp_ld(RP(d1, ia, 0, -1), kb), djk = awp_ld(RP(d1, ia, -1, -1), kb);
 
   for (int s = 0; s < nsteps; ++s) {
-    const int i = ia + s;
-    const float* xxp = xx + (long long)i * S;
-    const float* xyp = xy + (long long)i * S;
-    const float* xzp = xz + (long long)i * S;
-    const float* yyp = yy + (long long)i * S;
-    const float* yzp = yz + (long long)i * S;
-    const float* zzp = zz + (long long)i * S;
-    const float* d1n = d1 + (long long)(i + 1) * S;
-    float* up = u1 + (long long)i * S;
-    float* vp = v1 + (long long)i * S;
-    float* wp = w1 + (long long)i * S;
-
+    // readfirstlane hides the plane recurrence from loop strength reduction,
+    // which would otherwise turn every field address into a 64-bit VGPR
+    // induction pointer; the plane bases stay in scalar registers.
+    const int i = __builtin_amdgcn_readfirstlane(ia + s);
     // pipeline leads
-    const float xx_p1 = xxp[S + o];
-    const float xy_p2 = xyp[2 * S + o];
-    const float xz_p2 = xzp[2 * S + o];
-    const float ec = d1n[o], ejm = d1n[o - Y], ekm = d1n[o - 1], ejk = d1n[o - Y - 1];
+    const float xx_p1 = awp_ld(RP(xx, i + 1, 0, 0), kb);
+    const float xy_p2 = awp_ld(RP(xy, i + 2, 0, 0), kb);
+    const float xz_p2 = awp_ld(RP(xz, i + 2, 0, 0), kb);
+    const float ec = awp_ld(RP(d1, i + 1, 0, 0), kb), ejm = awp_ld(RP(d1, i + 1, -1, 0), kb),
+                ekm = awp_ld(RP(d1, i + 1, 0, -1), kb), ejk = awp_ld(RP(d1, i + 1, -1, -1), kb);
 
     // in-plane stencils
-    const float xy_jm1 = xyp[o - Y], xy_jp1 = xyp[o + Y], xy_jm2 = xyp[o - 2 * Y];
-    const float xz_km1 = xzp[o - 1], xz_kp1 = xzp[o + 1], xz_km2 = xzp[o - 2];
-    const float yy_0 = yyp[o], yy_jp1 = yyp[o + Y], yy_jp2 = yyp[o + 2 * Y], yy_jm1 = yyp[o - Y];
-    const float yz_0 = yzp[o], yz_km1 = yzp[o - 1], yz_kp1 = yzp[o + 1], yz_km2 = yzp[o - 2];
-    const float yz_jm1 = yzp[o - Y], yz_jp1 = yzp[o + Y], yz_jm2 = yzp[o - 2 * Y];
-    const float zz_0 = zzp[o], zz_kp1 = zzp[o + 1], zz_kp2 = zzp[o + 2], zz_km1 = zzp[o - 1];
-    const float u = up[o], v = vp[o], w = wp[o];
+    const float* XY = RP(xy, i, 0, 0);
+    const float* XZ = RP(xz, i, 0, 0);
+    const float* YY = RP(yy, i, 0, 0);
+    const float* YZ = RP(yz, i, 0, 0);
+    const float* ZZ = RP(zz, i, 0, 0);
+    const float xy_jm1 = awp_ld(XY - Y, kb), xy_jp1 = awp_ld(XY + Y, kb),
+                xy_jm2 = awp_ld(XY - 2 * Y, kb);
+    const float xz_km1 = awp_ld(XZ - 1, kb), xz_kp1 = awp_ld(XZ + 1, kb), xz_km2 = awp_ld(XZ - 2, kb);
+    const float yy_0 = awp_ld(YY, kb), yy_jp1 = awp_ld(YY + Y, kb), yy_jp2 = awp_ld(YY + 2 * Y, kb),
+                yy_jm1 = awp_ld(YY - Y, kb);
+    const float yz_0 = awp_ld(YZ, kb), yz_km1 = awp_ld(YZ - 1, kb), yz_kp1 = awp_ld(YZ + 1, kb),
+                yz_km2 = awp_ld(YZ - 2, kb);
+    const float yz_jm1 = awp_ld(YZ - Y, kb), yz_jp1 = awp_ld(YZ + Y, kb),
+                yz_jm2 = awp_ld(YZ - 2 * Y, kb);
+    const float zz_0 = awp_ld(ZZ, kb), zz_kp1 = awp_ld(ZZ + 1, kb), zz_kp2 = awp_ld(ZZ + 2, kb),
+                zz_km1 = awp_ld(ZZ - 1, kb);
+    float* U = RP(u1, i, 0, 0);
+    float* V = RP(v1, i, 0, 0);
+    float* W = RP(w1, i, 0, 0);
+    const float u = awp_ld(U, kb), v = awp_ld(V, kb), w = awp_ld(W, kb);
 
-    const float dcrj = dcrjx[i] * dyz * dz;
+    const float dcrj = dcrjx[i] * dy * dz;
     float f_d1 = 0.25f * (dc + djm + dkm + djk);
     float f_d2 = 0.25f * (dc + ec + dkm + ekm);
     float f_d3 = 0.25f * (dc + ec + djm + ejm);
@@ -93,15 +107,15 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     f_d2 = dth / f_d2;
     f_d3 = dth / f_d3;
 
-    up[o] = (u + f_d1 * (c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) + c1 * (xy_0 - xy_jm1) +
-                         c2 * (xy_jp1 - xy_jm2) + c1 * (xz_0 - xz_km1) + c2 * (xz_kp1 - xz_km2))) *
-            dcrj;
-    vp[o] = (v + f_d2 * (c1 * (xy_p1 - xy_0) + c2 * (xy_p2 - xy_m1) + c1 * (yy_jp1 - yy_0) +
-                         c2 * (yy_jp2 - yy_jm1) + c1 * (yz_0 - yz_km1) + c2 * (yz_kp1 - yz_km2))) *
-            dcrj;
-    wp[o] = (w + f_d3 * (c1 * (xz_p1 - xz_0) + c2 * (xz_p2 - xz_m1) + c1 * (yz_0 - yz_jm1) +
-                         c2 * (yz_jp1 - yz_jm2) + c1 * (zz_kp1 - zz_0) + c2 * (zz_kp2 - zz_km1))) *
-            dcrj;
+    awp_st(U, kb, (u + f_d1 * (c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) + c1 * (xy_0 - xy_jm1) +
+                               c2 * (xy_jp1 - xy_jm2) + c1 * (xz_0 - xz_km1) +
+                               c2 * (xz_kp1 - xz_km2))) * dcrj);
+    awp_st(V, kb, (v + f_d2 * (c1 * (xy_p1 - xy_0) + c2 * (xy_p2 - xy_m1) + c1 * (yy_jp1 - yy_0) +
+                               c2 * (yy_jp2 - yy_jm1) + c1 * (yz_0 - yz_km1) +
+                               c2 * (yz_kp1 - yz_km2))) * dcrj);
+    awp_st(W, kb, (w + f_d3 * (c1 * (xz_p1 - xz_0) + c2 * (xz_p2 - xz_m1) + c1 * (yz_0 - yz_jm1) +
+                               c2 * (yz_jp1 - yz_jm2) + c1 * (zz_kp1 - zz_0) +
+                               c2 * (zz_kp2 - zz_km1))) * dcrj);
 
     // advance pipelines
     xx_m2 = xx_m1; xx_m1 = xx_0; xx_0 = xx_p1;
@@ -109,6 +123,7 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     xz_m1 = xz_0; xz_0 = xz_p1; xz_p1 = xz_p2;
     dc = ec; djm = ejm; dkm = ekm; djk = ejk;
   }
+#undef RP
 }
 
 template <int BJ>
@@ -117,10 +132,10 @@ void launch_vel(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz
                 hipStream_t St, VelLaunch L) {
   const int nkj = L.nkt * L.njt;
   int nic, ichunk;
-  awp_split_i(L.ni, nkj, awp_tuning().vel_ichunk, awp_tuning().target_wgs, &nic, &ichunk);
+  awp_split_i(L.ni, nkj, awp_tuning().vel_ichunk, awp_tuning().vel_target_wgs, &nic, &ichunk);
   L.ichunk = ichunk;
   L.nblocks = nkj * nic;
-  hipLaunchKernelGGL(dvelcx_kernel<BJ>, dim3(L.nblocks), dim3(64, BJ), 0, St, u1, v1, w1, xx,
+  hipLaunchKernelGGL((dvelcx_kernel<BJ>), dim3(L.nblocks), dim3(64, BJ), 0, St, u1, v1, w1, xx,
                      yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, L);
 }
 
@@ -160,15 +175,20 @@ extern "C" void dvelcx_H_opt(float* u1, float* v1, float* w1, float* xx, float* 
   L.c2 = g.c2;
   L.dth = g.dth;
   L.remap = awp_tuning().xcd_remap;
+  L.kb = awp_tuning().kalign ? (L.k0 & ~31) : L.k0;
   const int bj = awp_tuning().vel_bj;
-  L.nkt = awp_cdiv(L.k1 - L.k0 + 1, 64);
+  L.nkt = awp_cdiv(L.k1 - L.kb + 1, 64);
   L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
   if (L.k1 < L.k0) return;
+#define AWP_VEL_LAUNCH(B) \
+  launch_vel<B>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, St, L);
   switch (bj) {
-    case 1: launch_vel<1>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, St, L); break;
-    case 2: launch_vel<2>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, St, L); break;
-    case 8: launch_vel<8>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, St, L); break;
-    default: launch_vel<4>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, St, L); break;
+    case 1: AWP_VEL_LAUNCH(1) break;
+    case 2: AWP_VEL_LAUNCH(2) break;
+    case 8: AWP_VEL_LAUNCH(8) break;
+    case 16: AWP_VEL_LAUNCH(16) break;
+    default: AWP_VEL_LAUNCH(4) break;
   }
+#undef AWP_VEL_LAUNCH
   awp_check_launch(W);
 }

@@ -61,7 +61,7 @@ def _compare(torch, dev, host, names, tag):
 
 
 @pytest.mark.parametrize("dims", SIZES, ids=lambda d: f"{d.nxt}x{d.nyt}x{d.nzt}")
-@pytest.mark.parametrize("bj,ichunk,remap", [(4, 0, 1), (1, 3, 0), (8, 7, 1), (2, 1, 1)])
+@pytest.mark.parametrize("bj,ichunk,remap", [(4, 0, 1), (1, 3, 0), (8, 7, 1), (2, 1, 1), (16, 2, 1)])
 def test_dvelcx_bit_exact(gpu, oracle_lib, dims, bj, ichunk, remap):
     torch, lib, _lib = gpu
     _lib.set_tuning("vel_bj", bj)
@@ -122,7 +122,7 @@ def test_dstrqc_bit_exact(gpu, oracle_lib, dims, variant):
     _run_dstrqc(torch, lib, _lib, oracle_lib, dims, variant)
 
 
-@pytest.mark.parametrize("bj,ichunk,remap", [(1, 5, 0), (8, 2, 1), (2, 0, 1)])
+@pytest.mark.parametrize("bj,ichunk,remap", [(1, 5, 0), (8, 2, 1), (2, 0, 1), (4, 3, 1), (16, 1, 0)])
 def test_dstrqc_launch_shapes(gpu, oracle_lib, bj, ichunk, remap):
     torch, lib, _lib = gpu
     _lib.set_tuning("str_bj", bj)
@@ -131,7 +131,7 @@ def test_dstrqc_launch_shapes(gpu, oracle_lib, bj, ichunk, remap):
     try:
         _run_dstrqc(torch, lib, _lib, oracle_lib, Dims(21, 19, 67), "dstrqc")
     finally:
-        _lib.set_tuning("str_bj", 4)
+        _lib.set_tuning("str_bj", 16)
         _lib.set_tuning("str_ichunk", 0)
         _lib.set_tuning("xcd_remap", 1)
 
