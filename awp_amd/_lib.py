@@ -31,6 +31,9 @@ SIGNATURES: dict[str, list] = {
     "addsrc_H": [_i, _i, _i, _fp, _i, _s] + [_fp] * 12 + [_i],
     "velbuffer_H": [_fp] * 8 + [_i] * 13 + [_s, _i, _fp, _i],
     "awp_receivers_gather": [_fp] * 4 + [_i, _fp, _i, _s, _i],
+    "drprecpc_calc_H_opt": [_fp] * 13 + [_i] * 5 + [_s, _i],
+    "awp_drprecpc_calc_tv": [_fp] * 13 + [_i] * 5 + [_s, _i, _f],
+    "drprecpc_app_H": [_fp] * 9 + [_i] * 5 + [_s, _i],
 }
 RESTYPES = {
     "awp_last_error": _i,

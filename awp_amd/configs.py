@@ -180,3 +180,50 @@ def anelastic(nx: int = 512, ny: int = 512, nz: int = 256, nt: int = 1000, DH: f
 
     return Workload("anelastic_%dx%dx%d" % (nx, ny, nz), dims, DH, DT, nt, mk, src, rec,
                     dict(f0=f0, M0=M0, seed=seed))
+
+
+def line_source(dims: Dims, nsub: int, nt: int, DT: float, f0: float, M0: float, seed: int,
+                depth: int | None = None, x0: int | None = None) -> PointSources:
+    """A line of nsub double-couple (xy) point subfaults along x at one depth,
+    each with a seeded amplitude share of M0 and a Ricker moment rate."""
+    rng = np.random.default_rng(seed)
+    nsub = min(nsub, max(dims.nxt - 2, 1))
+    x0 = x0 if x0 is not None else (dims.nxt - nsub) // 2 + 1
+    z = depth if depth is not None else max(dims.nzt // 2, 1)
+    y = dims.nyt // 2
+    cells = np.array([[x0 + n, y, z] for n in range(nsub)])
+    amp = rng.uniform(0.5, 1.5, nsub)
+    amp *= M0 / amp.sum()
+    t = np.arange(nt) * DT
+    rates = np.zeros((nsub, nt, 6))
+    for n in range(nsub):
+        rates[n, :, 5] = amp[n] * ricker(t, f0)
+    return PointSources(cells, rates)
+
+
+def plastic(nx: int = 64, ny: int = 64, nz: int = 48, nt: int = 100, DH: float = 75.0,
+            DT: float = 0.004, seed: int = 11, nsub: int = 200, M0: float = 1.0e19,
+            cohesion: float = 5.0e6, friction: float = 0.75, Tv: float = 0.01,
+            nrec: int = 8, f0: float = 1.0) -> Workload:
+    """Layered + random medium with Q and Drucker-Prager plasticity
+    (cohesion 5 MPa, friction 0.75, viscoplastic relaxation 0.01 s) driven by
+    a line of double-couple subfaults."""
+    from .plasticity import PlasticParams, constant_medium, protect_sources
+    dims = Dims(nx, ny, nz)
+    src = line_source(dims, nsub, nt, DT, f0, M0, seed)
+    rec = seeded_surface_receivers(dims, nrec, seed + 1, margin=min(20, nx // 4))
+
+    def mk():
+        vp, vs, rho, Qp, Qs = layered_random_model(dims, DH, seed)
+        return build_medium(dims, DH, DT, vp, vs, rho, Qp, Qs)
+
+    def mk_plastic(med):
+        pm = constant_medium(dims, DH, med.d1, cohesion, friction,
+                             PlasticParams(fmajor=1.0, fminor=1.0, Tv=Tv))
+        protect_sources(pm, dims, src.psrc(dims))
+        return pm
+
+    w = Workload("plastic_%dx%dx%d" % (nx, ny, nz), dims, DH, DT, nt, mk, src, rec,
+                 dict(cohesion=cohesion, friction=friction, Tv=Tv, M0=M0))
+    w.params["plastic_fn"] = mk_plastic
+    return w

@@ -1,17 +1,7 @@
 """One-GPU driver of the time-stepping path through the C ABI.
 
-Reproduces the per-step call order of the pmcl3d time loop on one rank
-(reference src/awp/pmcl3d.c:1628-2487 with no neighbours):
-
-    dvelcx_H_opt   over i in [2+ngsl, nxt+ngsl+1]            (pmcl3d.c:1690)
-    dstrqc_H       over the three x strips == all stress i   (:1845, :1904-1919)
-    addsrc_H       sample cur_step (if cur_step < NST)       (:2007-2013)
-    fstr_H         on grid 0                                 (:2243)
-    receivers      u, v at the surface, w one cell below     (:2246, kernel.cu:3948)
-
-with the first source sample added before the loop (:1175-1186).
-Device memory is torch-allocated (plumbing only); every update runs in
-libawp_hip.so.
+The per-step call order is pmcl3d's (awp_amd.stepping); every update runs in
+libawp_hip.so.  Device memory is torch-allocated (plumbing only).
 """
 from __future__ import annotations
 
@@ -24,6 +14,8 @@ from . import _lib
 from .configs import PointSources, Receivers
 from .grid import ALIGN, NGSL, Dims
 from .media import Medium
+from .plasticity import PlasticMedium
+from .stepping import PmclStepOrder
 
 FIELDS = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
           "r1", "r2", "r3", "r4", "r5", "r6")
@@ -33,12 +25,13 @@ def _ptr(t: torch.Tensor) -> int:
     return t.data_ptr()
 
 
-class Solver:
+class Solver(PmclStepOrder):
     """FP32 wavefield state and medium of one grid, resident in HBM."""
 
     def __init__(self, medium: Medium, sources: PointSources | None = None,
                  receivers: Receivers | None = None, nt_record: int = 0,
-                 device: str = "cuda", variant: str = "dstrqc", NPC: int = 0):
+                 device: str = "cuda", variant: str = "dstrqc", NPC: int = 0,
+                 plastic: PlasticMedium | None = None):
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a GPU: the HIP path has no CPU fallback")
         self.lib = _lib.load()
@@ -62,8 +55,23 @@ class Solver:
         nxt = (C.c_int * 1)(d.nxt)
         nyt = (C.c_int * 1)(d.nyt)
         nzt = (C.c_int * 1)(d.nzt)
-        z9 = (C.c_float * 9)()
-        self.lib.SetDeviceConstValue(DH, C.c_float(medium.DT), nxt, nyt, nzt, 1, 0.0, 0.0, z9, z9)
+        self.plastic = plastic
+        if plastic is not None:
+            pp = plastic.params
+            Rz, RzT = pp.rotation()
+            Rzc = (C.c_float * 9)(*Rz.tolist())
+            RzTc = (C.c_float * 9)(*RzT.tolist())
+            fmaj, fmin = pp.fmajor, pp.fminor
+            for name in ("sigma2", "cohes", "phi"):
+                self.m[name] = torch.from_numpy(
+                    np.ascontiguousarray(getattr(plastic, name), dtype=np.float32)).to(self.device)
+            self.f["yldfac"] = torch.ones(d.shape, **f32)
+            self.f["neta"] = torch.zeros(d.shape, **f32)
+        else:
+            Rzc = RzTc = (C.c_float * 9)()
+            fmaj = fmin = 0.0
+        self.lib.SetDeviceConstValue(DH, C.c_float(medium.DT), nxt, nyt, nzt, 1, fmaj, fmin, Rzc,
+                                     RzTc)
         _lib.check()
         # sources
         self.src = None
@@ -116,9 +124,14 @@ class Solver:
                               e_i, 0, 1)
 
     def stress(self) -> None:
+        d = self.dims
+        (s_i, e_i), (s_j, e_j) = d.i_range, d.j_range
+        self.stress_region(s_i, e_i, s_j, e_j)
+
+    def stress_region(self, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
         d, f, m = self.dims, self.f, self.m
-        s_i, e_i = d.i_range
-        s_j, e_j = d.j_range
+        if s_i > e_i or s_j > e_j:
+            return
         fn = self.lib.dstrqc_H if self.variant == "dstrqc" else self.lib.dstrqc_H_new
         fn(*(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
                                   "r5", "r6", "u1", "v1", "w1")),
@@ -126,6 +139,23 @@ class Solver:
            _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nyt, d.nzt, self.stream,
            _ptr(m["lam_mu"]), _ptr(m["vx1"]), _ptr(m["vx2"]), _ptr(m["ww"]), _ptr(m["wwo"]),
            d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, 0)
+
+    def plastic_calc(self, xs: int, xe: int, ys: int, ye: int) -> None:
+        f, m, d = self.f, self.m, self.dims
+        args = [_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")] + [
+            _ptr(m["mu"]), _ptr(m["d1"]), _ptr(m["sigma2"]), _ptr(f["yldfac"]),
+            _ptr(m["cohes"]), _ptr(m["phi"]), _ptr(f["neta"]), d.nzt, xs, xe, ys, ye,
+            self.stream, 0]
+        if self.plastic.params.Tv > 0:
+            self.lib.awp_drprecpc_calc_tv(*args, float(self.plastic.params.Tv))
+        else:
+            self.lib.drprecpc_calc_H_opt(*args)
+
+    def plastic_app(self, xs: int, xe: int, ys: int, ye: int) -> None:
+        f, m, d = self.f, self.m, self.dims
+        self.lib.drprecpc_app_H(*(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")),
+                                _ptr(m["mu"]), _ptr(m["sigma2"]), _ptr(f["yldfac"]), d.nzt, xs,
+                                xe, ys, ye, self.stream, 0)
 
     def add_source(self, sample_step: int) -> None:
         if self.src is None:
@@ -151,21 +181,6 @@ class Solver:
         self.lib.awp_receivers_gather(_ptr(f["u1"]), _ptr(f["v1"]), _ptr(f["w1"]),
                                       _ptr(r["idx"]), r["n"], _ptr(r["out"]), slot, self.stream,
                                       0)
-
-    def start(self) -> None:
-        """Initial source sample before the loop (pmcl3d.c:1175-1186)."""
-        self.add_source(1)
-        _lib.check()
-
-    def step(self) -> None:
-        """One time step in pmcl3d order; cur_step counts from 1."""
-        self.cur_step += 1
-        self.velocity()
-        self.stress()
-        if self.src is not None and self.cur_step < self.src["nst"]:
-            self.add_source(self.cur_step + 1)
-        self.free_surface()
-        self.record(self.cur_step - 1)
 
     def run(self, nt: int, check_every: int = 0) -> None:
         if self.cur_step == 0:

@@ -97,6 +97,34 @@ void velbuffer_H(const float* u1, const float* v1, const float* w1, const float*
                  int nskpz, int rec_nxt, int rec_nyt, int rec_nzt, hipStream_t St,
                  int FOLLOWBATHY, const int* bathy, int d_i);
 
+/* Replaces drprecpc_calc_H_opt, include/awp/kernel_launch.h:62-66
+ * (src/awp/kernel.cu:1425-1443, kernel drprecpc_calc_opt :1253-1422).
+ * Drucker-Prager yield factor yldfac and plastic strain neta (+=) on
+ * i in [xls, xre], j in [yls, yre], k in [align, nzt+align-1], with the
+ * initial stress rotated from sigma2 by SetDeviceConstValue's fmajor,
+ * fminor, Rz, RzT and fluid pressure 9.81e3*(depth+DH/2); viscoplastic
+ * relaxation time Tv = DH/vs. */
+void drprecpc_calc_H_opt(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                         float* mu, float* d1, float* sigma2, float* yldfac, float* cohes,
+                         float* phi, float* neta, int nzt, int xls, int xre, int yls, int yre,
+                         hipStream_t St, int d_i);
+
+/* No reference counterpart: drprecpc_calc_H_opt with a fixed viscoplastic
+ * relaxation time Tv > 0 in place of DH/vs (the reference hard-codes DH/vs,
+ * src/awp/kernel.cu:1362). */
+void awp_drprecpc_calc_tv(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                          float* mu, float* d1, float* sigma2, float* yldfac, float* cohes,
+                          float* phi, float* neta, int nzt, int xls, int xre, int yls, int yre,
+                          hipStream_t St, int d_i, float Tv);
+
+/* Replaces drprecpc_app_H, include/awp/kernel_launch.h:68-71
+ * (src/awp/kernel.cu:1445-1463, kernel drprecpc_app :1650-1778).  Scales the
+ * deviatoric normal stresses by yldfac and the shear stresses by the 4-point
+ * mean yield factor at their staggered points, about the initial stress. */
+void drprecpc_app_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                    float* mu, float* sigma2, float* yldfac, int nzt, int xls, int xre, int yls,
+                    int yre, hipStream_t St, int d_i);
+
 /* No reference counterpart: gathers u1, v1 at cell (i, j, k) and w1 at
  * (i, j, k-1) on grid 0 (k-1 -> k on d_i > 0), velbuffer's sampling rule, for
  * an arbitrary list of npts receiver cells idx[3*n + {0,1,2}] = (i, j, k) in

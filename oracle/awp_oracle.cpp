@@ -369,6 +369,184 @@ void addsrc(int i, int READ_STEP, int dim, const int* psrc, int npsrc,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Drucker-Prager plasticity (kernel.cu:43-64 rotate_principal, 1253-1422
+// drprecpc_calc_opt, 1650-1778 drprecpc_app).  Transcendentals (cos, sin, exp)
+// are evaluated in double and rounded to T: the reference's -use_fast_math
+// versions are approximations that no test pins, so a correctly rounded
+// value is used on both sides.
+template <typename T>
+struct PlastC {
+  T fmajor, fminor;
+  T Rz[9], RzT[9];
+};
+
+template <typename T>
+inline void matmul3(const T* a, const T* b, T* c) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      T acc = (T)0;
+      for (int k = 0; k < 3; ++k) acc += a[i * 3 + k] * b[k * 3 + j];
+      c[i * 3 + j] = acc;
+    }
+}
+
+template <typename T>
+inline void rotate_principal(T sigma2, T pfluid, const PlastC<T>& pc, T* ssp) {
+  T ss[9], tmp[9];
+  for (int k = 0; k < 9; ++k) ss[k] = tmp[k] = ssp[k] = (T)0;
+  ss[0] = (sigma2 + pfluid) * pc.fmajor;
+  ss[4] = sigma2 + pfluid;
+  ss[8] = (sigma2 + pfluid) * pc.fminor;
+  matmul3(pc.RzT, ss, tmp);
+  matmul3(tmp, pc.Rz, ssp);
+  ssp[0] -= pfluid;
+  ssp[4] -= pfluid;
+  ssp[8] -= pfluid;
+}
+
+template <typename T>
+inline T t_cos(T x) { return (T)std::cos((double)x); }
+template <typename T>
+inline T t_sin(T x) { return (T)std::sin((double)x); }
+template <typename T>
+inline T t_exp(T x) { return (T)std::exp((double)x); }
+
+// pfluid at depth (srfpos-k)*DH: (depth + DH/2) * 9.81e3, evaluated in double.
+template <typename T>
+inline T pfluid_at(int srfpos, int k, T DH) {
+  const T depth = (T)(srfpos - k) * DH;
+  if (depth > 0) return (T)(((double)depth + (double)DH * 0.5) * 9.81e3);
+  return (T)((double)DH / 2. * 9.81e3);
+}
+
+template <typename T>
+void drprecpc_calc(const T* xx, const T* yy, const T* zz, const T* xy, const T* xz,
+                   const T* yz, const T* mu, const T* d1, const T* sigma2, T* yldfac,
+                   const T* cohes, const T* phi, T* neta, int s_i, int e_i, int s_j, int e_j,
+                   const Grid<T>& g, const PlastC<T>& pc, T Tv_fixed) {
+  const long sl = g.slice, yl = g.yline;
+  const int srfpos = g.nzt + ALIGN - 1;
+#pragma omp parallel for schedule(static)
+  for (int i = s_i; i <= e_i; ++i) {
+    ftz_daz();
+    for (int j = s_j; j <= e_j; ++j) {
+      for (int k = ALIGN; k <= g.nzt + ALIGN - 1; ++k) {
+        const long p = i * sl + j * yl + k;
+        const T pfluid = pfluid_at<T>(srfpos, k, g.DH);
+        T ini[9], ini_ip1[9];
+        rotate_principal(sigma2[p], pfluid, pc, ini);
+        rotate_principal(sigma2[p + sl], pfluid, pc, ini_ip1);
+        const T iixx = 0.5f * (ini[0] + ini_ip1[0]);
+        const T iiyy = 0.5f * (ini[4] + ini_ip1[4]);
+        const T iizz = 0.5f * (ini[8] + ini_ip1[8]);
+        T Sxx = xx[p] + iixx;
+        T Syy = yy[p] + iiyy;
+        T Szz = zz[p] + iizz;
+        T Sxz = 0.25f * (xz[p] + xz[p + sl] + xz[p - 1] + xz[p + sl - 1]) + 0.5f * (ini[2] + ini_ip1[2]);
+        T Syz = 0.25f * (yz[p] + yz[p - yl] + yz[p - 1] + yz[p - yl - 1]) + 0.5f * (ini[5] + ini_ip1[5]);
+        T Sxy = 0.25f * (xy[p] + xy[p + sl] + xy[p - yl] + xy[p + sl - yl]) + 0.5f * (ini[1] + ini_ip1[1]);
+        const T mu_i = mu[p], mu_ip1 = mu[p + sl];
+        T Tv = g.DH / std::sqrt((T)1.0f / (mu_i * d1[p]));
+        if (Tv_fixed > 0) Tv = Tv_fixed;
+        const T Sxxp = Sxx, Syyp = Syy, Szzp = Szz, Sxyp = Sxy, Sxzp = Sxz, Syzp = Syz;
+        const T rphi = phi[p] * (T)0.017453292519943295f;
+        const T sigma_m = (Sxx + Syy + Szz) / (T)3.0f;
+        const T SDxx = Sxx - sigma_m, SDyy = Syy - sigma_m, SDzz = Szz - sigma_m;
+        const T secinv = 0.5f * (SDxx * SDxx + SDyy * SDyy + SDzz * SDzz) + Sxz * Sxz + Sxy * Sxy + Syz * Syz;
+        const T sqrtSecinv = std::sqrt(secinv);
+        const T taulim2 = cohes[p] * t_cos(rphi) - (sigma_m + pfluid) * t_sin(rphi);
+        const T taulim = taulim2 > 0.0f ? taulim2 : (T)0.0f;
+        T iyldfac;
+        if (sqrtSecinv > taulim) {
+          iyldfac = taulim / sqrtSecinv + ((T)1.0f - taulim / sqrtSecinv) * t_exp(-g.DT / Tv);
+          Sxx = SDxx * iyldfac + sigma_m;
+          Syy = SDyy * iyldfac + sigma_m;
+          Szz = SDzz * iyldfac + sigma_m;
+          Sxz = Sxz * iyldfac;
+          Sxy = Sxy * iyldfac;
+          Syz = Syz * iyldfac;
+        } else {
+          iyldfac = 1.0f;
+        }
+        yldfac[p] = iyldfac;
+        if (iyldfac < 1.0f) {
+          const T xm = 2.0f / (mu_i + mu_ip1);
+          const T dxx = (Sxx - Sxxp) / xm, dyy = (Syy - Syyp) / xm, dzz = (Szz - Szzp) / xm;
+          const T dxy = (Sxy - Sxyp) / xm, dxz = (Sxz - Sxzp) / xm, dyz = (Syz - Syzp) / xm;
+          neta[p] = neta[p] + std::sqrt(0.5f * (dxx * dxx + dyy * dyy + dzz * dzz) +
+                                        2.0f * (dxy * dxy + dxz * dxz + dyz * dyz));
+        } else {
+          yldfac[p] = 1.0f;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+void drprecpc_app(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz, const T* sigma2,
+                  const T* yldfac, int s_i, int e_i, int s_j, int e_j, const Grid<T>& g,
+                  const PlastC<T>& pc) {
+  const long sl = g.slice, yl = g.yline;
+  const int srfpos = g.nzt + ALIGN - 1;
+#pragma omp parallel for schedule(static)
+  for (int i = s_i; i <= e_i; ++i) {
+    ftz_daz();
+    for (int j = s_j; j <= e_j; ++j) {
+      for (int k = ALIGN; k <= g.nzt + ALIGN - 1; ++k) {
+        const long p = i * sl + j * yl + k;
+        const long im1 = p - sl, ip1 = p + sl, jp1 = p + yl, kp1 = p + 1;
+        const T pf = pfluid_at<T>(srfpos, k, g.DH);
+        const T pf1 = pfluid_at<T>(srfpos, k - 1, g.DH);  // depth of k+1 level
+        T ini[9], inb[9], t3[9], t4[9], t5[9], t6[9], t7[9], t8[9];
+        if (yldfac[p] < 1.) {
+          rotate_principal(sigma2[p], pf, pc, ini);
+          rotate_principal(sigma2[ip1], pf1, pc, inb);
+          const T iixx = (T)(0.5 * (double)(ini[0] + inb[0]));
+          const T iiyy = (T)(0.5 * (double)(ini[4] + inb[4]));
+          const T iizz = (T)(0.5 * (double)(ini[8] + inb[8]));
+          const T Sxx = xx[p] + iixx, Syy = yy[p] + iiyy, Szz = zz[p] + iizz;
+          const T sigma_m = (T)((double)(Sxx + Syy + Szz) / 3.);
+          const T SDxx = xx[p] + iixx - sigma_m;
+          const T SDyy = yy[p] + iiyy - sigma_m;
+          const T SDzz = zz[p] + iizz - sigma_m;
+          xx[p] = SDxx * yldfac[p] + sigma_m - iixx;
+          yy[p] = SDyy * yldfac[p] + sigma_m - iiyy;
+          zz[p] = SDzz * yldfac[p] + sigma_m - iizz;
+        }
+        T iy = (T)(0.25 * (double)(yldfac[im1] + yldfac[p] + yldfac[im1 + 1] + yldfac[kp1]));
+        if (iy < 1.) {
+          rotate_principal(sigma2[p], pf, pc, ini);
+          rotate_principal(sigma2[kp1], pf1, pc, inb);
+          const T iist = (T)(0.5 * (double)(ini[2] + inb[2]));
+          xz[p] = (xz[p] + iist) * iy - iist;
+        }
+        iy = (T)(0.25 * (double)(yldfac[p] + yldfac[jp1] + yldfac[jp1 + 1] + yldfac[kp1]));
+        if (iy < 1.) {
+          rotate_principal(sigma2[p], pf, pc, ini);
+          rotate_principal(sigma2[ip1], pf, pc, inb);
+          rotate_principal(sigma2[kp1], pf1, pc, t3);
+          rotate_principal(sigma2[ip1 + 1], pf1, pc, t4);
+          rotate_principal(sigma2[jp1], pf, pc, t5);
+          rotate_principal(sigma2[ip1 + yl], pf, pc, t6);
+          rotate_principal(sigma2[jp1 + 1], pf1, pc, t7);
+          rotate_principal(sigma2[ip1 + yl + 1], pf1, pc, t8);
+          const T iist = (T)(0.125 * (double)(ini[5] + inb[5] + t3[5] + t4[5] + t5[5] + t6[5] + t7[5] + t8[5]));
+          yz[p] = (yz[p] + iist) * iy - iist;
+        }
+        iy = (T)(0.25 * (double)(yldfac[p] + yldfac[jp1] + yldfac[im1] + yldfac[im1 + yl]));
+        if (iy < 1.) {
+          rotate_principal(sigma2[p], pf, pc, ini);
+          rotate_principal(sigma2[jp1], pf, pc, inb);
+          const T iist = (T)(0.5 * (double)(ini[1] + inb[1]));
+          xy[p] = (xy[p] + iist) * iy - iist;
+        }
+      }
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -432,5 +610,53 @@ AWP_ORACLE_FSTR(f64, double)
   }
 AWP_ORACLE_ADDSRC(f32, float)
 AWP_ORACLE_ADDSRC(f64, double)
+
+#define AWP_ORACLE_PLAST(SUF, T)                                                        \
+  void awp_oracle_drprecpc_calc_##SUF(const T* xx, const T* yy, const T* zz, const T* xy,  \
+                                      const T* xz, const T* yz, const T* mu, const T* d1,  \
+                                      const T* sigma2, T* yldfac, const T* cohes,          \
+                                      const T* phi, T* neta, int nxt, int nyt, int nzt,    \
+                                      double DH, double DT, int s_i, int e_i, int s_j,     \
+                                      int e_j, const double* pl, double Tv_fixed) {        \
+    ftz_daz();                                                                          \
+    const Grid<T> g = make_grid<T>(DH, DT, nxt, nyt, nzt);                              \
+    PlastC<T> pc;                                                                       \
+    pc.fmajor = (T)pl[0];                                                               \
+    pc.fminor = (T)pl[1];                                                               \
+    for (int q = 0; q < 9; ++q) {                                                       \
+      pc.Rz[q] = (T)pl[2 + q];                                                          \
+      pc.RzT[q] = (T)pl[11 + q];                                                        \
+    }                                                                                   \
+    drprecpc_calc<T>(xx, yy, zz, xy, xz, yz, mu, d1, sigma2, yldfac, cohes, phi, neta,   \
+                     s_i, e_i, s_j, e_j, g, pc, (T)Tv_fixed);                           \
+  }                                                                                     \
+  void awp_oracle_drprecpc_app_##SUF(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz,          \
+                                     const T* sigma2, const T* yldfac, int nxt, int nyt,   \
+                                     int nzt, double DH, int s_i, int e_i, int s_j,        \
+                                     int e_j, const double* pl) {                          \
+    ftz_daz();                                                                          \
+    const Grid<T> g = make_grid<T>(DH, 1.0, nxt, nyt, nzt);                             \
+    PlastC<T> pc;                                                                       \
+    pc.fmajor = (T)pl[0];                                                               \
+    pc.fminor = (T)pl[1];                                                               \
+    for (int q = 0; q < 9; ++q) {                                                       \
+      pc.Rz[q] = (T)pl[2 + q];                                                          \
+      pc.RzT[q] = (T)pl[11 + q];                                                        \
+    }                                                                                   \
+    drprecpc_app<T>(xx, yy, zz, xy, xz, yz, sigma2, yldfac, s_i, e_i, s_j, e_j, g, pc);  \
+  }
+AWP_ORACLE_PLAST(f32, float)
+AWP_ORACLE_PLAST(f64, double)
+
+void awp_oracle_rotate_principal_f64(double sigma2, double pfluid, const double* pl, double* out9) {
+  PlastC<double> pc;
+  pc.fmajor = pl[0];
+  pc.fminor = pl[1];
+  for (int q = 0; q < 9; ++q) {
+    pc.Rz[q] = pl[2 + q];
+    pc.RzT[q] = pl[11 + q];
+  }
+  rotate_principal<double>(sigma2, pfluid, pc, out9);
+}
 
 }  // extern "C"

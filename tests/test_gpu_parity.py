@@ -221,3 +221,75 @@ def test_velbuffer_and_receivers(gpu):
         assert o[n, 0] == st["u1"][i, j, k]
         assert o[n, 1] == st["v1"][i, j, k]
         assert o[n, 2] == st["w1"][i, j, k - 1]
+
+
+def _plastic_state(dims, seed):
+    from awp_amd.plasticity import lithostatic_sigma2
+    st, med = random_state(dims, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    sh = dims.shape
+    med["sigma2"] = lithostatic_sigma2(dims, 75.0, med["d1"].astype(np.float64)).astype(np.float32)
+    med["cohes"] = rng.uniform(0.0, 3.0e6, sh).astype(np.float32)
+    med["phi"] = rng.uniform(15.0, 40.0, sh).astype(np.float32)
+    st["neta"] = rng.uniform(0.0, 1e-3, sh).astype(np.float32)
+    y = rng.uniform(0.3, 1.0, sh)
+    y[rng.uniform(size=sh) < 0.3] = 1.0
+    st["yldfac"] = y.astype(np.float32)
+    return st, med
+
+
+@pytest.mark.parametrize("Tv", [0.0, 0.01])
+def test_drprecpc_calc_bit_exact(gpu, oracle_lib, Tv):
+    from awp_amd.plasticity import PlasticParams
+    torch, lib, _lib = gpu
+    dims = Dims(19, 22, 70)
+    st, med = _plastic_state(dims, 21)
+    pp = PlasticParams(fmajor=1.2, fminor=0.8, strike=(30.0, 0.0, 120.0))
+    Rz, RzT = pp.rotation()
+    F, I = C.c_float, C.c_int
+    lib.SetDeviceConstValue((F * 1)(75.0), F(0.004), (I * 1)(dims.nxt), (I * 1)(dims.nyt),
+                            (I * 1)(dims.nzt), 1, pp.fmajor, pp.fminor, (F * 9)(*Rz.tolist()),
+                            (F * 9)(*RzT.tolist()))
+    dev = _dev(torch, {**st, **med})
+    xs, xe, ys, ye = 10, dims.nxt + 9, 11, dims.nyt + 8
+    args = [_p(dev[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "mu", "d1", "sigma2",
+                                   "yldfac", "cohes", "phi", "neta")]
+    if Tv > 0:
+        lib.awp_drprecpc_calc_tv(*args, dims.nzt, xs, xe, ys, ye, _stream(torch), 0, Tv)
+    else:
+        lib.drprecpc_calc_H_opt(*args, dims.nzt, xs, xe, ys, ye, _stream(torch), 0)
+    _lib.check()
+    host = {k: v.copy() for k, v in st.items()}
+    pl = np.ascontiguousarray(pp.packed())
+    oracle_lib.load().awp_oracle_drprecpc_calc_f32(
+        *(_np(host[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")), _np(med["mu"]),
+        _np(med["d1"]), _np(med["sigma2"]), _np(host["yldfac"]), _np(med["cohes"]),
+        _np(med["phi"]), _np(host["neta"]), dims.nxt, dims.nyt, dims.nzt, 75.0, 0.004, xs, xe,
+        ys, ye, _np(pl), Tv)
+    assert (host["yldfac"] < 1).sum() > 100, "too few yielding cells for a meaningful test"
+    _compare(torch, dev, host, ("yldfac", "neta"), "drprecpc_calc")
+
+
+def test_drprecpc_app_bit_exact(gpu, oracle_lib):
+    from awp_amd.plasticity import PlasticParams
+    torch, lib, _lib = gpu
+    dims = Dims(21, 18, 66)
+    st, med = _plastic_state(dims, 33)
+    pp = PlasticParams(fmajor=1.3, fminor=0.75, strike=(15.0, 0.0, 105.0))
+    Rz, RzT = pp.rotation()
+    F, I = C.c_float, C.c_int
+    lib.SetDeviceConstValue((F * 1)(75.0), F(0.004), (I * 1)(dims.nxt), (I * 1)(dims.nyt),
+                            (I * 1)(dims.nzt), 1, pp.fmajor, pp.fminor, (F * 9)(*Rz.tolist()),
+                            (F * 9)(*RzT.tolist()))
+    dev = _dev(torch, {**st, **med})
+    xs, xe, ys, ye = 10, dims.nxt + 9, 10, dims.nyt + 9
+    lib.drprecpc_app_H(*(_p(dev[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "mu",
+                                               "sigma2", "yldfac")), dims.nzt, xs, xe, ys, ye,
+                       _stream(torch), 0)
+    _lib.check()
+    host = {k: v.copy() for k, v in st.items()}
+    pl = np.ascontiguousarray(pp.packed())
+    oracle_lib.load().awp_oracle_drprecpc_app_f32(
+        *(_np(host[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")), _np(med["sigma2"]),
+        _np(host["yldfac"]), dims.nxt, dims.nyt, dims.nzt, 75.0, xs, xe, ys, ye, _np(pl))
+    _compare(torch, dev, host, ("xx", "yy", "zz", "xy", "xz", "yz"), "drprecpc_app")
