@@ -34,6 +34,10 @@ SIGNATURES: dict[str, list] = {
     "drprecpc_calc_H_opt": [_fp] * 13 + [_i] * 5 + [_s, _i],
     "awp_drprecpc_calc_tv": [_fp] * 13 + [_i] * 5 + [_s, _i, _f],
     "drprecpc_app_H": [_fp] * 9 + [_i] * 5 + [_s, _i],
+    "dvelc2_H": [_fp] * 13 + [_i, _i, _s, _i],
+    "dstrqc2_H": [_fp] * 22 + [_i, _i, _s] + [_fp] * 5 + [_i] * 5,
+    "intp3d_H": [_fp] * 18 + [_i, _i, _i, _s, _i],
+    "swap_H": [_fp] * 18 + [_i, _i] + [_fp] * 4 + [_i, _s, _i],
 }
 RESTYPES = {
     "awp_last_error": _i,

@@ -227,3 +227,68 @@ def plastic(nx: int = 64, ny: int = 64, nz: int = 48, nt: int = 100, DH: float =
                  dict(cohesion=cohesion, friction=friction, Tv=Tv, M0=M0))
     w.params["plastic_fn"] = mk_plastic
     return w
+
+
+def layered_random_lattice(nx: int, ny: int, zmax: float, h: float, seed: int,
+                           std: float = 0.05, corr: float = 500.0, nu: float = 0.5):
+    """A seeded von Karman perturbation on a lattice of spacing h covering
+    nx*h x ny*h x zmax, and the model function (x, y, depth) -> vp, vs, rho,
+    Qp, Qs of the layered + random medium sampled from it (trilinear), for
+    meshes whose grids have different spacings."""
+    from .dm import sample_lattice
+    nz = int(np.ceil(zmax / h))
+    F = von_karman((nx, ny, nz), h, corr, nu, std, seed)
+
+    def model(x, y, z):
+        pert = sample_lattice(F, h, x, y, z).astype(np.float64)
+        vs1d = layered_vs(z, zmax)
+        vs = vs1d[None, None, :] * (1.0 + pert)
+        vp = 1.73 * vs
+        rho = 310.0 * vp ** 0.25
+        Qs = 50.0 * vs / 1000.0
+        return vp, vs, rho, 2.0 * Qs, Qs
+
+    return model
+
+
+def dm_plastic(nx: int = 256, ny: int = 256, nz: int = 192, nz_fine: int = 96,
+               DH: float = 75.0, DT: float = 0.0025, nt: int = 3000, seed: int = 13,
+               nsub: int = 200, M0: float = 4.0e19, cohesion: float = 5.0e6,
+               friction: float = 0.75, Tv: float = 0.01, nrec: int = 8, f0: float = 1.0,
+               ND: int = 20, src_depth: int | None = None, src_grid: int = 1) -> Workload:
+    """Discontinuous mesh + plasticity: a fine top grid (3nx x 3ny x nz_fine,
+    spacing DH/3) over the coarse grid (nx x ny x nz, spacing DH), layered +
+    random medium with Q, Drucker-Prager plasticity (cohesion, friction,
+    viscoplastic relaxation Tv) on both grids, and a line of nsub double-couple
+    subfaults along x in grid src_grid (default the coarse grid) at model depth
+    src_depth of that grid.  medium() returns the two media (fine first);
+    sources is the per-grid list."""
+    from .dm import build_stack, depth_offsets, stack_dims
+    from .plasticity import PlasticParams, constant_medium, protect_sources
+    dims = stack_dims(nx, ny, (nz_fine, nz))
+    DHs = [DH / 3.0, DH]
+    zmax = depth_offsets(dims, DHs)[1] + nz * DH
+    zsrc = src_depth if src_depth is not None else max(dims[src_grid].nzt // 2, 1)
+    sources = [None, None]
+    sources[src_grid] = line_source(dims[src_grid], nsub, nt, DT, f0, M0, seed, depth=zsrc)
+    rec = seeded_surface_receivers(dims[0], nrec, seed + 1, margin=min(20 * 3, dims[0].nxt // 4))
+
+    def mk():
+        model = layered_random_lattice(nx, ny, zmax, DH, seed)
+        return build_stack(dims, DHs[0], DT, model, ND=ND)
+
+    def mk_plastic(media):
+        out = []
+        for p, med in enumerate(media):
+            pm = constant_medium(med.dims, med.DH, med.d1, cohesion, friction,
+                                 PlasticParams(fmajor=1.0, fminor=1.0, Tv=Tv))
+            if sources[p] is not None:
+                protect_sources(pm, med.dims, sources[p].psrc(med.dims))
+            out.append(pm)
+        return out
+
+    w = Workload("dm_plastic_%dx%dx%d_%dx%dx%d" % (dims[0].nxt, dims[0].nyt, nz_fine, nx, ny, nz),
+                 dims[0], DHs[0], DT, nt, mk, sources, rec,
+                 dict(cohesion=cohesion, friction=friction, Tv=Tv, M0=M0, dims=dims, DH=DHs))
+    w.params["plastic_fn"] = mk_plastic
+    return w

@@ -1,0 +1,493 @@
+// Discontinuous mesh: the kernels that couple a fine grid (index d) to the
+// coarse grid below it (index d+1, 3x the spacing, 1/3 the lateral size).
+//
+//   dvelc2_H   2nd-order velocity at k = align+2 of a fine grid
+//              (reference dvelc2, src/awp/kernel.cu:2289-2356)
+//   dstrqc2_H  2nd-order stress + Q at k = align+2 of a fine grid
+//              (dstrqc2, kernel.cu:2564-2784)
+//   intp3d_H   coarse level nzt+align-3 -> fine level align+1, 4x4 bilinear
+//              (intp3d, kernel.cu:2359-2561, 2786-2804)
+//   swap_H     fine levels align+1..align+8 -> coarse levels nzt+align-2 and
+//              nzt+align-1, triangular 5x5x5 weighted average
+//              (swap3, kernel.cu:3032-3234)
+//
+// All four work on planes (O(nx*ny) per call); k is not a loop dimension, so
+// the lane map is one lane per (i, j) column with j fastest.  swap3 reads
+// 125 fine values per output; it stages the 8 fine levels of a tile into LDS
+// once per field instead of re-reading them through the caches.
+#include "awp_common.h"
+#include "../../include/awp_hip.h"
+
+namespace {
+
+struct PlaneLaunch {
+  int i0, i1, j0, j1;  // inclusive column range
+  int yline;
+  long long slice;
+  float dth, dh1, DT;
+};
+
+// ---------------------------------------------------------------------------
+// dvelc2: w from the xz/yz/zz differences, u and v from 2nd-order stress
+// differences; the density means are float sums scaled by the double 0.25.
+__global__ __launch_bounds__(256) void dvelc2_kernel(
+    float* __restrict__ u1, float* __restrict__ v1, float* __restrict__ w1,
+    const float* __restrict__ xx, const float* __restrict__ yy, const float* __restrict__ zz,
+    const float* __restrict__ xy, const float* __restrict__ xz, const float* __restrict__ yz,
+    const float* __restrict__ dcrjx, const float* __restrict__ dcrjy,
+    const float* __restrict__ dcrjz, const float* __restrict__ d1, const PlaneLaunch L) {
+  const int j = L.j0 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int i = L.i0 + (int)(blockIdx.y * 4 + threadIdx.y);
+  if (j > L.j1 || i > L.i1) return;
+  const int k = AWP_ALIGN + 2;
+  const long long S = L.slice, Y = L.yline;
+  const long long p = (long long)i * S + (long long)j * Y + k;
+  float f_d3 = (float)(0.25 * (double)(d1[p] + d1[p + S] + d1[p - Y] + d1[p + S - Y]));
+  f_d3 = L.dth / f_d3;
+  const float f_dcrj = dcrjz[k] * dcrjx[i] * dcrjy[j];
+  w1[p] = (w1[p] + f_d3 * ((xz[p + S] - xz[p]) + (yz[p] - yz[p - Y]) + (zz[p + 1] - zz[p]))) *
+          f_dcrj;
+  float f_d1 = (float)(0.25 * (double)(d1[p] + d1[p - Y] + d1[p - 1] + d1[p - Y - 1]));
+  float f_d2 = (float)(0.25 * (double)(d1[p] + d1[p + S] + d1[p - 1] + d1[p + S - 1]));
+  f_d1 = L.dth / f_d1;
+  f_d2 = L.dth / f_d2;
+  u1[p] = (u1[p] + f_d1 * ((xx[p] - xx[p - S]) + (xy[p] - xy[p - Y]) + (xz[p] - xz[p - 1]))) *
+          f_dcrj;
+  v1[p] = (v1[p] + f_d2 * ((xy[p + S] - xy[p]) + (yy[p + Y] - yy[p]) + (yz[p] - yz[p - 1]))) *
+          f_dcrj;
+}
+
+// ---------------------------------------------------------------------------
+// dstrqc2.  Q weights with this kernel's literal types: the qpa test is single
+// precision and the else branches carry no factor 2 (kernel.cu:2638-2683).
+__device__ __forceinline__ float dm_qweight(const float* coeff, int ww, float wwo, float q) {
+  float w;
+  if (1. / ((double)q * 2.0) <= 200.0)
+    w = (float)((double)coeff[ww * 2 - 2] * (2. * (double)q) * (2. * (double)q) +
+                (double)coeff[ww * 2 - 1] * (2. * (double)q));
+  else
+    w = wwo * q;
+  return w / wwo;
+}
+
+__global__ __launch_bounds__(256) void dstrqc2_kernel(
+    float* __restrict__ xx, float* __restrict__ yy, float* __restrict__ zz,
+    float* __restrict__ xy, float* __restrict__ xz, float* __restrict__ yz,
+    float* __restrict__ r1, float* __restrict__ r2, float* __restrict__ r3,
+    float* __restrict__ r4, float* __restrict__ r5, float* __restrict__ r6,
+    const float* __restrict__ u1, const float* __restrict__ v1, const float* __restrict__ w1,
+    const float* __restrict__ lam, const float* __restrict__ mu, const float* __restrict__ qp,
+    const float* __restrict__ qs, const float* __restrict__ dcrjx,
+    const float* __restrict__ dcrjy, const float* __restrict__ dcrjz,
+    const float* __restrict__ coeff, const float* __restrict__ vx1p,
+    const float* __restrict__ vx2p, const int* __restrict__ wwp,
+    const float* __restrict__ wwop, const PlaneLaunch L) {
+  const int j = L.j0 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int i = L.i0 + (int)(blockIdx.y * 4 + threadIdx.y);
+  if (j > L.j1 || i > L.i1) return;
+  const int k = AWP_ALIGN + 2;
+  const long long S = L.slice, Y = L.yline;
+  const long long p = (long long)i * S + (long long)j * Y + k;
+  const long long ip1 = p + S, jm1 = p - Y, ijk = p + S - Y, km1 = p - 1, ik1 = p + S - 1,
+                  jk1 = p - Y - 1, ijk1 = p + S - Y - 1;
+  const float DT = L.DT, dth = L.dth, dh1 = L.dh1;
+  const float f_dcrj = dcrjz[k] * dcrjx[i] * dcrjy[j];
+  const float f_vx1 = vx1p[p], f_vx2 = vx2p[p], f_wwo = wwop[p];
+  const int f_ww = wwp[p];
+  float xl = (float)(8.0 / (double)(lam[p] + lam[ip1] + lam[jm1] + lam[ijk] + lam[km1] +
+                                    lam[ik1] + lam[jk1] + lam[ijk1]));
+  float xm = (float)(16.0 / (double)(mu[p] + mu[ip1] + mu[jm1] + mu[ijk] + mu[km1] + mu[ik1] +
+                                     mu[jk1] + mu[ijk1]));
+  float xmu1 = (float)(2.0 / (double)(mu[p] + mu[km1]));
+  float xmu2 = (float)(2.0 / (double)(mu[p] + mu[jm1]));
+  float xmu3 = (float)(2.0 / (double)(mu[p] + mu[ip1]));
+  xl = xl + xm;
+  float qpa = (float)(0.0625 * (double)(qp[p] + qp[ip1] + qp[jm1] + qp[ijk] + qp[km1] +
+                                        qp[ik1] + qp[jk1] + qp[ijk1]));
+  float qpaw;
+  if (1.0f / (qpa * 2.0f) <= 200.0f)
+    qpaw = (float)((double)coeff[f_ww * 2 - 2] * (2. * (double)qpa) * (2. * (double)qpa) +
+                   (double)coeff[f_ww * 2 - 1] * (2. * (double)qpa));
+  else
+    qpaw = f_wwo * qpa;
+  qpaw = qpaw / f_wwo;
+  float h = 0.0625f * (qs[p] + qs[ip1] + qs[jm1] + qs[ijk] + qs[km1] + qs[ik1] + qs[jk1] +
+                       qs[ijk1]);
+  const float hw = dm_qweight(coeff, f_ww, f_wwo, h);
+  float h1 = (float)(0.25 * (double)(qs[p] + qs[km1]));
+  const float h1w = dm_qweight(coeff, f_ww, f_wwo, h1);
+  float h2 = (float)(0.250 * (double)(qs[p] + qs[jm1]));
+  const float h2w = dm_qweight(coeff, f_ww, f_wwo, h2);
+  float h3 = (float)(0.250 * (double)(qs[p] + qs[ip1]));
+  const float h3w = dm_qweight(coeff, f_ww, f_wwo, h3);
+  h = -xm * hw * dh1;
+  h1 = -xmu1 * h1w * dh1;
+  h2 = -xmu2 * h2w * dh1;
+  h3 = -xmu3 * h3w * dh1;
+  qpa = -qpaw * xl * dh1;
+  xm = xm * dth;
+  xmu1 = xmu1 * dth;
+  xmu2 = xmu2 * dth;
+  xmu3 = xmu3 * dth;
+  xl = xl * dth;
+  h = h * f_vx1;
+  h1 = h1 * f_vx1;
+  h2 = h2 * f_vx1;
+  h3 = h3 * f_vx1;
+  qpa = qpa * f_vx1;
+  xm = xm + DT * h;
+  xmu1 = xmu1 + DT * h1;
+  xmu2 = xmu2 + DT * h2;
+  xmu3 = xmu3 + DT * h3;
+  const float vx1 = DT * (1 + f_vx2 * f_vx1);
+  float f_r, f_rtmp, t, vs1, vs2, vs3;
+
+  vs1 = u1[p + 1] - u1[p];
+  vs2 = w1[p] - w1[p - S];
+  f_r = r5[p];
+  f_rtmp = h2 * (vs1 + vs2);
+  t = xz[p] + xmu2 * (vs1 + vs2) + vx1 * f_r;
+  r5[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+  f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+  xz[p] = (t + DT * f_rtmp) * f_dcrj;
+
+  vs1 = v1[p + 1] - v1[p];
+  vs2 = w1[p + Y] - w1[p];
+  f_r = r6[p];
+  f_rtmp = h3 * (vs1 + vs2);
+  t = yz[p] + xmu3 * (vs1 + vs2) + vx1 * f_r;
+  r6[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+  f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+  yz[p] = (t + DT * f_rtmp) * f_dcrj;
+
+  vs1 = u1[p + S] - u1[p];
+  vs2 = v1[p] - v1[p - Y];
+  vs3 = w1[p] - w1[p - 1];
+  float tmp = xl * (vs1 + vs2 + vs3);
+  const float a1 = qpa * (vs1 + vs2 + vs3);
+  tmp = tmp + DT * a1;
+
+  f_r = r1[p];
+  f_rtmp = -h * (vs2 + vs3) + a1;
+  t = xx[p] + tmp - xm * (vs2 + vs3) + vx1 * f_r;
+  r1[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+  f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+  xx[p] = (t + DT * f_rtmp) * f_dcrj;
+
+  // yy and zz take f_dcrj twice, as the reference (kernel.cu:2727-2740)
+  f_r = r2[p];
+  f_rtmp = -h * (vs1 + vs3) + a1;
+  t = (yy[p] + tmp - xm * (vs1 + vs3) + vx1 * f_r) * f_dcrj;
+  r2[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+  f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+  yy[p] = (t + DT * f_rtmp) * f_dcrj;
+
+  f_r = r3[p];
+  f_rtmp = -h * (vs1 + vs2) + a1;
+  t = (zz[p] + tmp - xm * (vs1 + vs2) + vx1 * f_r) * f_dcrj;
+  r3[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+  f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+  zz[p] = (t + DT * f_rtmp) * f_dcrj;
+
+  vs1 = u1[p + Y] - u1[p];
+  vs2 = v1[p] - v1[p - S];
+  f_r = r4[p];
+  f_rtmp = h1 * (vs1 + vs2);
+  t = xy[p] + xmu1 * (vs1 + vs2) + vx1 * f_r;
+  r4[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+  f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+  xy[p] = (t + DT * f_rtmp) * f_dcrj;
+}
+
+// ---------------------------------------------------------------------------
+// Field order of the nine-pointer packs below: u1 v1 w1 xx yy zz xy xz yz
+// (the swap-buffer order, include/awp/pmcl3d_cons.h:80-88).
+struct Fields9 {
+  float* f[9];
+};
+
+struct Coupling {
+  int ci0, ci1, cj0, cj1;  // coarse (i, j) ranges, inclusive
+  int kc;                 // coarse level (intp3d) or top coarse level (swap)
+  int ylc, ylf;
+  long long slc, slf;
+  int nxf_pad, nyf_pad;   // fine padded extents nxt+4+ngsl2, nyt+4+ngsl2
+};
+
+// intp3d: per coarse column four corner values per field, 16 fine writes.
+// Fine offsets (added to (ii-1, jj-1)) per field, kernel.cu:2397-2557.
+__constant__ int c_intp_ox[9] = {0, 1, 1, 1, 1, 1, 0, 0, 1};
+__constant__ int c_intp_oy[9] = {0, 1, 0, 0, 0, 0, 1, 0, 1};
+
+__global__ __launch_bounds__(256) void intp3d_kernel(const Fields9 L9, Fields9 H9,
+                                                     const Coupling C) {
+  const int j = C.cj0 + (int)(blockIdx.x * 64 + threadIdx.x);
+  const int i = C.ci0 + (int)(blockIdx.y * 4 + threadIdx.y);
+  if (j > C.cj1 || i > C.ci1) return;
+  const float w[4] = {1.0f, (float)(2. / 3.), (float)(1. / 3.), 0.0f};
+  const long long pl = (long long)i * C.slc + (long long)j * C.ylc + C.kc;
+  const int ih = (i - 2 - AWP_NGSL) * 3 + 2 + AWP_NGSL;
+  const int jh = (j - 2 - AWP_NGSL) * 3 + 2 + AWP_NGSL;
+  const long long ph = (long long)ih * C.slf + (long long)jh * C.ylf + AWP_ALIGN + 1;
+#pragma unroll
+  for (int f = 0; f < 9; ++f) {
+    const float* a = L9.f[f];
+    const float v0 = a[pl], v1 = a[pl + C.slc], v2 = a[pl + C.ylc], v3 = a[pl + C.slc + C.ylc];
+    float* h = H9.f[f];
+    const long long base = ph + (long long)c_intp_ox[f] * C.slf + (long long)c_intp_oy[f] * C.ylf;
+#pragma unroll
+    for (int jj = 1; jj <= 4; ++jj)
+#pragma unroll
+      for (int ii = 1; ii <= 4; ++ii)
+        h[base + (long long)(ii - 1) * C.slf + (long long)(jj - 1) * C.ylf] =
+            v0 * w[ii - 1] * w[jj - 1] + v1 * w[4 - ii] * w[jj - 1] + v2 * w[ii - 1] * w[4 - jj] +
+            v3 * w[4 - ii] * w[4 - jj];
+  }
+}
+
+// swap3.  One wave per tile of SW_TI x SW_TJ coarse columns.  For each field
+// the fine slab (3*TI+3) x (3*TJ+3) columns x 8 levels (align+1..align+8) is
+// copied to LDS with 32-byte column segments, then every lane forms its
+// outputs for that field from LDS.  Sums follow the reference exactly:
+// loops jj, ii, kk; term = ((1/729 * wi) * wj) * wk * v in double; the
+// running sum is rounded to float after every term.
+constexpr int SW_TI = 4, SW_TJ = 16;
+constexpr int SW_FI = 3 * SW_TI + 3, SW_FJ = 3 * SW_TJ + 3, SW_NL = 8;
+constexpr int SW_CP = SW_NL + 1;  // column pitch in LDS (odd: spreads banks)
+constexpr int SW_WWL = 2;         // include/awp/pmcl3d_cons.h:92
+
+struct SwapPass {
+  int kc;      // 0: coarse level nzt+align-2, 1: nzt+align-1
+  int kh;      // fine centre level minus (align+1)
+  int oi, oj;  // fine base offsets: ih = 3*(i-10) + oi + ngsl
+};
+
+// Per field: number of passes and the passes (kernel.cu:3071-3213).
+__constant__ int c_swap_np[9] = {2, 2, 1, 2, 2, 2, 2, 1, 1};
+__constant__ SwapPass c_swap_pass[9][2] = {
+    {{0, 2, 2, 2}, {1, 5, 2, 2}},  // u1: kh align+3, align+6
+    {{0, 2, 3, 3}, {1, 5, 3, 3}},  // v1
+    {{0, 3, 3, 2}, {0, 3, 3, 2}},  // w1: kh align+4
+    {{0, 2, 3, 2}, {1, 5, 3, 2}},  // xx
+    {{0, 2, 3, 2}, {1, 5, 3, 2}},  // yy
+    {{0, 2, 3, 2}, {1, 5, 3, 2}},  // zz
+    {{0, 2, 2, 3}, {1, 5, 2, 3}},  // xy
+    {{0, 3, 2, 2}, {0, 3, 2, 2}},  // xz: kh align+4
+    {{0, 3, 3, 3}, {0, 3, 3, 3}},  // yz: kh align+4
+};
+
+__global__ __launch_bounds__(64) void swap3_kernel(Fields9 L9, const Fields9 H9,
+                                                   const Coupling C, int ntj) {
+  __shared__ float slab[SW_FI * SW_FJ * SW_CP];
+  __shared__ double wt[125];
+  const int lane = (int)threadIdx.x;
+  const int bi = (int)blockIdx.x / ntj, bj = (int)blockIdx.x % ntj;
+  const int ti = lane / SW_TJ, tj = lane % SW_TJ;
+  const int i0 = C.ci0 + bi * SW_TI, j0 = C.cj0 + bj * SW_TJ;
+  const int i = i0 + ti, j = j0 + tj;
+  const bool active = (i <= C.ci1 && j <= C.cj1);
+  // fine row/column of slab index 0: 3*(i0-2-ngsl) + 2 + ngsl - WWL
+  const int fi0 = (i0 - 2 - AWP_NGSL) * 3 + AWP_NGSL;
+  const int fj0 = (j0 - 2 - AWP_NGSL) * 3 + AWP_NGSL;
+
+  // weights ((1/ttl * (3-|ii|)) * (3-|jj|)) * (3-|kk|), n = (jj*5 + ii)*5 + kk;
+  // ttl = sum of the unnormalised products (kernel.cu:3053-3062), exact 729
+  for (int n = lane; n < 125; n += 64) {
+    double ttl = 0.;
+    for (int jj = -SW_WWL; jj <= SW_WWL; ++jj)
+      for (int ii = -SW_WWL; ii <= SW_WWL; ++ii)
+        for (int kk = -SW_WWL; kk <= SW_WWL; ++kk)
+          ttl += (1. + SW_WWL - abs(jj)) * (1. + SW_WWL - abs(ii)) * (1. + SW_WWL - abs(kk));
+    const int jj = n / 25 - SW_WWL, ii = (n / 5) % 5 - SW_WWL, kk = n % 5 - SW_WWL;
+    wt[n] = 1 / ttl * (1. + SW_WWL - abs(ii)) * (1. + SW_WWL - abs(jj)) *
+            (1. + SW_WWL - abs(kk));
+  }
+
+  const long long pc = (long long)i * C.slc + (long long)j * C.ylc + C.kc;
+  for (int f = 0; f < 9; ++f) {
+    __syncthreads();
+    const float* src = H9.f[f];
+    for (int t = lane; t < SW_FI * SW_FJ * SW_NL; t += 64) {
+      const int l = t & (SW_NL - 1);
+      const int col = t >> 3;
+      const int b = col % SW_FJ, a = col / SW_FJ;
+      const int fi = fi0 + a, fj = fj0 + b;
+      if (fi < C.nxf_pad && fj < C.nyf_pad)
+        slab[(a * SW_FJ + b) * SW_CP + l] =
+            src[(long long)fi * C.slf + (long long)fj * C.ylf + AWP_ALIGN + 1 + l];
+    }
+    __syncthreads();
+    if (!active) continue;
+    const int np = c_swap_np[f];
+    for (int q = 0; q < np; ++q) {
+      const SwapPass P = c_swap_pass[f][q];
+      const int a0 = 3 * ti + P.oi - SW_WWL, b0 = 3 * tj + P.oj - SW_WWL, l0 = P.kh - SW_WWL;
+      float sum = 0.0f;
+      int n = 0;
+      for (int jj = 0; jj < 5; ++jj)
+        for (int ii = 0; ii < 5; ++ii) {
+          const float* col = &slab[((a0 + ii) * SW_FJ + (b0 + jj)) * SW_CP + l0];
+#pragma unroll
+          for (int kk = 0; kk < 5; ++kk, ++n)
+            sum = (float)((double)sum + wt[n] * (double)col[kk]);
+        }
+      L9.f[f][pc + P.kc] = sum;
+    }
+  }
+}
+
+bool dm_grid_pair(int fine, const char* W, const AwpGrid** gf, const AwpGrid** gc) {
+  const AwpState& s = awp_state();
+  if (!awp_grid_ok(fine, W) || !awp_grid_ok(fine + 1, W)) return false;
+  *gf = &s.grid[fine];
+  *gc = &s.grid[fine + 1];
+  if ((*gf)->nxt != 3 * (*gc)->nxt || (*gf)->nyt != 3 * (*gc)->nyt) {
+    awp_set_error(AWP_ERR_ARG, "%s: fine grid %d (%d x %d) is not 3x coarse grid %d (%d x %d)",
+                  W, fine, (*gf)->nxt, (*gf)->nyt, fine + 1, (*gc)->nxt, (*gc)->nyt);
+    return false;
+  }
+  if ((*gc)->nzt < 4) {
+    awp_set_error(AWP_ERR_ARG, "%s: coarse grid %d has nzt=%d < 4", W, fine + 1, (*gc)->nzt);
+    return false;
+  }
+  return true;
+}
+
+Coupling make_coupling(const AwpGrid& gf, const AwpGrid& gc) {
+  Coupling C;
+  C.ylc = gc.yline;
+  C.slc = gc.slice;
+  C.ylf = gf.yline;
+  C.slf = gf.slice;
+  C.nxf_pad = gf.nxt + 4 + AWP_NGSL2;
+  C.nyf_pad = gf.nyt + 4 + AWP_NGSL2;
+  return C;
+}
+
+}  // namespace
+
+extern "C" {
+
+void dvelc2_H(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz, float* xy,
+              float* xz, float* yz, float* dcrjx, float* dcrjy, float* dcrjz, float* d_1, int nxt,
+              int nyt, hipStream_t St, int d_i) {
+  const char* W = "dvelc2_H";
+  if (!awp_grid_ok(d_i, W)) return;
+  const AwpGrid& g = awp_state().grid[d_i];
+  if (nxt != g.nxt || nyt != g.nyt) {
+    awp_set_error(AWP_ERR_ARG, "%s: nxt=%d nyt=%d disagree with grid %d (%d, %d)", W, nxt, nyt,
+                  d_i, g.nxt, g.nyt);
+    return;
+  }
+  if (g.nzt < 3) {
+    awp_set_error(AWP_ERR_ARG, "%s: grid %d has nzt=%d < 3", W, d_i, g.nzt);
+    return;
+  }
+  PlaneLaunch L;
+  L.i0 = 2 + AWP_NGSL; L.i1 = nxt + 1 + AWP_NGSL;   // kernel.cu:2301
+  L.j0 = 2 + AWP_NGSL; L.j1 = nyt + 1 + AWP_NGSL;
+  L.yline = g.yline; L.slice = g.slice;
+  L.dth = g.dth; L.dh1 = g.dh1; L.DT = g.DT;
+  dim3 grid(awp_cdiv(L.j1 - L.j0 + 1, 64), awp_cdiv(L.i1 - L.i0 + 1, 4));
+  hipLaunchKernelGGL(dvelc2_kernel, grid, dim3(64, 4), 0, St, u1, v1, w1, xx, yy, zz, xy, xz, yz,
+                     dcrjx, dcrjy, dcrjz, d_1, L);
+  awp_check_launch(W);
+}
+
+void dstrqc2_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
+               float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
+               float* w1, float* lam, float* mu, float* qp, float* qs, float* dcrjx, float* dcrjy,
+               float* dcrjz, int nxt, int nyt, hipStream_t St, float* coeff, float* d_vx1,
+               float* d_vx2, int* d_ww, float* d_wwo, int s_i, int e_i, int s_j, int e_j,
+               int d_i) {
+  const char* W = "dstrqc2_H";
+  if (!awp_grid_ok(d_i, W)) return;
+  const AwpGrid& g = awp_state().grid[d_i];
+  if (nxt != g.nxt || nyt != g.nyt) {
+    awp_set_error(AWP_ERR_ARG, "%s: nxt=%d nyt=%d disagree with grid %d (%d, %d)", W, nxt, nyt,
+                  d_i, g.nxt, g.nyt);
+    return;
+  }
+  if (g.nzt < 3) {
+    awp_set_error(AWP_ERR_ARG, "%s: grid %d has nzt=%d < 3", W, d_i, g.nzt);
+    return;
+  }
+  if (s_i > e_i || s_j > e_j) return;
+  // i-1 .. i+1 and j-1 .. j+1 must stay in the padded grid
+  if (s_i < 2 || e_i + 2 > g.nxt + 3 + AWP_NGSL2 || s_j < 2 || e_j + 2 > g.nyt + 3 + AWP_NGSL2) {
+    awp_set_error(AWP_ERR_ARG, "%s: range i[%d,%d] j[%d,%d] leaves the padded grid (%d x %d)", W,
+                  s_i, e_i, s_j, e_j, g.nxt, g.nyt);
+    return;
+  }
+  PlaneLaunch L;
+  L.i0 = s_i; L.i1 = e_i; L.j0 = s_j; L.j1 = e_j;
+  L.yline = g.yline; L.slice = g.slice;
+  L.dth = g.dth; L.dh1 = g.dh1; L.DT = g.DT;
+  dim3 grid(awp_cdiv(e_j - s_j + 1, 64), awp_cdiv(e_i - s_i + 1, 4));
+  hipLaunchKernelGGL(dstrqc2_kernel, grid, dim3(64, 4), 0, St, xx, yy, zz, xy, xz, yz, r1, r2,
+                     r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, qs, dcrjx, dcrjy, dcrjz, coeff,
+                     d_vx1, d_vx2, d_ww, d_wwo, L);
+  awp_check_launch(W);
+}
+
+void intp3d_H(float* u1l, float* v1l, float* w1l, float* xxl, float* yyl, float* zzl, float* xyl,
+              float* xzl, float* yzl, float* u1h, float* v1h, float* w1h, float* xxh, float* yyh,
+              float* zzh, float* xyh, float* xzh, float* yzh, int nxtl, int nytl, int rank,
+              hipStream_t St, int d_i) {
+  (void)rank;
+  const char* W = "intp3d_H";
+  const AwpGrid *gf, *gc;
+  if (d_i < 1) {
+    awp_set_error(AWP_ERR_ARG, "%s: d_i=%d names the coarse grid and must be >= 1", W, d_i);
+    return;
+  }
+  if (!dm_grid_pair(d_i - 1, W, &gf, &gc)) return;
+  if (nxtl != gc->nxt || nytl != gc->nyt) {
+    awp_set_error(AWP_ERR_ARG, "%s: nxtl=%d nytl=%d disagree with grid %d (%d, %d)", W, nxtl,
+                  nytl, d_i, gc->nxt, gc->nyt);
+    return;
+  }
+  Coupling C = make_coupling(*gf, *gc);
+  C.ci0 = AWP_NGSL; C.ci1 = nxtl + 3 + AWP_NGSL;  // kernel.cu:2379
+  C.cj0 = AWP_NGSL; C.cj1 = nytl + 3 + AWP_NGSL;
+  C.kc = gc->nzt + AWP_ALIGN - 3;
+  Fields9 L9 = {{u1l, v1l, w1l, xxl, yyl, zzl, xyl, xzl, yzl}};
+  Fields9 H9 = {{u1h, v1h, w1h, xxh, yyh, zzh, xyh, xzh, yzh}};
+  dim3 grid(awp_cdiv(C.cj1 - C.cj0 + 1, 64), awp_cdiv(C.ci1 - C.ci0 + 1, 4));
+  hipLaunchKernelGGL(intp3d_kernel, grid, dim3(64, 4), 0, St, L9, H9, C);
+  awp_check_launch(W);
+}
+
+void swap_H(float* xxl, float* yyl, float* zzl, float* xyl, float* xzl, float* yzl, float* u1l,
+            float* v1l, float* w1l, float* xxh, float* yyh, float* zzh, float* xyh, float* xzh,
+            float* yzh, float* u1h, float* v1h, float* w1h, int nxtl, int nytl, float* buf_L,
+            float* buf_R, float* buf_F, float* buf_B, int rank, hipStream_t St, int d_i) {
+  // The halo buffers serve bgcaccess (kernel.cu:2809-2852) for fine indices
+  // outside the padded array; with WWL = 2 the indices read here span
+  // [2, nxt_fine+19] x [2, nyt_fine+19], inside it, so they are never read.
+  (void)buf_L; (void)buf_R; (void)buf_F; (void)buf_B; (void)rank;
+  const char* W = "swap_H";
+  const AwpGrid *gf, *gc;
+  if (!dm_grid_pair(d_i, W, &gf, &gc)) return;
+  if (nxtl != gc->nxt || nytl != gc->nyt) {
+    awp_set_error(AWP_ERR_ARG, "%s: nxtl=%d nytl=%d disagree with grid %d (%d, %d)", W, nxtl,
+                  nytl, d_i + 1, gc->nxt, gc->nyt);
+    return;
+  }
+  if (gf->nzt < 9) {
+    awp_set_error(AWP_ERR_ARG, "%s: fine grid %d has nzt=%d < 9", W, d_i, gf->nzt);
+    return;
+  }
+  Coupling C = make_coupling(*gf, *gc);
+  C.ci0 = AWP_NGSL; C.ci1 = nxtl + AWP_NGSL + 4;  // kernel.cu:3066
+  C.cj0 = AWP_NGSL; C.cj1 = nytl + AWP_NGSL + 4;
+  C.kc = gc->nzt + AWP_ALIGN - 2;
+  Fields9 L9 = {{u1l, v1l, w1l, xxl, yyl, zzl, xyl, xzl, yzl}};
+  Fields9 H9 = {{u1h, v1h, w1h, xxh, yyh, zzh, xyh, xzh, yzh}};
+  const int nti = awp_cdiv(C.ci1 - C.ci0 + 1, SW_TI);
+  const int ntj = awp_cdiv(C.cj1 - C.cj0 + 1, SW_TJ);
+  hipLaunchKernelGGL(swap3_kernel, dim3(nti * ntj), dim3(64), 0, St, L9, H9, C, ntj);
+  awp_check_launch(W);
+}
+
+}  // extern "C"

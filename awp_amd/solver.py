@@ -1,11 +1,14 @@
 """One-GPU driver of the time-stepping path through the C ABI.
 
 The per-step call order is pmcl3d's (awp_amd.stepping); every update runs in
-libawp_hip.so.  Device memory is torch-allocated (plumbing only).
+libawp_hip.so.  Device memory is torch-allocated (plumbing only).  A run has
+one grid, or a stack of grids coupled by the discontinuous-mesh kernels
+(grid 0 on top, each lower grid 3x coarser).
 """
 from __future__ import annotations
 
 import ctypes as C
+from typing import Sequence
 
 import numpy as np
 import torch
@@ -19,117 +22,215 @@ from .stepping import PmclStepOrder
 
 FIELDS = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
           "r1", "r2", "r3", "r4", "r5", "r6")
+# argument order of the nine-field packs of intp3d_H and swap_H
+INTP_ORDER = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")
+SWAP_ORDER = ("xx", "yy", "zz", "xy", "xz", "yz", "u1", "v1", "w1")
 
 
 def _ptr(t: torch.Tensor) -> int:
     return t.data_ptr()
 
 
-class Solver(PmclStepOrder):
-    """FP32 wavefield state and medium of one grid, resident in HBM."""
+def _as_list(x, n: int) -> list:
+    if isinstance(x, (list, tuple)):
+        if len(x) != n:
+            raise ValueError(f"expected {n} per-grid entries, got {len(x)}")
+        return list(x)
+    return [x] + [None] * (n - 1)
 
-    def __init__(self, medium: Medium, sources: PointSources | None = None,
+
+def check_source_cells(dims: Dims, psrc: np.ndarray) -> None:
+    """Source cells (tpsrc convention) must land in the padded grid."""
+    idx = psrc[:, 0] + 1 + NGSL
+    idy = psrc[:, 1] + 1 + NGSL
+    idz = psrc[:, 2] + ALIGN - 1
+    sh = dims.shape
+    if (idx.min() < 0 or idx.max() >= sh[0] or idy.min() < 0 or idy.max() >= sh[1]
+            or idz.min() < 0 or idz.max() >= sh[2]):
+        raise ValueError("source cell outside the padded grid")
+
+
+def check_receiver_cells(dims: Dims, idx: np.ndarray) -> None:
+    sh = dims.shape
+    if (idx[:, 0].min() < 0 or idx[:, 0].max() >= sh[0] or idx[:, 1].min() < 0
+            or idx[:, 1].max() >= sh[1] or idx[:, 2].min() < 1 or idx[:, 2].max() >= sh[2]):
+        raise ValueError("receiver cell outside the padded grid")
+
+
+def check_dm_stack(media: Sequence[Medium]) -> None:
+    """Grid p+1 must be 3x coarser than grid p with a third of its lateral
+    cells (pmcl3d.c:370-377); the coupling kernels index on that."""
+    for p in range(len(media) - 1):
+        f, c = media[p].dims, media[p + 1].dims
+        if f.nxt != 3 * c.nxt or f.nyt != 3 * c.nyt:
+            raise ValueError(f"grid {p} ({f.nxt}x{f.nyt}) is not 3x grid {p + 1} "
+                             f"({c.nxt}x{c.nyt})")
+        if not np.isclose(media[p].DH * 3.0, media[p + 1].DH, rtol=1e-6):
+            raise ValueError(f"grid {p + 1} spacing {media[p + 1].DH} is not 3x {media[p].DH}")
+        if f.nzt < 9 or c.nzt < 4:
+            raise ValueError("discontinuous mesh needs nzt >= 9 on fine and >= 4 on coarse grids")
+        if media[p].DT != media[p + 1].DT:
+            raise ValueError("all grids share one time step")
+
+
+class _Level:
+    """Device state of one grid: wavefield, medium, sources, plasticity."""
+
+    def __init__(self, medium: Medium, sources: PointSources | None,
+                 plastic: PlasticMedium | None, device: torch.device):
+        self.dims: Dims = medium.dims
+        self.medium = medium
+        d = self.dims
+        f32 = dict(dtype=torch.float32, device=device)
+        self.f = {name: torch.zeros(d.shape, **f32) for name in FIELDS}
+        self.m = {}
+        for name in ("d1", "lam", "mu", "qp", "qs", "vx1", "vx2", "wwo", "coeff", "lam_mu",
+                     "dcrjx", "dcrjy", "dcrjz"):
+            self.m[name] = torch.from_numpy(
+                np.ascontiguousarray(getattr(medium, name), dtype=np.float32)).to(device)
+        self.m["ww"] = torch.from_numpy(np.ascontiguousarray(medium.ww, dtype=np.int32)).to(device)
+        self.plastic = plastic
+        if plastic is not None:
+            for name in ("sigma2", "cohes", "phi"):
+                self.m[name] = torch.from_numpy(
+                    np.ascontiguousarray(getattr(plastic, name), dtype=np.float32)).to(device)
+            self.f["yldfac"] = torch.ones(d.shape, **f32)
+            self.f["neta"] = torch.zeros(d.shape, **f32)
+        self.src = None
+        if sources is not None and sources.npsrc > 0:
+            psrc = sources.psrc(d)
+            check_source_cells(d, psrc)
+            self.src = dict(
+                psrc=torch.from_numpy(psrc.reshape(-1).astype(np.int32)).to(device),
+                cols=[torch.from_numpy(c.astype(np.float32)).to(device)
+                      for c in sources.columns()],
+                npsrc=sources.npsrc, nst=sources.nst)
+
+
+class Solver(PmclStepOrder):
+    """FP32 wavefield state and media, resident in HBM.
+
+    medium: one Medium, or a list (grid 0 on top) for a discontinuous mesh.
+    sources / plastic: one entry (grid 0) or a per-grid list.
+    receivers: surface receivers on grid 0."""
+
+    def __init__(self, medium: Medium | Sequence[Medium],
+                 sources: PointSources | Sequence | None = None,
                  receivers: Receivers | None = None, nt_record: int = 0,
                  device: str = "cuda", variant: str = "dstrqc", NPC: int = 0,
-                 plastic: PlasticMedium | None = None):
+                 plastic: PlasticMedium | Sequence | None = None):
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a GPU: the HIP path has no CPU fallback")
         self.lib = _lib.load()
-        self.dims: Dims = medium.dims
-        self.medium = medium
+        media = list(medium) if isinstance(medium, (list, tuple)) else [medium]
+        G = len(media)
+        check_dm_stack(media)
+        self.ngrids = G
         self.device = torch.device(device)
         self.variant = variant
         self.NPC = NPC
-        d = self.dims
-        f32 = dict(dtype=torch.float32, device=self.device)
-        self.f = {name: torch.zeros(d.shape, **f32) for name in FIELDS}
-        self.m = {}
-        for name in ("d1", "lam", "mu", "qp", "qs", "vx1", "vx2", "wwo"):
-            self.m[name] = torch.from_numpy(
-                np.ascontiguousarray(getattr(medium, name), dtype=np.float32)).to(self.device)
-        self.m["ww"] = torch.from_numpy(np.ascontiguousarray(medium.ww, dtype=np.int32)).to(self.device)
-        for name in ("coeff", "lam_mu", "dcrjx", "dcrjy", "dcrjz"):
-            self.m[name] = torch.from_numpy(
-                np.ascontiguousarray(getattr(medium, name), dtype=np.float32)).to(self.device)
-        DH = (C.c_float * 1)(medium.DH)
-        nxt = (C.c_int * 1)(d.nxt)
-        nyt = (C.c_int * 1)(d.nyt)
-        nzt = (C.c_int * 1)(d.nzt)
-        self.plastic = plastic
-        if plastic is not None:
-            pp = plastic.params
+        srcs = _as_list(sources, G)
+        pls = _as_list(plastic, G) if plastic is not None else [None] * G
+        if plastic is not None and any(p is None for p in pls):
+            raise ValueError("plasticity needs a plastic medium on every grid")
+        self.plastic = pls if plastic is not None else None
+        self.levels = [_Level(media[p], srcs[p], pls[p], self.device) for p in range(G)]
+        DH = (C.c_float * G)(*[m.DH for m in media])
+        nxt = (C.c_int * G)(*[m.dims.nxt for m in media])
+        nyt = (C.c_int * G)(*[m.dims.nyt for m in media])
+        nzt = (C.c_int * G)(*[m.dims.nzt for m in media])
+        if self.plastic is not None:
+            pp = pls[0].params
             Rz, RzT = pp.rotation()
             Rzc = (C.c_float * 9)(*Rz.tolist())
             RzTc = (C.c_float * 9)(*RzT.tolist())
             fmaj, fmin = pp.fmajor, pp.fminor
-            for name in ("sigma2", "cohes", "phi"):
-                self.m[name] = torch.from_numpy(
-                    np.ascontiguousarray(getattr(plastic, name), dtype=np.float32)).to(self.device)
-            self.f["yldfac"] = torch.ones(d.shape, **f32)
-            self.f["neta"] = torch.zeros(d.shape, **f32)
         else:
             Rzc = RzTc = (C.c_float * 9)()
             fmaj = fmin = 0.0
-        self.lib.SetDeviceConstValue(DH, C.c_float(medium.DT), nxt, nyt, nzt, 1, fmaj, fmin, Rzc,
-                                     RzTc)
+        self.lib.SetDeviceConstValue(DH, C.c_float(media[0].DT), nxt, nyt, nzt, G, fmaj, fmin,
+                                     Rzc, RzTc)
         _lib.check()
-        # sources
-        self.src = None
-        if sources is not None and sources.npsrc > 0:
-            psrc = sources.psrc(d)
-            self._check_sources(psrc)
-            cols = sources.columns()
-            self.src = dict(
-                psrc=torch.from_numpy(psrc.reshape(-1).copy()).to(self.device),
-                cols=[torch.from_numpy(c.astype(np.float32)).to(self.device) for c in cols],
-                npsrc=sources.npsrc, nst=sources.nst)
-        # receivers
+        # receivers on grid 0
         self.rec = None
+        d0 = self.levels[0].dims
         if receivers is not None and len(receivers.xy) > 0:
-            idx = receivers.array_index(d)
-            self._check_receivers(idx)
+            idx = receivers.array_index(d0)
+            check_receiver_cells(d0, idx)
+            f32 = dict(dtype=torch.float32, device=self.device)
             self.rec = dict(idx=torch.from_numpy(idx.reshape(-1).copy()).to(self.device),
                             n=len(idx), out=torch.zeros((max(nt_record, 1), len(idx), 3), **f32))
         self.cur_step = 0
 
-    # -- validation of host-built index lists before they reach a kernel --
-    def _check_sources(self, psrc: np.ndarray) -> None:
-        d = self.dims
-        idx = psrc[:, 0] + 1 + NGSL
-        idy = psrc[:, 1] + 1 + NGSL
-        idz = psrc[:, 2] + ALIGN - 1
-        sh = d.shape
-        if (idx.min() < 0 or idx.max() >= sh[0] or idy.min() < 0 or idy.max() >= sh[1]
-                or idz.min() < 0 or idz.max() >= sh[2]):
-            raise ValueError("source cell outside the padded grid")
+    # grid-0 views kept for single-grid callers
+    @property
+    def dims(self) -> Dims:
+        return self.levels[0].dims
 
-    def _check_receivers(self, idx: np.ndarray) -> None:
-        sh = self.dims.shape
-        if (idx[:, 0].min() < 0 or idx[:, 0].max() >= sh[0] or idx[:, 1].min() < 0
-                or idx[:, 1].max() >= sh[1] or idx[:, 2].min() < 1 or idx[:, 2].max() >= sh[2]):
-            raise ValueError("receiver cell outside the padded grid")
+    @property
+    def medium(self) -> Medium:
+        return self.levels[0].medium
+
+    @property
+    def f(self) -> dict:
+        return self.levels[0].f
+
+    @property
+    def m(self) -> dict:
+        return self.levels[0].m
 
     @property
     def stream(self) -> int:
         return torch.cuda.current_stream(self.device).cuda_stream
 
+    def grid_dims(self, p: int) -> Dims:
+        return self.levels[p].dims
+
+    def source_steps(self, p: int) -> int:
+        s = self.levels[p].src
+        return s["nst"] if s is not None else 0
+
     # -- the launchers --
-    def velocity(self) -> None:
-        d, f, m = self.dims, self.f, self.m
+    def velocity(self, p: int = 0) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
         s_i, e_i = d.i_range
         self.lib.dvelcx_H_opt(_ptr(f["u1"]), _ptr(f["v1"]), _ptr(f["w1"]), _ptr(f["xx"]),
                               _ptr(f["yy"]), _ptr(f["zz"]), _ptr(f["xy"]), _ptr(f["xz"]),
                               _ptr(f["yz"]), _ptr(m["dcrjx"]), _ptr(m["dcrjy"]),
                               _ptr(m["dcrjz"]), _ptr(m["d1"]), d.nyt, d.nzt, self.stream, s_i,
-                              e_i, 0, 1)
+                              e_i, p, self.ngrids)
 
-    def stress(self) -> None:
-        d = self.dims
+    def velocity_dm(self, p: int) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
+        self.lib.dvelc2_H(*(_ptr(f[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz",
+                                                 "yz")),
+                          _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), _ptr(m["d1"]),
+                          d.nxt, d.nyt, self.stream, p)
+
+    def interpolate(self, p: int) -> None:
+        """Coarse grid p onto the bottom of fine grid p-1 (intp3d_H)."""
+        lo, hi = self.levels[p], self.levels[p - 1]
+        self.lib.intp3d_H(*(_ptr(lo.f[n]) for n in INTP_ORDER),
+                          *(_ptr(hi.f[n]) for n in INTP_ORDER), lo.dims.nxt, lo.dims.nyt, 0,
+                          self.stream, p)
+
+    def swap(self, p: int) -> None:
+        """Fine grid p onto the top of coarse grid p+1 (swap_H)."""
+        hi, lo = self.levels[p], self.levels[p + 1]
+        self.lib.swap_H(*(_ptr(lo.f[n]) for n in SWAP_ORDER),
+                        *(_ptr(hi.f[n]) for n in SWAP_ORDER), lo.dims.nxt, lo.dims.nyt,
+                        None, None, None, None, 0, self.stream, p)
+
+    def stress(self, p: int = 0) -> None:
+        d = self.levels[p].dims
         (s_i, e_i), (s_j, e_j) = d.i_range, d.j_range
-        self.stress_region(s_i, e_i, s_j, e_j)
+        self.stress_region(p, s_i, e_i, s_j, e_j)
 
-    def stress_region(self, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
-        d, f, m = self.dims, self.f, self.m
+    def stress_region(self, p: int, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:
             return
         fn = self.lib.dstrqc_H if self.variant == "dstrqc" else self.lib.dstrqc_H_new
@@ -138,34 +239,47 @@ class Solver(PmclStepOrder):
            _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["coeff"]), _ptr(m["qs"]),
            _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nyt, d.nzt, self.stream,
            _ptr(m["lam_mu"]), _ptr(m["vx1"]), _ptr(m["vx2"]), _ptr(m["ww"]), _ptr(m["wwo"]),
-           d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, 0)
+           d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, p)
 
-    def plastic_calc(self, xs: int, xe: int, ys: int, ye: int) -> None:
-        f, m, d = self.f, self.m, self.dims
+    def stress_dm(self, p: int, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
+        self.lib.dstrqc2_H(*(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2",
+                                                  "r3", "r4", "r5", "r6", "u1", "v1", "w1")),
+                           _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["qs"]),
+                           _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nxt, d.nyt,
+                           self.stream, _ptr(m["coeff"]), _ptr(m["vx1"]), _ptr(m["vx2"]),
+                           _ptr(m["ww"]), _ptr(m["wwo"]), s_i, e_i, s_j, e_j, p)
+
+    def plastic_calc(self, p: int, xs: int, xe: int, ys: int, ye: int) -> None:
+        L = self.levels[p]
+        f, m, d = L.f, L.m, L.dims
         args = [_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")] + [
             _ptr(m["mu"]), _ptr(m["d1"]), _ptr(m["sigma2"]), _ptr(f["yldfac"]),
             _ptr(m["cohes"]), _ptr(m["phi"]), _ptr(f["neta"]), d.nzt, xs, xe, ys, ye,
-            self.stream, 0]
-        if self.plastic.params.Tv > 0:
-            self.lib.awp_drprecpc_calc_tv(*args, float(self.plastic.params.Tv))
+            self.stream, p]
+        if L.plastic.params.Tv > 0:
+            self.lib.awp_drprecpc_calc_tv(*args, float(L.plastic.params.Tv))
         else:
             self.lib.drprecpc_calc_H_opt(*args)
 
-    def plastic_app(self, xs: int, xe: int, ys: int, ye: int) -> None:
-        f, m, d = self.f, self.m, self.dims
+    def plastic_app(self, p: int, xs: int, xe: int, ys: int, ye: int) -> None:
+        L = self.levels[p]
+        f, m, d = L.f, L.m, L.dims
         self.lib.drprecpc_app_H(*(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")),
                                 _ptr(m["mu"]), _ptr(m["sigma2"]), _ptr(f["yldfac"]), d.nzt, xs,
-                                xe, ys, ye, self.stream, 0)
+                                xe, ys, ye, self.stream, p)
 
-    def add_source(self, sample_step: int) -> None:
-        if self.src is None:
+    def add_source(self, p: int, sample_step: int) -> None:
+        L = self.levels[p]
+        if L.src is None:
             return
-        f, s = self.f, self.src
+        f, s = L.f, L.src
         axx, ayy, azz, axz, ayz, axy = s["cols"]
         self.lib.addsrc_H(sample_step, s["nst"], 3, _ptr(s["psrc"]), s["npsrc"], self.stream,
                           _ptr(axx), _ptr(ayy), _ptr(azz), _ptr(axz), _ptr(ayz), _ptr(axy),
                           _ptr(f["xx"]), _ptr(f["yy"]), _ptr(f["zz"]), _ptr(f["xy"]),
-                          _ptr(f["yz"]), _ptr(f["xz"]), 0)
+                          _ptr(f["yz"]), _ptr(f["xz"]), p)
 
     def free_surface(self) -> None:
         d, f = self.dims, self.f
@@ -195,6 +309,6 @@ class Solver(PmclStepOrder):
         torch.cuda.synchronize(self.device)
         return self.rec["out"].cpu().numpy() if self.rec is not None else np.zeros((0, 0, 3))
 
-    def field(self, name: str) -> np.ndarray:
+    def field(self, name: str, p: int = 0) -> np.ndarray:
         torch.cuda.synchronize(self.device)
-        return self.f[name].cpu().numpy()
+        return self.levels[p].f[name].cpu().numpy()

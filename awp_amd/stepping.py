@@ -1,27 +1,30 @@
 """The per-step call order of the pmcl3d time loop on one rank
 (reference src/awp/pmcl3d.c:1628-2487), shared by every driver.
 
-A driver supplies the launch primitives (velocity, stress_region,
-add_source, plastic_calc, plastic_app, free_surface, record); this mixin
-sequences them exactly as pmcl3d does:
+A driver holds ngrids grids, grid 0 on top (finest), each coarser grid 3x the
+spacing of the one above it (pmcl3d.c:370-377), and supplies the launch
+primitives per grid p: velocity, velocity_dm, interpolate, swap,
+stress_region, stress_dm, add_source, plastic_calc, plastic_app, plus
+free_surface and record on grid 0.  This mixin sequences them as pmcl3d does:
 
-  NVE = 1 (visco-elastic):
-    dvelcx_H_opt  i in [xvs, xve]                                  :1690
-    dstrqc_H      inner x strip, then the left/right strips        :1845, :1904-1919
-    addsrc_H      sample cur_step, while cur_step < NST            :2007-2013
-    fstr_H                                                         :2243
-    receivers                                                      :2246
-  NVE = 3 (visco-elasto-plastic) adds the region split:
-    dstrqc_H      inner x strip outside [yls2, yre2], then left/right  :1858-1877
-    addsrc_H
-    drprecpc_calc front, back; dstrqc_H on the inner box; drprecpc_calc
-    left, right, inside; drprecpc_app on everything               :2109-2209
-    fstr_H, receivers
+  dvelcx_H_opt   every grid, i in [xvs, xve]                          :1690
+  dvelc2_H       every grid above the lowest                          :1727-1730
+  intp3d_H       coarse p -> fine p-1                                  :1765-1770
+  swap_H         fine p -> coarse p+1                                  :1802-1806
+  dstrqc_H       NVE = 1: the whole stress region of every grid        :1836-1852
+                 NVE = 3: the inner x strip outside [yls2, yre2]      :1858-1877
+                 then the left/right strips of every grid              :1895-1920
+  dstrqc2_H      every grid above the lowest, [xss1, xse3] x [yls, yre] :1936-1941
+  addsrc_H       every grid with sources, while cur_step < NST         :2007-2015
+  intp3d_H, swap_H again                                               :2045-2092
+  NVE = 3: drprecpc_calc front/back; dstrqc_H on the inner box;
+           drprecpc_calc left/right, inside; drprecpc_app              :2109-2209
+  fstr_H on grid 0, receivers                                          :2243-2246
 On one rank the NVE = 1 strips cover the stress region exactly once, so a
 single dstrqc_H over all of it is the same update.  In the NVE = 3 order the
-inner box's stress update runs after the source injection, and the
-drprecpc_calc regions overlap (neta grows more than once in the overlaps):
-both are pmcl3d's behaviour and are kept.
+inner box's stress update runs after the source injection and after the
+grid coupling, and the drprecpc_calc regions overlap (neta grows more than
+once in the overlaps): all of it is pmcl3d's behaviour and is kept.
 """
 from __future__ import annotations
 
@@ -31,38 +34,68 @@ from .plasticity import nve3_regions
 
 class PmclStepOrder:
     cur_step: int
-    plastic = None
+    ngrids: int = 1
+    plastic = None  # None, or one plastic medium per grid
 
     def start(self) -> None:
         """Initial source sample before the loop (pmcl3d.c:1175-1186)."""
-        self.add_source(1)
+        for p in range(self.ngrids):
+            self.add_source(p, 1)
 
-    def _source(self) -> None:
-        if self.src is not None and self.cur_step < self.src["nst"]:
-            self.add_source(self.cur_step + 1)
+    def _sources(self) -> None:
+        for p in range(self.ngrids):
+            nst = self.source_steps(p)
+            if nst and self.cur_step < nst:
+                self.add_source(p, self.cur_step + 1)
+
+    def _couple(self) -> None:
+        G = self.ngrids
+        for p in range(1, G):
+            self.interpolate(p)
+        for p in range(G - 1):
+            self.swap(p)
 
     def step(self) -> None:
         self.cur_step += 1
-        self.velocity()
-        d = self.dims
+        G = self.ngrids
+        for p in range(G):
+            self.velocity(p)
+        for p in range(G - 1):
+            self.velocity_dm(p)
+        self._couple()
+        regions = [nve3_regions(self.grid_dims(p)) for p in range(G)]
         if self.plastic is None:
-            (s_i, e_i), (s_j, e_j) = d.i_range, d.j_range
-            self.stress_region(s_i, e_i, s_j, e_j)
-            self._source()
+            for p in range(G):
+                d = self.grid_dims(p)
+                (s_i, e_i), (s_j, e_j) = d.i_range, d.j_range
+                self.stress_region(p, s_i, e_i, s_j, e_j)
         else:
-            R = nve3_regions(d)
-            self.stress_region(R["xss2"], R["xse2"], R["yls"], R["yls2"] - 1)
-            self.stress_region(R["xss2"], R["xse2"], R["yre2"] + 1, R["yre"])
-            self.stress_region(R["xss1"], R["xse1"], R["yls"], R["yre"])
-            self.stress_region(R["xss3"], R["xse3"], R["yls"], R["yre"])
-            self._source()
-            xl, xr, yl, yr = R["xls"], R["xre"], R["yls"], R["yre"]
-            self.plastic_calc(xl, xr, yl, yl + NGSL)
-            self.plastic_calc(xl, xr, yr - NGSL, yr)
-            self.stress_region(R["xss2"], R["xse2"], R["yls2"], R["yre2"])
-            self.plastic_calc(xl, xl + NGSL, yl, yr)
-            self.plastic_calc(xr - NGSL, xr, yl, yr)
-            self.plastic_calc(xl + NGSL, xr - NGSL, yl + NGSL, yr - NGSL)
-            self.plastic_app(xl, xr, yl, yr)
+            for p, R in enumerate(regions):
+                self.stress_region(p, R["xss2"], R["xse2"], R["yls"], R["yls2"] - 1)
+                self.stress_region(p, R["xss2"], R["xse2"], R["yre2"] + 1, R["yre"])
+            for p, R in enumerate(regions):
+                self.stress_region(p, R["xss1"], R["xse1"], R["yls"], R["yre"])
+                self.stress_region(p, R["xss3"], R["xse3"], R["yls"], R["yre"])
+        for p in range(G - 1):
+            R = regions[p]
+            self.stress_dm(p, R["xss1"], R["xse3"], R["yls"], R["yre"])
+        self._sources()
+        self._couple()
+        if self.plastic is not None:
+            for p, R in enumerate(regions):
+                xl, xr, yl, yr = R["xls"], R["xre"], R["yls"], R["yre"]
+                self.plastic_calc(p, xl, xr, yl, yl + NGSL)
+                self.plastic_calc(p, xl, xr, yr - NGSL, yr)
+            for p, R in enumerate(regions):
+                self.stress_region(p, R["xss2"], R["xse2"], R["yls2"], R["yre2"])
+            for p, R in enumerate(regions):
+                xl, xr, yl, yr = R["xls"], R["xre"], R["yls"], R["yre"]
+                self.plastic_calc(p, xl, xl + NGSL, yl, yr)
+                self.plastic_calc(p, xr - NGSL, xr, yl, yr)
+            for p, R in enumerate(regions):
+                xl, xr, yl, yr = R["xls"], R["xre"], R["yls"], R["yre"]
+                self.plastic_calc(p, xl + NGSL, xr - NGSL, yl + NGSL, yr - NGSL)
+            for p, R in enumerate(regions):
+                self.plastic_app(p, R["xls"], R["xre"], R["yls"], R["yre"])
         self.free_surface()
         self.record(self.cur_step - 1)

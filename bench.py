@@ -145,8 +145,8 @@ def main():
         e[1].record(stream)
         sol.stress()
         e[2].record(stream)
-        if sol.src is not None and sol.cur_step < sol.src["nst"]:
-            sol.add_source(sol.cur_step + 1)
+        if sol.cur_step < sol.source_steps(0):
+            sol.add_source(0, sol.cur_step + 1)
         sol.free_surface()
         sol.record(sol.cur_step - 1)
         e[3].record(stream)

@@ -132,6 +132,45 @@ void drprecpc_app_H(float* xx, float* yy, float* zz, float* xy, float* xz, float
 void awp_receivers_gather(const float* u1, const float* v1, const float* w1, const int* idx,
                           int npts, float* out, int step_slot, hipStream_t St, int d_i);
 
+/* Replaces dvelc2_H, include/awp/kernel_launch.h:92-95
+ * (src/awp/kernel.cu:2289-2356).  Discontinuous mesh, fine grid d_i: velocity
+ * at k = align+2 with 2nd-order differences, on the interior columns. */
+void dvelc2_H(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz, float* xy,
+              float* xz, float* yz, float* dcrjx, float* dcrjy, float* dcrjz, float* d_1, int nxt,
+              int nyt, hipStream_t St, int d_i);
+
+/* Replaces dstrqc2_H, include/awp/kernel_launch.h:97-103
+ * (src/awp/kernel.cu:2564-2784).  Discontinuous mesh, fine grid d_i: stress
+ * and Q memory variables at k = align+2 with 2nd-order differences on
+ * i in [s_i, e_i], j in [s_j, e_j]. */
+void dstrqc2_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
+               float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
+               float* w1, float* lam, float* mu, float* qp, float* qs, float* dcrjx, float* dcrjy,
+               float* dcrjz, int nxt, int nyt, hipStream_t St, float* coeff, float* d_vx1,
+               float* d_vx2, int* d_ww, float* d_wwo, int s_i, int e_i, int s_j, int e_j,
+               int d_i);
+
+/* Replaces intp3d_H, include/awp/kernel_launch.h:105-109
+ * (src/awp/kernel.cu:2359-2561, 2786-2804).  d_i is the coarse grid ("l"
+ * arrays); the fine grid d_i-1 ("h" arrays) receives, at k = align+1, the
+ * 4x4 bilinear interpolation of the coarse level k = nzt+align-3.  rank is
+ * unused. */
+void intp3d_H(float* u1l, float* v1l, float* w1l, float* xxl, float* yyl, float* zzl, float* xyl,
+              float* xzl, float* yzl, float* u1h, float* v1h, float* w1h, float* xxh, float* yyh,
+              float* zzh, float* xyh, float* xzh, float* yzh, int nxtl, int nytl, int rank,
+              hipStream_t St, int d_i);
+
+/* Replaces swap_H, include/awp/kernel_launch.h:111-116
+ * (src/awp/kernel.cu:3032-3234).  d_i is the fine grid ("h" arrays); the
+ * coarse grid d_i+1 ("l" arrays) receives at k = nzt+align-2 and nzt+align-1
+ * the 5x5x5 triangular-weighted averages of the fine levels align+1..align+8.
+ * The halo buffers buf_L/R/F/B are accepted and never read: every fine index
+ * the average touches lies inside the padded fine array.  rank is unused. */
+void swap_H(float* xxl, float* yyl, float* zzl, float* xyl, float* xzl, float* yzl, float* u1l,
+            float* v1l, float* w1l, float* xxh, float* yyh, float* zzh, float* xyh, float* xzh,
+            float* yzh, float* u1h, float* v1h, float* w1h, int nxtl, int nytl, float* buf_L,
+            float* buf_R, float* buf_F, float* buf_B, int rank, hipStream_t St, int d_i);
+
 #ifdef __cplusplus
 }
 #endif

@@ -547,6 +547,234 @@ void drprecpc_app(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz, const T* sigma2,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Discontinuous mesh (kernel.cu:2289-3234).  A fine grid (d_i) sits on a
+// coarse grid (d_i+1) with 3x refinement; each level's constants come from
+// its own Grid.
+constexpr int WWL = 2;  // include/awp/pmcl3d_cons.h:92
+
+// dvelc2 (kernel.cu:2289-2343): 2nd-order velocity at k = align+2 of a fine grid.
+template <typename T>
+void dvelc2(T* u1, T* v1, T* w1, const T* xx, const T* yy, const T* zz, const T* xy,
+            const T* xz, const T* yz, const T* dcrjx, const T* dcrjy, const T* dcrjz,
+            const T* d1, const Grid<T>& g) {
+  const long sl = g.slice, yl = g.yline;
+  const int k = 2 + ALIGN;
+  for (int i = 2 + NGSL; i < g.nxt + 2 + NGSL; ++i)
+    for (int j = 2 + NGSL; j < g.nyt + 2 + NGSL; ++j) {
+      const long p = i * sl + j * yl + k;
+      T f_d3 = (T)(0.25 * (double)(d1[p] + d1[p + sl] + d1[p - yl] + d1[p + sl - yl]));
+      f_d3 = g.dth / f_d3;
+      const T f_dcrj = dcrjz[k] * dcrjx[i] * dcrjy[j];
+      w1[p] = (w1[p] + f_d3 * ((xz[p + sl] - xz[p]) + (yz[p] - yz[p - yl]) + (zz[p + 1] - zz[p]))) * f_dcrj;
+      T f_d1 = (T)(0.25 * (double)(d1[p] + d1[p - yl] + d1[p - 1] + d1[p - yl - 1]));
+      T f_d2 = (T)(0.25 * (double)(d1[p] + d1[p + sl] + d1[p - 1] + d1[p + sl - 1]));
+      f_d1 = g.dth / f_d1;
+      f_d2 = g.dth / f_d2;
+      u1[p] = (u1[p] + f_d1 * ((xx[p] - xx[p - sl]) + (xy[p] - xy[p - yl]) + (xz[p] - xz[p - 1]))) * f_dcrj;
+      v1[p] = (v1[p] + f_d2 * ((xy[p + sl] - xy[p]) + (yy[p + yl] - yy[p]) + (yz[p] - yz[p - 1]))) * f_dcrj;
+    }
+}
+
+// dstrqc2 (kernel.cu:2564-2761): 2nd-order stress + Q at k = align+2, with
+// that kernel's Q-branch literals (qpa: float test, no factor 2 in the
+// fallback; others: double test).
+template <typename T>
+void dstrqc2(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz, T* r1, T* r2, T* r3, T* r4, T* r5,
+             T* r6, const T* u1, const T* v1, const T* w1, const T* lam, const T* mu,
+             const T* qp, const T* qs, const T* dcrjx, const T* dcrjy, const T* dcrjz,
+             const T* coeff, const T* vx1, const T* vx2, const int* ww, const T* wwo, int s_i,
+             int e_i, int s_j, int e_j, const Grid<T>& g) {
+  const long sl = g.slice, yl = g.yline;
+  const int k = ALIGN + 2;
+  const T DT = g.DT, dth = g.dth, dh1 = g.dh1;
+  for (int i = s_i; i <= e_i; ++i)
+    for (int j = s_j; j <= e_j; ++j) {
+      const long p = i * sl + j * yl + k;
+      const long ip1 = p + sl, jm1 = p - yl, ijk = p + sl - yl, km1 = p - 1, ik1 = p + sl - 1,
+                 jk1 = p - yl - 1, ijk1 = p + sl - yl - 1;
+      const T f_dcrj = dcrjz[k] * dcrjx[i] * dcrjy[j];
+      const T f_vx1 = vx1[p], f_vx2 = vx2[p], f_wwo = wwo[p];
+      const int f_ww = ww[p];
+      T xl = (T)(8.0 / (double)(lam[p] + lam[ip1] + lam[jm1] + lam[ijk] + lam[km1] + lam[ik1] + lam[jk1] + lam[ijk1]));
+      T xm = (T)(16.0 / (double)(mu[p] + mu[ip1] + mu[jm1] + mu[ijk] + mu[km1] + mu[ik1] + mu[jk1] + mu[ijk1]));
+      T xmu1 = (T)(2.0 / (double)(mu[p] + mu[km1]));
+      T xmu2 = (T)(2.0 / (double)(mu[p] + mu[jm1]));
+      T xmu3 = (T)(2.0 / (double)(mu[p] + mu[ip1]));
+      xl = xl + xm;
+      T qpa = (T)(0.0625 * (double)(qp[p] + qp[ip1] + qp[jm1] + qp[ijk] + qp[km1] + qp[ik1] + qp[jk1] + qp[ijk1]));
+      T qpaw;
+      if (1.0f / (qpa * 2.0f) <= 200.0f)
+        qpaw = (T)((double)coeff[f_ww * 2 - 2] * (2. * (double)qpa) * (2. * (double)qpa) +
+                   (double)coeff[f_ww * 2 - 1] * (2. * (double)qpa));
+      else
+        qpaw = f_wwo * qpa;
+      qpaw = qpaw / f_wwo;
+      auto qw2 = [&](T q) {
+        T w;
+        if (1. / ((double)q * 2.0) <= 200.0)
+          w = (T)((double)coeff[f_ww * 2 - 2] * (2. * (double)q) * (2. * (double)q) +
+                  (double)coeff[f_ww * 2 - 1] * (2. * (double)q));
+        else
+          w = f_wwo * q;
+        return w / f_wwo;
+      };
+      T h = 0.0625f * (qs[p] + qs[ip1] + qs[jm1] + qs[ijk] + qs[km1] + qs[ik1] + qs[jk1] + qs[ijk1]);
+      const T hw = qw2(h);
+      T h1 = (T)(0.25 * (double)(qs[p] + qs[km1]));
+      const T h1w = qw2(h1);
+      T h2 = (T)(0.250 * (double)(qs[p] + qs[jm1]));
+      const T h2w = qw2(h2);
+      T h3 = (T)(0.250 * (double)(qs[p] + qs[ip1]));
+      const T h3w = qw2(h3);
+      h = -xm * hw * dh1;
+      h1 = -xmu1 * h1w * dh1;
+      h2 = -xmu2 * h2w * dh1;
+      h3 = -xmu3 * h3w * dh1;
+      qpa = -qpaw * xl * dh1;
+      xm = xm * dth;
+      xmu1 = xmu1 * dth;
+      xmu2 = xmu2 * dth;
+      xmu3 = xmu3 * dth;
+      xl = xl * dth;
+      h = h * f_vx1;
+      h1 = h1 * f_vx1;
+      h2 = h2 * f_vx1;
+      h3 = h3 * f_vx1;
+      qpa = qpa * f_vx1;
+      xm = xm + DT * h;
+      xmu1 = xmu1 + DT * h1;
+      xmu2 = xmu2 + DT * h2;
+      xmu3 = xmu3 + DT * h3;
+      const T vx1p = DT * (1 + f_vx2 * f_vx1);
+      T f_r, f_rtmp, t, vs1, vs2, vs3;
+      // xz, yz
+      vs1 = u1[p + 1] - u1[p];
+      vs2 = w1[p] - w1[p - sl];
+      f_r = r5[p];
+      f_rtmp = h2 * (vs1 + vs2);
+      t = xz[p] + xmu2 * (vs1 + vs2) + vx1p * f_r;
+      r5[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+      xz[p] = (t + DT * f_rtmp) * f_dcrj;
+      vs1 = v1[p + 1] - v1[p];
+      vs2 = w1[p + yl] - w1[p];
+      f_r = r6[p];
+      f_rtmp = h3 * (vs1 + vs2);
+      t = yz[p] + xmu3 * (vs1 + vs2) + vx1p * f_r;
+      r6[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+      yz[p] = (t + DT * f_rtmp) * f_dcrj;
+      // normal
+      vs1 = u1[p + sl] - u1[p];
+      vs2 = v1[p] - v1[p - yl];
+      vs3 = w1[p] - w1[p - 1];
+      T tmp = xl * (vs1 + vs2 + vs3);
+      const T a1 = qpa * (vs1 + vs2 + vs3);
+      tmp = tmp + DT * a1;
+      f_r = r1[p];
+      f_rtmp = -h * (vs2 + vs3) + a1;
+      t = xx[p] + tmp - xm * (vs2 + vs3) + vx1p * f_r;
+      r1[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+      xx[p] = (t + DT * f_rtmp) * f_dcrj;
+      f_r = r2[p];
+      f_rtmp = -h * (vs1 + vs3) + a1;
+      t = (yy[p] + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
+      r2[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+      yy[p] = (t + DT * f_rtmp) * f_dcrj;
+      f_r = r3[p];
+      f_rtmp = -h * (vs1 + vs2) + a1;
+      t = (zz[p] + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
+      r3[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+      zz[p] = (t + DT * f_rtmp) * f_dcrj;
+      // xy
+      vs1 = u1[p + yl] - u1[p];
+      vs2 = v1[p] - v1[p - sl];
+      f_r = r4[p];
+      f_rtmp = h1 * (vs1 + vs2);
+      t = xy[p] + xmu1 * (vs1 + vs2) + vx1p * f_r;
+      r4[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
+      xy[p] = (t + DT * f_rtmp) * f_dcrj;
+    }
+}
+
+// intp3d (kernel.cu:2359-2561): coarse (gl) -> fine (gh) bilinear 4x4 fill of
+// the fine level align+1 from the coarse level nzt+align-3, weights
+// 1, 2/3, 1/3, 0 in float; per-field staggered offsets (ox, oy).
+template <typename T>
+void intp3d(T* const* L9, T* const* H9, const Grid<T>& gl, const Grid<T>& gh) {
+  // field order: u1 v1 w1 xx yy zz xy xz yz; offsets added to (ii-1, jj-1)
+  static const int ox[9] = {0, 1, 1, 1, 1, 1, 0, 0, 1};
+  static const int oy[9] = {0, 1, 0, 0, 0, 0, 1, 0, 1};
+  const T w[4] = {(T)1., (T)(2. / 3.), (T)(1. / 3.), (T)0.};
+  const long sl = gl.slice, yl = gl.yline;
+  const int k = gl.nzt + ALIGN - 3;
+  for (int i = NGSL; i <= gl.nxt + 3 + NGSL; ++i)
+    for (int j = NGSL; j <= gl.nyt + 3 + NGSL; ++j) {
+      const long p = i * sl + j * yl + k;
+      const int ih = (i - 1 - 1 - NGSL) * 3 + 2 + NGSL;
+      const int jh = (j - 1 - 1 - NGSL) * 3 + 2 + NGSL;
+      const long ph = (long)ih * gh.slice + (long)jh * gh.yline + ALIGN + 1;
+      for (int f = 0; f < 9; ++f) {
+        const T* a = L9[f];
+        const T v0 = a[p], v1 = a[p + sl], v2 = a[p + yl], v3 = a[p + sl + yl];
+        for (int jj = 1; jj <= 4; ++jj)
+          for (int ii = 1; ii <= 4; ++ii) {
+            const long idx = ph + (long)(ii - 1 + ox[f]) * gh.slice + (long)(jj - 1 + oy[f]) * gh.yline;
+            H9[f][idx] = v0 * w[ii - 1] * w[jj - 1] + v1 * w[4 - ii] * w[jj - 1] +
+                         v2 * w[ii - 1] * w[4 - jj] + v3 * w[4 - ii] * w[4 - jj];
+          }
+      }
+    }
+}
+
+// swap3 (kernel.cu:3032-3216): fine (gh) -> coarse (gl) triangular-weighted
+// 5x5x5 averages onto the coarse levels nzt-2 and nzt-1 (+align).  The
+// weights and the accumulation are double; the running sum is float.
+// Fine-grid reads go through bgcaccess (kernel.cu:2809-2852), which takes a
+// halo buffer only for an index outside [0, nxt+20) x [0, nyt+20); with
+// WWL = 2 the index spans [2, nxt_fine+19], so every read is of the array.
+template <typename T>
+void swap3(T* const* L9, const T* const* H9, const Grid<T>& gl, const Grid<T>& gh) {
+  // field order u1 v1 w1 xx yy zz xy xz yz; sbvpos of each (pmcl3d_cons.h:80-88)
+  double ttl = 0.;
+  for (int jj = -WWL; jj <= WWL; ++jj)
+    for (int ii = -WWL; ii <= WWL; ++ii)
+      for (int kk = -WWL; kk <= WWL; ++kk)
+        ttl += (1. + WWL - std::abs(jj)) * (1. + WWL - std::abs(ii)) * (1. + WWL - std::abs(kk));
+  struct Pass { int kc_off, kh, f[3], oi[3], oj[3]; };
+  // (field, offset of ih/jh base: 2 or 3) per block of the reference kernel
+  const Pass passes[5] = {
+      {-2, ALIGN + 3, {0, 1, 6}, {2, 3, 2}, {2, 3, 3}},
+      {-2, ALIGN + 3, {3, 4, 5}, {3, 3, 3}, {2, 2, 2}},
+      {-2, ALIGN + 4, {7, 2, 8}, {2, 3, 3}, {2, 2, 3}},
+      {-1, ALIGN + 6, {0, 1, 6}, {2, 3, 2}, {2, 3, 3}},
+      {-1, ALIGN + 6, {3, 4, 5}, {3, 3, 3}, {2, 2, 2}}};
+  const long sl = gl.slice, yl = gl.yline;
+  for (int i = NGSL; i < gl.nxt + NGSL + 5; ++i)
+    for (int j = NGSL; j < gl.nyt + NGSL + 5; ++j)
+      for (const Pass& ps : passes) {
+        const long p = i * sl + j * yl + ALIGN + gl.nzt + ps.kc_off;
+        T sum[3] = {(T)0., (T)0., (T)0.};
+        for (int jj = -WWL; jj <= WWL; ++jj)
+          for (int ii = -WWL; ii <= WWL; ++ii)
+            for (int kk = -WWL; kk <= WWL; ++kk)
+              for (int q = 0; q < 3; ++q) {
+                const int ih = (i - 1 - 1 - NGSL) * 3 + ps.oi[q] + NGSL;
+                const int jh = (j - 1 - 1 - NGSL) * 3 + ps.oj[q] + NGSL;
+                const T v = H9[ps.f[q]][(long)(ih + ii) * gh.slice + (long)(jh + jj) * gh.yline + ps.kh + kk];
+                sum[q] = (T)((double)sum[q] + 1 / ttl * (1. + WWL - std::abs(ii)) *
+                                                  (1. + WWL - std::abs(jj)) *
+                                                  (1. + WWL - std::abs(kk)) * (double)v);
+              }
+        for (int q = 0; q < 3; ++q) L9[ps.f[q]][p] = sum[q];
+      }
+}
+
 }  // namespace
 
 extern "C" {
@@ -658,5 +886,42 @@ void awp_oracle_rotate_principal_f64(double sigma2, double pfluid, const double*
   }
   rotate_principal<double>(sigma2, pfluid, pc, out9);
 }
+
+#define AWP_ORACLE_DM(SUF, T)                                                            \
+  void awp_oracle_dvelc2_##SUF(T* u1, T* v1, T* w1, const T* xx, const T* yy, const T* zz,  \
+                               const T* xy, const T* xz, const T* yz, const T* dcrjx,       \
+                               const T* dcrjy, const T* dcrjz, const T* d1, int nxt,        \
+                               int nyt, int nzt, double DH, double DT) {                    \
+    ftz_daz();                                                                           \
+    dvelc2<T>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d1,               \
+              make_grid<T>(DH, DT, nxt, nyt, nzt));                                      \
+  }                                                                                      \
+  void awp_oracle_dstrqc2_##SUF(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz, T* r1, T* r2,      \
+                                T* r3, T* r4, T* r5, T* r6, const T* u1, const T* v1,       \
+                                const T* w1, const T* lam, const T* mu, const T* qp,        \
+                                const T* qs, const T* dcrjx, const T* dcrjy,                \
+                                const T* dcrjz, const T* coeff, const T* vx1,               \
+                                const T* vx2, const int* ww, const T* wwo, int nxt,         \
+                                int nyt, int nzt, double DH, double DT, int s_i, int e_i,   \
+                                int s_j, int e_j) {                                         \
+    ftz_daz();                                                                           \
+    dstrqc2<T>(xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp,   \
+               qs, dcrjx, dcrjy, dcrjz, coeff, vx1, vx2, ww, wwo, s_i, e_i, s_j, e_j,      \
+               make_grid<T>(DH, DT, nxt, nyt, nzt));                                     \
+  }                                                                                      \
+  void awp_oracle_intp3d_##SUF(T* const* L9, T* const* H9, int nxtl, int nytl, int nztl,   \
+                               int nxth, int nyth, int nzth) {                             \
+    ftz_daz();                                                                           \
+    intp3d<T>(L9, H9, make_grid<T>(1.0, 1.0, nxtl, nytl, nztl),                          \
+              make_grid<T>(1.0, 1.0, nxth, nyth, nzth));                                  \
+  }                                                                                      \
+  void awp_oracle_swap3_##SUF(T* const* L9, const T* const* H9, int nxtl, int nytl,        \
+                              int nztl, int nxth, int nyth, int nzth) {                    \
+    ftz_daz();                                                                           \
+    swap3<T>(L9, H9, make_grid<T>(1.0, 1.0, nxtl, nytl, nztl),                           \
+             make_grid<T>(1.0, 1.0, nxth, nyth, nzth));                                   \
+  }
+AWP_ORACLE_DM(f32, float)
+AWP_ORACLE_DM(f64, double)
 
 }  // extern "C"

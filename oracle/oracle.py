@@ -47,6 +47,12 @@ def load() -> C.CDLL:
                 [P] * 13 + [I, I, I, D, D, I, I, I, I, P, D])
             getattr(_lib, f"awp_oracle_drprecpc_app_{suf}").argtypes = (
                 [P] * 8 + [I, I, I, D, I, I, I, I, P])
+        for suf in ("f32", "f64"):
+            getattr(_lib, f"awp_oracle_dvelc2_{suf}").argtypes = [P] * 13 + [I, I, I, D, D]
+            getattr(_lib, f"awp_oracle_dstrqc2_{suf}").argtypes = (
+                [P] * 27 + [I, I, I, D, D, I, I, I, I])
+            getattr(_lib, f"awp_oracle_intp3d_{suf}").argtypes = [P, P] + [I] * 6
+            getattr(_lib, f"awp_oracle_swap3_{suf}").argtypes = [P, P] + [I] * 6
         _lib.awp_oracle_rotate_principal_f64.argtypes = [D, D, P, P]
         _lib.awp_oracle_dstrqc_new_f32.argtypes = [P] * 28 + [I, I, I, D, D] + [I] * 9
         _lib.awp_oracle_set_threads.argtypes = [I]
@@ -65,9 +71,51 @@ def _p(a: np.ndarray) -> int:
 
 FIELDS = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
           "r1", "r2", "r3", "r4", "r5", "r6")
+# the oracle's nine-field pack order for the grid coupling
+PACK9 = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")
+
+
+def _as_list(x, n):
+    if isinstance(x, (list, tuple)):
+        if len(x) != n:
+            raise ValueError(f"expected {n} per-grid entries, got {len(x)}")
+        return list(x)
+    return [x] + [None] * (n - 1)
+
+
+class _Level:
+    def __init__(self, medium, sources, plastic, dtype):
+        self.medium = medium
+        self.dims = d = medium.dims
+        self.f = {n: np.zeros(d.shape, dtype=dtype) for n in FIELDS}
+        self.m = {}
+        for n in ("d1", "lam", "mu", "qp", "qs", "vx1", "vx2", "wwo", "coeff", "lam_mu",
+                  "dcrjx", "dcrjy", "dcrjz"):
+            self.m[n] = np.ascontiguousarray(getattr(medium, n), dtype=dtype)
+        self.m["ww"] = np.ascontiguousarray(medium.ww, dtype=np.int32)
+        self.DH, self.DT = float(medium.DH), float(medium.DT)
+        self.src = None
+        if sources is not None and sources.npsrc > 0:
+            self.src = dict(psrc=np.ascontiguousarray(sources.psrc(d).reshape(-1), dtype=np.int32),
+                            cols=[np.ascontiguousarray(c, dtype=dtype) for c in sources.columns()],
+                            npsrc=sources.npsrc, nst=sources.nst)
+        self.plastic = plastic
+        if plastic is not None:
+            for n in ("sigma2", "cohes", "phi"):
+                self.m[n] = np.ascontiguousarray(getattr(plastic, n), dtype=dtype)
+            self.f["yldfac"] = np.ones(d.shape, dtype=dtype)
+            self.f["neta"] = np.zeros(d.shape, dtype=dtype)
+            self.pl = np.ascontiguousarray(plastic.params.packed())
+
+    def pack(self, names=PACK9):
+        arr = (C.c_void_p * 9)(*[_p(self.f[n]) for n in names])
+        return arr
 
 
 class OracleSolver(PmclStepOrder):
+    """medium: one Medium or a per-grid list (grid 0 on top); sources and
+    plastic: one entry (grid 0) or per-grid lists; receivers on grid 0."""
+
     def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
                  dtype=np.float32, variant: str = "dstrqc", NPC: int = 0, plastic=None):
         self.lib = load()
@@ -77,47 +125,80 @@ class OracleSolver(PmclStepOrder):
             raise ValueError("dstrqc_new literal variant exists in float32 only")
         self.variant = variant
         self.NPC = NPC
-        self.medium = medium
-        self.dims = d = medium.dims
-        self.f = {n: np.zeros(d.shape, dtype=self.dtype) for n in FIELDS}
-        self.m = {}
-        for n in ("d1", "lam", "mu", "qp", "qs", "vx1", "vx2", "wwo", "coeff", "lam_mu",
-                  "dcrjx", "dcrjy", "dcrjz"):
-            self.m[n] = np.ascontiguousarray(getattr(medium, n), dtype=self.dtype)
-        self.m["ww"] = np.ascontiguousarray(medium.ww, dtype=np.int32)
-        self.DH, self.DT = float(medium.DH), float(medium.DT)
-        self.src = None
-        if sources is not None and sources.npsrc > 0:
-            self.src = dict(psrc=np.ascontiguousarray(sources.psrc(d).reshape(-1), dtype=np.int32),
-                            cols=[np.ascontiguousarray(c, dtype=self.dtype) for c in sources.columns()],
-                            npsrc=sources.npsrc, nst=sources.nst)
+        media = list(medium) if isinstance(medium, (list, tuple)) else [medium]
+        G = self.ngrids = len(media)
+        srcs = _as_list(sources, G)
+        pls = _as_list(plastic, G) if plastic is not None else [None] * G
+        self.plastic = pls if plastic is not None else None
+        self.levels = [_Level(media[p], srcs[p], pls[p], self.dtype) for p in range(G)]
+        d = self.levels[0].dims
         self.rec = None
         if receivers is not None and len(receivers.xy) > 0:
-            self.rec = dict(idx=receivers.array_index(d), out=np.zeros((max(nt_record, 1), len(receivers.xy), 3), dtype=self.dtype))
-        self.plastic = plastic
-        if plastic is not None:
-            for n in ("sigma2", "cohes", "phi"):
-                self.m[n] = np.ascontiguousarray(getattr(plastic, n), dtype=self.dtype)
-            self.f["yldfac"] = np.ones(d.shape, dtype=self.dtype)
-            self.f["neta"] = np.zeros(d.shape, dtype=self.dtype)
-            self.pl = np.ascontiguousarray(plastic.params.packed())
+            self.rec = dict(idx=receivers.array_index(d),
+                            out=np.zeros((max(nt_record, 1), len(receivers.xy), 3),
+                                         dtype=self.dtype))
         self.cur_step = 0
 
-    def velocity(self) -> None:
-        d, f, m = self.dims, self.f, self.m
+    @property
+    def dims(self):
+        return self.levels[0].dims
+
+    @property
+    def medium(self):
+        return self.levels[0].medium
+
+    @property
+    def f(self):
+        return self.levels[0].f
+
+    @property
+    def m(self):
+        return self.levels[0].m
+
+    def grid_dims(self, p):
+        return self.levels[p].dims
+
+    def source_steps(self, p):
+        s = self.levels[p].src
+        return s["nst"] if s is not None else 0
+
+    def velocity(self, p: int = 0) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
         s_i, e_i = d.i_range
         getattr(self.lib, f"awp_oracle_dvelcx_{self.suf}")(
             *(_p(f[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")),
             _p(m["dcrjx"]), _p(m["dcrjy"]), _p(m["dcrjz"]), _p(m["d1"]), d.nxt, d.nyt, d.nzt,
-            self.DH, self.DT, s_i, e_i, 0, 1)
+            L.DH, L.DT, s_i, e_i, p, self.ngrids)
 
-    def stress(self) -> None:
-        d = self.dims
+    def velocity_dm(self, p: int) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
+        getattr(self.lib, f"awp_oracle_dvelc2_{self.suf}")(
+            *(_p(f[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")),
+            _p(m["dcrjx"]), _p(m["dcrjy"]), _p(m["dcrjz"]), _p(m["d1"]), d.nxt, d.nyt, d.nzt,
+            L.DH, L.DT)
+
+    def interpolate(self, p: int) -> None:
+        lo, hi = self.levels[p], self.levels[p - 1]
+        getattr(self.lib, f"awp_oracle_intp3d_{self.suf}")(
+            lo.pack(), hi.pack(), lo.dims.nxt, lo.dims.nyt, lo.dims.nzt, hi.dims.nxt,
+            hi.dims.nyt, hi.dims.nzt)
+
+    def swap(self, p: int) -> None:
+        hi, lo = self.levels[p], self.levels[p + 1]
+        getattr(self.lib, f"awp_oracle_swap3_{self.suf}")(
+            lo.pack(), hi.pack(), lo.dims.nxt, lo.dims.nyt, lo.dims.nzt, hi.dims.nxt,
+            hi.dims.nyt, hi.dims.nzt)
+
+    def stress(self, p: int = 0) -> None:
+        d = self.levels[p].dims
         (s_i, e_i), (s_j, e_j) = d.i_range, d.j_range
-        self.stress_region(s_i, e_i, s_j, e_j)
+        self.stress_region(p, s_i, e_i, s_j, e_j)
 
-    def stress_region(self, s_i, e_i, s_j, e_j) -> None:
-        d, f, m = self.dims, self.f, self.m
+    def stress_region(self, p, s_i, e_i, s_j, e_j) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:
             return
         name = (f"awp_oracle_dstrqc_{self.suf}" if self.variant == "dstrqc"
@@ -127,31 +208,44 @@ class OracleSolver(PmclStepOrder):
                                  "r5", "r6", "u1", "v1", "w1")),
             _p(m["lam"]), _p(m["mu"]), _p(m["qp"]), _p(m["coeff"]), _p(m["qs"]),
             _p(m["dcrjx"]), _p(m["dcrjy"]), _p(m["dcrjz"]), _p(m["lam_mu"]), _p(m["vx1"]),
-            _p(m["vx2"]), _p(m["ww"]), _p(m["wwo"]), d.nxt, d.nyt, d.nzt, self.DH, self.DT,
-            d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, 0)
+            _p(m["vx2"]), _p(m["ww"]), _p(m["wwo"]), d.nxt, d.nyt, d.nzt, L.DH, L.DT,
+            d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, p)
 
-    def plastic_calc(self, xs, xe, ys, ye) -> None:
-        d, f, m = self.dims, self.f, self.m
+    def stress_dm(self, p, s_i, e_i, s_j, e_j) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
+        getattr(self.lib, f"awp_oracle_dstrqc2_{self.suf}")(
+            *(_p(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                 "r5", "r6", "u1", "v1", "w1")),
+            _p(m["lam"]), _p(m["mu"]), _p(m["qp"]), _p(m["qs"]), _p(m["dcrjx"]),
+            _p(m["dcrjy"]), _p(m["dcrjz"]), _p(m["coeff"]), _p(m["vx1"]), _p(m["vx2"]),
+            _p(m["ww"]), _p(m["wwo"]), d.nxt, d.nyt, d.nzt, L.DH, L.DT, s_i, e_i, s_j, e_j)
+
+    def plastic_calc(self, p, xs, xe, ys, ye) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
         getattr(self.lib, f"awp_oracle_drprecpc_calc_{self.suf}")(
             *(_p(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")), _p(m["mu"]), _p(m["d1"]),
             _p(m["sigma2"]), _p(f["yldfac"]), _p(m["cohes"]), _p(m["phi"]), _p(f["neta"]),
-            d.nxt, d.nyt, d.nzt, self.DH, self.DT, xs, xe, ys, ye, _p(self.pl),
-            float(self.plastic.params.Tv))
+            d.nxt, d.nyt, d.nzt, L.DH, L.DT, xs, xe, ys, ye, _p(L.pl),
+            float(L.plastic.params.Tv))
 
-    def plastic_app(self, xs, xe, ys, ye) -> None:
-        d, f, m = self.dims, self.f, self.m
+    def plastic_app(self, p, xs, xe, ys, ye) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
         getattr(self.lib, f"awp_oracle_drprecpc_app_{self.suf}")(
             *(_p(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz")), _p(m["sigma2"]),
-            _p(f["yldfac"]), d.nxt, d.nyt, d.nzt, self.DH, xs, xe, ys, ye, _p(self.pl))
+            _p(f["yldfac"]), d.nxt, d.nyt, d.nzt, L.DH, xs, xe, ys, ye, _p(L.pl))
 
-    def add_source(self, sample_step: int) -> None:
-        if self.src is None:
+    def add_source(self, p: int, sample_step: int) -> None:
+        L = self.levels[p]
+        if L.src is None:
             return
-        f, s, d = self.f, self.src, self.dims
+        f, s, d = L.f, L.src, L.dims
         getattr(self.lib, f"awp_oracle_addsrc_{self.suf}")(
             sample_step, s["nst"], 3, _p(s["psrc"]), s["npsrc"], *(_p(c) for c in s["cols"]),
             _p(f["xx"]), _p(f["yy"]), _p(f["zz"]), _p(f["xy"]), _p(f["yz"]), _p(f["xz"]),
-            d.nxt, d.nyt, d.nzt, self.DH, self.DT)
+            d.nxt, d.nyt, d.nzt, L.DH, L.DT)
 
     def free_surface(self) -> None:
         d, f = self.dims, self.f
