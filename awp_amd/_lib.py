@@ -38,6 +38,15 @@ SIGNATURES: dict[str, list] = {
     "dstrqc2_H": [_fp] * 22 + [_i, _i, _s] + [_fp] * 5 + [_i] * 5,
     "intp3d_H": [_fp] * 18 + [_i, _i, _i, _s, _i],
     "swap_H": [_fp] * 18 + [_i, _i] + [_fp] * 4 + [_i, _s, _i],
+    "awp_dvelcx_s16": [_fp] * 13 + [_i, _i, _s, _i, _i, _i, _i, _i, _fp],
+    "awp_dstrqc_s16": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9 + [_i, _fp],
+    "awp_fstr_s16": [_fp] * 3 + [_s, _i, _i, _i, _i, _i, _fp],
+    "awp_addsrc_s16": [_i, _i, _i, _fp, _i, _s] + [_fp] * 12 + [_i, _i, _fp],
+    "awp_receivers_gather_s16": [_fp] * 4 + [_i, _fp, _i, _s, _i, _i, _fp],
+    "awp_s16_encode": [_fp, _fp, C.c_longlong, _i, _f, _s],
+    "awp_s16_decode": [_fp, _fp, C.c_longlong, _i, _f, _s],
+    "awp_s16_rescale": [_fp, C.c_longlong, _i, _i, _s],
+    "awp_s16_absmax": [_fp, C.c_longlong, _i, _fp, _s],
 }
 RESTYPES = {
     "awp_last_error": _i,

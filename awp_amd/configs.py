@@ -142,7 +142,7 @@ def von_karman(shape: tuple[int, int, int], h: float, a: float, nu: float, std: 
     amp = (1.0 + k2 * a * a) ** (-(nu + 1.5) / 2.0)
     F *= amp.astype(np.float32)
     del amp, k2
-    f = np.fft.irfftn(F, s=shape).astype(np.float32)
+    f = np.fft.irfftn(F, s=shape, axes=(0, 1, 2)).astype(np.float32)
     f -= f.mean()
     f *= std / f.std()
     return f

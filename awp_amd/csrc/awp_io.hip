@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include "awp_common.h"
+#include "awp_store.h"
 #include "awp_tuning.h"
 #include "../../include/awp_hip.h"
 
@@ -21,15 +22,13 @@ struct SrcLaunch {
 };
 
 // One lane per source, as the reference.
-__global__ __launch_bounds__(256) void addsrc_kernel(const int* __restrict__ psrc,
-                                                     const float* __restrict__ axx,
-                                                     const float* __restrict__ ayy,
-                                                     const float* __restrict__ azz,
-                                                     const float* __restrict__ axz,
-                                                     const float* __restrict__ ayz,
-                                                     const float* __restrict__ axy, float* xx,
-                                                     float* yy, float* zz, float* xy, float* yz,
-                                                     float* xz, const SrcLaunch A) {
+template <class ST>
+__global__ __launch_bounds__(256) void addsrc_kernel(
+    const int* __restrict__ psrc, const float* __restrict__ axx, const float* __restrict__ ayy,
+    const float* __restrict__ azz, const float* __restrict__ axz, const float* __restrict__ ayz,
+    const float* __restrict__ axy, typename ST::T* xx, typename ST::T* yy, typename ST::T* zz,
+    typename ST::T* xy, typename ST::T* yz, typename ST::T* xz, const SrcLaunch A,
+    const WaveScales SC) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= A.npsrc) return;
   const int idx = psrc[j * A.dim] + 1 + AWP_NGSL;
@@ -38,12 +37,12 @@ __global__ __launch_bounds__(256) void addsrc_kernel(const int* __restrict__ psr
   const long long p = (long long)idx * A.slice + (long long)idy * A.yline + idz;
   const long long s = (long long)j * A.read_step + A.sample;
   const float v = A.vtst;
-  xx[p] = xx[p] - v * axx[s];
-  yy[p] = yy[p] - v * ayy[s];
-  zz[p] = zz[p] - v * azz[s];
-  xz[p] = xz[p] - v * axz[s];
-  yz[p] = yz[p] - v * ayz[s];
-  xy[p] = xy[p] - v * axy[s];
+  ST::st(xx, p, ST::ld(xx, p, SC.s[AWP_SXX]) - v * axx[s], SC.inv[AWP_SXX]);
+  ST::st(yy, p, ST::ld(yy, p, SC.s[AWP_SYY]) - v * ayy[s], SC.inv[AWP_SYY]);
+  ST::st(zz, p, ST::ld(zz, p, SC.s[AWP_SZZ]) - v * azz[s], SC.inv[AWP_SZZ]);
+  ST::st(xz, p, ST::ld(xz, p, SC.s[AWP_SXZ]) - v * axz[s], SC.inv[AWP_SXZ]);
+  ST::st(yz, p, ST::ld(yz, p, SC.s[AWP_SYZ]) - v * ayz[s], SC.inv[AWP_SYZ]);
+  ST::st(xy, p, ST::ld(xy, p, SC.s[AWP_SXY]) - v * axy[s], SC.inv[AWP_SXY]);
 }
 
 struct VbLaunch {
@@ -92,20 +91,73 @@ struct RecLaunch {
   long long slice;
 };
 
-__global__ __launch_bounds__(256) void receivers_kernel(const float* __restrict__ u1,
-                                                        const float* __restrict__ v1,
-                                                        const float* __restrict__ w1,
+template <class ST>
+__global__ __launch_bounds__(256) void receivers_kernel(const typename ST::T* __restrict__ u1,
+                                                        const typename ST::T* __restrict__ v1,
+                                                        const typename ST::T* __restrict__ w1,
                                                         const int* __restrict__ idx,
                                                         float* __restrict__ out,
-                                                        const RecLaunch R) {
+                                                        const RecLaunch R, const WaveScales SC) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= R.npts) return;
   const long long p =
       (long long)idx[3 * n] * R.slice + (long long)idx[3 * n + 1] * R.yline + idx[3 * n + 2];
   float* o = out + (long long)R.slot * 3 * R.npts + 3 * n;
-  o[0] = u1[p];
-  o[1] = v1[p];
-  o[2] = w1[p - R.zshift];
+  o[0] = ST::ld(u1, p, SC.s[AWP_SU]);
+  o[1] = ST::ld(v1, p, SC.s[AWP_SV]);
+  o[2] = ST::ld(w1, p - R.zshift, SC.s[AWP_SW]);
+}
+
+template <class ST>
+void addsrc_common(const char* W, int i, int READ_STEP, int dim, int* psrc, int npsrc,
+                   hipStream_t St, float* axx, float* ayy, float* azz, float* axz, float* ayz,
+                   float* axy, void* xx, void* yy, void* zz, void* xy, void* yz, void* xz,
+                   int d_i, const WaveScales& SC) {
+  typedef typename ST::T T;
+  if (!awp_grid_ok(d_i, W)) return;
+  if (npsrc <= 0) return;
+  if (i < 1 || i > READ_STEP || dim < 3) {
+    awp_set_error(AWP_ERR_ARG, "%s: sample %d outside [1, READ_STEP=%d] or dim=%d < 3", W, i,
+                  READ_STEP, dim);
+    return;
+  }
+  const AwpGrid& g = awp_state().grid[d_i];
+  SrcLaunch A;
+  A.sample = i - 1;
+  A.read_step = READ_STEP;
+  A.dim = dim;
+  A.npsrc = npsrc;
+  A.yline = g.yline;
+  A.slice = g.slice;
+  A.vtst = g.DT / (g.DH * g.DH * g.DH);  // kernel.cu:1790, (_prec)d_DT/(DH*DH*DH)
+  const int block = npsrc < 256 ? npsrc : 256;
+  hipLaunchKernelGGL((addsrc_kernel<ST>), dim3(awp_cdiv(npsrc, block)), dim3(block), 0, St, psrc,
+                     axx, ayy, azz, axz, ayz, axy, (T*)xx, (T*)yy, (T*)zz, (T*)xy, (T*)yz,
+                     (T*)xz, A, SC);
+  awp_check_launch(W);
+}
+
+template <class ST>
+void receivers_common(const char* W, const void* u1, const void* v1, const void* w1,
+                      const int* idx, int npts, float* out, int step_slot, hipStream_t St,
+                      int d_i, const WaveScales& SC) {
+  typedef typename ST::T T;
+  if (!awp_grid_ok(d_i, W)) return;
+  if (npts <= 0) return;
+  if (step_slot < 0) {
+    awp_set_error(AWP_ERR_ARG, "%s: negative step_slot %d", W, step_slot);
+    return;
+  }
+  const AwpGrid& g = awp_state().grid[d_i];
+  RecLaunch R;
+  R.npts = npts;
+  R.slot = step_slot;
+  R.zshift = (d_i == 0) ? 1 : 0;
+  R.yline = g.yline;
+  R.slice = g.slice;
+  hipLaunchKernelGGL((receivers_kernel<ST>), dim3(awp_cdiv(npts, 256)), dim3(256), 0, St,
+                     (const T*)u1, (const T*)v1, (const T*)w1, idx, out, R, SC);
+  awp_check_launch(W);
 }
 
 }  // namespace
@@ -140,27 +192,24 @@ int awp_set_tuning(const char* key, int value) {
 void addsrc_H(int i, int READ_STEP, int dim, int* psrc, int npsrc, hipStream_t St,
               float* axx, float* ayy, float* azz, float* axz, float* ayz, float* axy,
               float* xx, float* yy, float* zz, float* xy, float* yz, float* xz, int d_i) {
-  const char* W = "addsrc_H";
-  if (!awp_grid_ok(d_i, W)) return;
-  if (npsrc <= 0) return;
-  if (i < 1 || i > READ_STEP || dim < 3) {
-    awp_set_error(AWP_ERR_ARG, "%s: sample %d outside [1, READ_STEP=%d] or dim=%d < 3", W, i,
-                  READ_STEP, dim);
-    return;
-  }
-  const AwpGrid& g = awp_state().grid[d_i];
-  SrcLaunch A;
-  A.sample = i - 1;
-  A.read_step = READ_STEP;
-  A.dim = dim;
-  A.npsrc = npsrc;
-  A.yline = g.yline;
-  A.slice = g.slice;
-  A.vtst = g.DT / (g.DH * g.DH * g.DH);  // kernel.cu:1790, (_prec)d_DT/(DH*DH*DH)
-  const int block = npsrc < 256 ? npsrc : 256;
-  hipLaunchKernelGGL(addsrc_kernel, dim3(awp_cdiv(npsrc, block)), dim3(block), 0, St, psrc, axx,
-                     ayy, azz, axz, ayz, axy, xx, yy, zz, xy, yz, xz, A);
-  awp_check_launch(W);
+  WaveScales SC = {};
+  addsrc_common<StF32>("addsrc_H", i, READ_STEP, dim, psrc, npsrc, St, axx, ayy, azz, axz, ayz,
+                       axy, xx, yy, zz, xy, yz, xz, d_i, SC);
+}
+
+void awp_addsrc_s16(int i, int READ_STEP, int dim, int* psrc, int npsrc, hipStream_t St,
+                    float* axx, float* ayy, float* azz, float* axz, float* ayz, float* axy,
+                    uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* yz,
+                    uint16_t* xz, int d_i, int fmt, const float* scale) {
+  const char* W = "awp_addsrc_s16";
+  WaveScales SC;
+  if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
+  if (fmt == AWP_FMT_BF16)
+    addsrc_common<StBF16>(W, i, READ_STEP, dim, psrc, npsrc, St, axx, ayy, azz, axz, ayz, axy,
+                          xx, yy, zz, xy, yz, xz, d_i, SC);
+  else
+    addsrc_common<StF16>(W, i, READ_STEP, dim, psrc, npsrc, St, axx, ayy, azz, axz, ayz, axy,
+                         xx, yy, zz, xy, yz, xz, d_i, SC);
 }
 
 void velbuffer_H(const float* u1, const float* v1, const float* w1, const float* neta,
@@ -192,23 +241,21 @@ void velbuffer_H(const float* u1, const float* v1, const float* w1, const float*
 
 void awp_receivers_gather(const float* u1, const float* v1, const float* w1, const int* idx,
                           int npts, float* out, int step_slot, hipStream_t St, int d_i) {
-  const char* W = "awp_receivers_gather";
-  if (!awp_grid_ok(d_i, W)) return;
-  if (npts <= 0) return;
-  if (step_slot < 0) {
-    awp_set_error(AWP_ERR_ARG, "%s: negative step_slot %d", W, step_slot);
-    return;
-  }
-  const AwpGrid& g = awp_state().grid[d_i];
-  RecLaunch R;
-  R.npts = npts;
-  R.slot = step_slot;
-  R.zshift = (d_i == 0) ? 1 : 0;
-  R.yline = g.yline;
-  R.slice = g.slice;
-  hipLaunchKernelGGL(receivers_kernel, dim3(awp_cdiv(npts, 256)), dim3(256), 0, St, u1, v1, w1,
-                     idx, out, R);
-  awp_check_launch(W);
+  WaveScales SC = {};
+  receivers_common<StF32>("awp_receivers_gather", u1, v1, w1, idx, npts, out, step_slot, St,
+                          d_i, SC);
+}
+
+void awp_receivers_gather_s16(const uint16_t* u1, const uint16_t* v1, const uint16_t* w1,
+                              const int* idx, int npts, float* out, int step_slot,
+                              hipStream_t St, int d_i, int fmt, const float* scale) {
+  const char* W = "awp_receivers_gather_s16";
+  WaveScales SC;
+  if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
+  if (fmt == AWP_FMT_BF16)
+    receivers_common<StBF16>(W, u1, v1, w1, idx, npts, out, step_slot, St, d_i, SC);
+  else
+    receivers_common<StF16>(W, u1, v1, w1, idx, npts, out, step_slot, St, d_i, SC);
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 // Operation order and literal types follow the reference (double-literal
 // sub-expressions are evaluated in double), built with -ffp-contract=off.
 #include "awp_common.h"
+#include "awp_store.h"
 #include "awp_tuning.h"
 #include "../../include/awp_hip.h"
 
@@ -50,19 +51,22 @@ __device__ __forceinline__ float qw_new(float q, float ca, float cb, float wwo, 
   return w / wwo;
 }
 
-template <int BJ, bool NEWLIT>
+template <int BJ, bool NEWLIT, class ST>
 __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
-    float* __restrict__ xx, float* __restrict__ yy, float* __restrict__ zz,
-    float* __restrict__ xy, float* __restrict__ xz, float* __restrict__ yz,
-    float* __restrict__ r1, float* __restrict__ r2, float* __restrict__ r3,
-    float* __restrict__ r4, float* __restrict__ r5, float* __restrict__ r6,
-    const float* __restrict__ u1, const float* __restrict__ v1, const float* __restrict__ w1,
+    typename ST::T* __restrict__ xx, typename ST::T* __restrict__ yy,
+    typename ST::T* __restrict__ zz, typename ST::T* __restrict__ xy,
+    typename ST::T* __restrict__ xz, typename ST::T* __restrict__ yz,
+    typename ST::T* __restrict__ r1, typename ST::T* __restrict__ r2,
+    typename ST::T* __restrict__ r3, typename ST::T* __restrict__ r4,
+    typename ST::T* __restrict__ r5, typename ST::T* __restrict__ r6,
+    const typename ST::T* __restrict__ u1, const typename ST::T* __restrict__ v1,
+    const typename ST::T* __restrict__ w1,
     const float* __restrict__ lam, const float* __restrict__ mu, const float* __restrict__ qp,
     const float* __restrict__ coeff, const float* __restrict__ qs,
     const float* __restrict__ dcrjx, const float* __restrict__ dcrjy,
     const float* __restrict__ dcrjz, const float* __restrict__ vx1,
     const float* __restrict__ vx2, const int* __restrict__ ww, const float* __restrict__ wwo,
-    const StrLaunch L) {
+    const StrLaunch L, const WaveScales SC) {
   __shared__ float s_coeff[16];
   const int tid = threadIdx.y * 64 + threadIdx.x;
   if (tid < 16) s_coeff[tid] = coeff[tid];
@@ -86,11 +90,12 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
   const float dy = dcrjy[j];
   const float dz = dcrjz[k];
 
+  const float su = SC.s[AWP_SU], sv = SC.s[AWP_SV], sw = SC.s[AWP_SW];
   const long long p0 = (long long)ia * S + o;
   // x windows: u needs i-1..i+2, v and w need i-2..i+1
-  float u_m1 = u1[p0 - S], u_0 = u1[p0], u_p1 = u1[p0 + S];
-  float v_m2 = v1[p0 - 2 * S], v_m1 = v1[p0 - S], v_0 = v1[p0];
-  float w_m2 = w1[p0 - 2 * S], w_m1 = w1[p0 - S], w_0 = w1[p0];
+  float u_m1 = ST::ld(u1, p0 - S, su), u_0 = ST::ld(u1, p0, su), u_p1 = ST::ld(u1, p0 + S, su);
+  float v_m2 = ST::ld(v1, p0 - 2 * S, sv), v_m1 = ST::ld(v1, p0 - S, sv), v_0 = ST::ld(v1, p0, sv);
+  float w_m2 = ST::ld(w1, p0 - 2 * S, sw), w_m1 = ST::ld(w1, p0 - S, sw), w_0 = ST::ld(w1, p0, sw);
   // material corners at plane i: (j,k), (j-1,k), (j,k-1), (j-1,k-1)
   float lc = lam[p0], ljm = lam[p0 - Y], lkm = lam[p0 - 1], ljk = lam[p0 - Y - 1];
   float mc = mu[p0], mjm = mu[p0 - Y], mkm = mu[p0 - 1], mjk = mu[p0 - Y - 1];
@@ -100,27 +105,27 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
   for (int s = 0; s < nsteps; ++s) {
     const int i = ia + s;
     const long long pl = (long long)i * S;
-    const float* up = u1 + pl;
-    const float* vp = v1 + pl;
-    const float* wp = w1 + pl;
+    const typename ST::T* up = u1 + pl;
+    const typename ST::T* vp = v1 + pl;
+    const typename ST::T* wp = w1 + pl;
     const long long pn = pl + S;  // plane i+1
 
     // leads
-    const float u_p2 = up[2 * S + o];
-    const float v_p1 = vp[S + o];
-    const float w_p1 = wp[S + o];
+    const float u_p2 = ST::ld(up, 2 * S + o, su);
+    const float v_p1 = ST::ld(vp, S + o, sv);
+    const float w_p1 = ST::ld(wp, S + o, sw);
     const float Lc = lam[pn + o], Ljm = lam[pn + o - Y], Lkm = lam[pn + o - 1], Ljk = lam[pn + o - Y - 1];
     const float Mc = mu[pn + o], Mjm = mu[pn + o - Y], Mkm = mu[pn + o - 1], Mjk = mu[pn + o - Y - 1];
     const float Pc = qp[pn + o], Pjm = qp[pn + o - Y], Pkm = qp[pn + o - 1], Pjk = qp[pn + o - Y - 1];
     const float Sc = qs[pn + o], Sjm = qs[pn + o - Y], Skm = qs[pn + o - 1], Sjk = qs[pn + o - Y - 1];
 
     // in-plane stencils
-    const float u_jp1 = up[o + Y], u_jp2 = up[o + 2 * Y], u_jm1 = up[o - Y];
-    const float u_kp1 = up[o + 1], u_kp2 = up[o + 2], u_km1 = up[o - 1];
-    const float v_jm1 = vp[o - Y], v_jp1 = vp[o + Y], v_jm2 = vp[o - 2 * Y];
-    const float v_kp1 = vp[o + 1], v_kp2 = vp[o + 2], v_km1 = vp[o - 1];
-    const float w_km1 = wp[o - 1], w_kp1 = wp[o + 1], w_km2 = wp[o - 2];
-    const float w_jp1 = wp[o + Y], w_jp2 = wp[o + 2 * Y], w_jm1 = wp[o - Y];
+    const float u_jp1 = ST::ld(up, o + Y, su), u_jp2 = ST::ld(up, o + 2 * Y, su), u_jm1 = ST::ld(up, o - Y, su);
+    const float u_kp1 = ST::ld(up, o + 1, su), u_kp2 = ST::ld(up, o + 2, su), u_km1 = ST::ld(up, o - 1, su);
+    const float v_jm1 = ST::ld(vp, o - Y, sv), v_jp1 = ST::ld(vp, o + Y, sv), v_jm2 = ST::ld(vp, o - 2 * Y, sv);
+    const float v_kp1 = ST::ld(vp, o + 1, sv), v_kp2 = ST::ld(vp, o + 2, sv), v_km1 = ST::ld(vp, o - 1, sv);
+    const float w_km1 = ST::ld(wp, o - 1, sw), w_kp1 = ST::ld(wp, o + 1, sw), w_km2 = ST::ld(wp, o - 2, sw);
+    const float w_jp1 = ST::ld(wp, o + Y, sw), w_jp2 = ST::ld(wp, o + 2 * Y, sw), w_jm1 = ST::ld(wp, o - Y, sw);
 
     const long long p = pl + o;
     const float f_vx1 = vx1[p], f_vx2 = vx2[p], f_wwo = wwo[p];
@@ -200,53 +205,53 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     const float one_m_vx1 = 1 - f_vx1;
     const float wwo_m1 = f_wwo - 1;
 
-    f_r = r1[p];
+    f_r = ST::ld(r1, p, SC.s[AWP_SR1]);
     f_rtmp = -h * (vs2 + vs3) + a1;
-    t = xx[p] + tmp - xm * (vs2 + vs3) + vx1p * f_r;
-    r1[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    t = ST::ld(xx, p, SC.s[AWP_SXX]) + tmp - xm * (vs2 + vs3) + vx1p * f_r;
+    ST::st(r1, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR1]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-    xx[p] = (t + DT * f_rtmp) * f_dcrj;
+    ST::st(xx, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SXX]);
 
-    f_r = r2[p];
+    f_r = ST::ld(r2, p, SC.s[AWP_SR2]);
     f_rtmp = -h * (vs1 + vs3) + a1;
-    t = (yy[p] + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
-    r2[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    t = (ST::ld(yy, p, SC.s[AWP_SYY]) + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
+    ST::st(r2, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR2]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-    yy[p] = (t + DT * f_rtmp) * f_dcrj;
+    ST::st(yy, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SYY]);
 
-    f_r = r3[p];
+    f_r = ST::ld(r3, p, SC.s[AWP_SR3]);
     f_rtmp = -h * (vs1 + vs2) + a1;
-    t = (zz[p] + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
-    r3[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    t = (ST::ld(zz, p, SC.s[AWP_SZZ]) + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
+    ST::st(r3, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR3]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-    zz[p] = (t + DT * f_rtmp) * f_dcrj;
+    ST::st(zz, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SZZ]);
 
     vs1 = c1 * (u_jp1 - u_0) + c2 * (u_jp2 - u_jm1);
     vs2 = c1 * (v_0 - v_m1) + c2 * (v_p1 - v_m2);
-    f_r = r4[p];
+    f_r = ST::ld(r4, p, SC.s[AWP_SR4]);
     f_rtmp = h1 * (vs1 + vs2);
-    t = xy[p] + xmu1 * (vs1 + vs2) + vx1p * f_r;
-    r4[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    t = ST::ld(xy, p, SC.s[AWP_SXY]) + xmu1 * (vs1 + vs2) + vx1p * f_r;
+    ST::st(r4, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR4]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-    xy[p] = (t + DT * f_rtmp) * f_dcrj;
+    ST::st(xy, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SXY]);
 
     vs1 = c1 * (u_kp1 - u_0) + c2 * (u_kp2 - u_km1);
     vs2 = c1 * (w_0 - w_m1) + c2 * (w_p1 - w_m2);
-    f_r = r5[p];
+    f_r = ST::ld(r5, p, SC.s[AWP_SR5]);
     f_rtmp = h2 * (vs1 + vs2);
-    t = xz[p] + xmu2 * (vs1 + vs2) + vx1p * f_r;
-    r5[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    t = ST::ld(xz, p, SC.s[AWP_SXZ]) + xmu2 * (vs1 + vs2) + vx1p * f_r;
+    ST::st(r5, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR5]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-    xz[p] = (t + DT * f_rtmp) * f_dcrj;
+    ST::st(xz, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SXZ]);
 
     vs1 = c1 * (v_kp1 - v_0) + c2 * (v_kp2 - v_km1);
     vs2 = c1 * (w_jp1 - w_0) + c2 * (w_jp2 - w_jm1);
-    f_r = r6[p];
+    f_r = ST::ld(r6, p, SC.s[AWP_SR6]);
     f_rtmp = h3 * (vs1 + vs2);
-    t = yz[p] + xmu3 * (vs1 + vs2) + vx1p * f_r;
-    r6[p] = f_vx2 * f_r + f_wwo * f_rtmp;
+    t = ST::ld(yz, p, SC.s[AWP_SYZ]) + xmu3 * (vs1 + vs2) + vx1p * f_r;
+    ST::st(r6, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR6]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-    yz[p] = (t + DT * f_rtmp) * f_dcrj;
+    ST::st(yz, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SYZ]);
 
     // advance x windows and material corners
     u_m1 = u_0; u_0 = u_p1; u_p1 = u_p2;
@@ -271,46 +276,51 @@ struct FsLaunch {
   long long slice;
 };
 
-template <bool NEWLIT>
-__global__ __launch_bounds__(256) void fs_ghost_kernel(float* __restrict__ u1,
-                                                       float* __restrict__ v1,
-                                                       float* __restrict__ w1,
+template <bool NEWLIT, class ST>
+__global__ __launch_bounds__(256) void fs_ghost_kernel(typename ST::T* __restrict__ u1,
+                                                       typename ST::T* __restrict__ v1,
+                                                       typename ST::T* __restrict__ w1,
                                                        const float* __restrict__ lam_mu,
-                                                       const FsLaunch F) {
+                                                       const FsLaunch F, const WaveScales SC) {
   const int j = F.j0 + (int)(blockIdx.x * 64 + threadIdx.x);
   const int i = F.i0 + (int)(blockIdx.y * 4 + threadIdx.y);
   if (j > F.j1 || i > F.i1) return;
   const long long S = F.slice;
   const int Y = F.yline;
   const long long p = (long long)i * S + (long long)j * Y + F.ktop;
-  const float uc = u1[p], vc = v1[p], wc = w1[p];
-  const float ug = uc - (wc - w1[p - S]);
-  const float vg = vc - (w1[p + Y] - wc);
-  u1[p + 1] = ug;
-  v1[p + 1] = vg;
+  const float su = SC.s[AWP_SU], sv = SC.s[AWP_SV], sw = SC.s[AWP_SW];
+  const float uc = ST::ld(u1, p, su), vc = ST::ld(v1, p, sv), wc = ST::ld(w1, p, sw);
+  const float ug = uc - (wc - ST::ld(w1, p - S, sw));
+  const float vg = vc - (ST::ld(w1, p + Y, sw) - wc);
+  ST::st(u1, p + 1, ug, SC.inv[AWP_SU]);
+  ST::st(v1, p + 1, vg, SC.inv[AWP_SV]);
   float vs1, vs2;
   int gi = F.nxt * F.rankx + i - AWP_NGSL - 1;
   if (gi < F.NX || (NEWLIT && F.NPC == 2))
-    vs1 = u1[p + S] - (w1[p + S] - wc);
+    vs1 = ST::ld(u1, p + S, su) - (ST::ld(w1, p + S, sw) - wc);
   else
     vs1 = 0.0f;
   gi = F.nyt * F.ranky + j - AWP_NGSL - 1;
   if (gi > 1 || F.NPC == 2)
-    vs2 = v1[p - Y] - (wc - w1[p - Y]);
+    vs2 = ST::ld(v1, p - Y, sv) - (wc - ST::ld(w1, p - Y, sw));
   else
     vs2 = 0.0f;
-  w1[p + 1] = w1[p - 1] - lam_mu[(long long)i * F.lam_mu_ld + j] *
-                              ((vs1 - ug) + (u1[p + S] - uc) + (vg - vs2) + (vc - v1[p - Y]));
+  ST::st(w1, p + 1,
+         ST::ld(w1, p - 1, sw) - lam_mu[(long long)i * F.lam_mu_ld + j] *
+                                     ((vs1 - ug) + (ST::ld(u1, p + S, su) - uc) + (vg - vs2) +
+                                      (vc - ST::ld(v1, p - Y, sv))),
+         SC.inv[AWP_SW]);
 }
 
-template <bool NEWLIT>
-void dstrqc_common(const char* W, float* xx, float* yy, float* zz, float* xy, float* xz,
-                   float* yz, float* r1, float* r2, float* r3, float* r4, float* r5, float* r6,
-                   float* u1, float* v1, float* w1, float* lam, float* mu, float* qp,
+// f15: xx yy zz xy xz yz r1..r6 u1 v1 w1 in the storage type of ST.
+template <bool NEWLIT, class ST>
+void dstrqc_common(const char* W, void* const* f15, float* lam, float* mu, float* qp,
                    float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
                    int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
                    float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i, int s_j,
-                   int e_j, int d_i) {
+                   int e_j, int d_i, const WaveScales& SC) {
+  typedef typename ST::T T;
+  T* const* A = reinterpret_cast<T* const*>(f15);
   if (!awp_grid_ok(d_i, W)) return;
   const AwpGrid& g = awp_state().grid[d_i];
   if (nyt != g.nyt || nzt != g.nzt) {
@@ -332,7 +342,8 @@ void dstrqc_common(const char* W, float* xx, float* yy, float* zz, float* xy, fl
     F.lam_mu_ld = nyt + 4 + AWP_NGSL2;
     F.nxt = g.nxt; F.nyt = g.nyt; F.NX = NX; F.NPC = NPC; F.rankx = rankx; F.ranky = ranky;
     dim3 grid(awp_cdiv(e_j - s_j + 1, 64), awp_cdiv(e_i - s_i + 1, 4));
-    hipLaunchKernelGGL(fs_ghost_kernel<NEWLIT>, grid, dim3(64, 4), 0, St, u1, v1, w1, lam_mu, F);
+    hipLaunchKernelGGL((fs_ghost_kernel<NEWLIT, ST>), grid, dim3(64, 4), 0, St, A[12], A[13],
+                       A[14], lam_mu, F, SC);
     if (!awp_check_launch(W)) return;
   }
   StrLaunch L;
@@ -358,14 +369,14 @@ void dstrqc_common(const char* W, float* xx, float* yy, float* zz, float* xy, fl
   L.ichunk = ichunk;
   L.nblocks = L.nkt * L.njt * nic;
 #define AWP_STR_ARGS                                                                          \
-  xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, coeff, qs, dcrjx, \
-      dcrjy, dcrjz, vx1, vx2, ww, wwo, L
+  A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], A[8], A[9], A[10], A[11], A[12], A[13],    \
+      A[14], lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, vx1, vx2, ww, wwo, L, SC
   switch (bj) {
-    case 1: hipLaunchKernelGGL((dstrqc_kernel<1, NEWLIT>), dim3(L.nblocks), dim3(64, 1), 0, St, AWP_STR_ARGS); break;
-    case 2: hipLaunchKernelGGL((dstrqc_kernel<2, NEWLIT>), dim3(L.nblocks), dim3(64, 2), 0, St, AWP_STR_ARGS); break;
-    case 4: hipLaunchKernelGGL((dstrqc_kernel<4, NEWLIT>), dim3(L.nblocks), dim3(64, 4), 0, St, AWP_STR_ARGS); break;
-    case 8: hipLaunchKernelGGL((dstrqc_kernel<8, NEWLIT>), dim3(L.nblocks), dim3(64, 8), 0, St, AWP_STR_ARGS); break;
-    default: hipLaunchKernelGGL((dstrqc_kernel<16, NEWLIT>), dim3(L.nblocks), dim3(64, 16), 0, St, AWP_STR_ARGS); break;
+    case 1: hipLaunchKernelGGL((dstrqc_kernel<1, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 1), 0, St, AWP_STR_ARGS); break;
+    case 2: hipLaunchKernelGGL((dstrqc_kernel<2, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 2), 0, St, AWP_STR_ARGS); break;
+    case 4: hipLaunchKernelGGL((dstrqc_kernel<4, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 4), 0, St, AWP_STR_ARGS); break;
+    case 8: hipLaunchKernelGGL((dstrqc_kernel<8, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 8), 0, St, AWP_STR_ARGS); break;
+    default: hipLaunchKernelGGL((dstrqc_kernel<16, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 16), 0, St, AWP_STR_ARGS); break;
   }
 #undef AWP_STR_ARGS
   awp_check_launch(W);
@@ -377,55 +388,36 @@ struct FstrLaunch {
   long long slice;
 };
 
-__global__ __launch_bounds__(256) void fstr_kernel(float* __restrict__ zz, float* __restrict__ xz,
-                                                   float* __restrict__ yz, const FstrLaunch F) {
+// Every store is a negated copy or a zero, so in 16-bit storage it acts on
+// the stored bits exactly as on the decoded values.
+template <class ST>
+__global__ __launch_bounds__(256) void fstr_kernel(typename ST::T* __restrict__ zz,
+                                                   typename ST::T* __restrict__ xz,
+                                                   typename ST::T* __restrict__ yz,
+                                                   const FstrLaunch F, const WaveScales SC) {
   const int j = F.j0 + (int)(blockIdx.x * 64 + threadIdx.x);
   const int i = F.i0 + (int)(blockIdx.y * 4 + threadIdx.y);
   if (j > F.j1 || i > F.i1) return;
   const long long p = (long long)i * F.slice + (long long)j * F.yline + F.ktop;
-  const float zc = zz[p], zm1 = zz[p - 1];
-  const float xm1 = xz[p - 1], xm2 = xz[p - 2];
-  const float ym1 = yz[p - 1], ym2 = yz[p - 2];
-  zz[p + 1] = -zc;
-  zz[p + 2] = -zm1;
-  xz[p + 1] = -xm1;
-  xz[p + 2] = -xm2;
-  yz[p + 1] = -ym1;
-  yz[p + 2] = -ym2;
-  xz[p] = 0.0f;
-  yz[p] = 0.0f;
+  const float sz = SC.s[AWP_SZZ], sx = SC.s[AWP_SXZ], sy = SC.s[AWP_SYZ];
+  const float iz = SC.inv[AWP_SZZ], ix = SC.inv[AWP_SXZ], iy = SC.inv[AWP_SYZ];
+  const float zc = ST::ld(zz, p, sz), zm1 = ST::ld(zz, p - 1, sz);
+  const float xm1 = ST::ld(xz, p - 1, sx), xm2 = ST::ld(xz, p - 2, sx);
+  const float ym1 = ST::ld(yz, p - 1, sy), ym2 = ST::ld(yz, p - 2, sy);
+  ST::st(zz, p + 1, -zc, iz);
+  ST::st(zz, p + 2, -zm1, iz);
+  ST::st(xz, p + 1, -xm1, ix);
+  ST::st(xz, p + 2, -xm2, ix);
+  ST::st(yz, p + 1, -ym1, iy);
+  ST::st(yz, p + 2, -ym2, iy);
+  ST::st(xz, p, 0.0f, ix);
+  ST::st(yz, p, 0.0f, iy);
 }
 
-}  // namespace
-
-extern "C" {
-
-void dstrqc_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
-              float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
-              float* w1, float* lam, float* mu, float* qp, float* coeff, float* qs,
-              float* dcrjx, float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St,
-              float* lam_mu, float* vx1, float* vx2, int* ww, float* wwo, int NX, int NPC,
-              int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i) {
-  dstrqc_common<false>("dstrqc_H", xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1,
-                       lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt, St, lam_mu, vx1,
-                       vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j, d_i);
-}
-
-void dstrqc_H_new(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
-                  float* r1, float* r2, float* r3, float* r4, float* r5, float* r6,
-                  float* u1, float* v1, float* w1, float* lam, float* mu, float* qp,
-                  float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
-                  int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
-                  float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i,
-                  int s_j, int e_j, int d_i) {
-  dstrqc_common<true>("dstrqc_H_new", xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1,
-                      w1, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt, St, lam_mu,
-                      vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j, d_i);
-}
-
-void fstr_H(float* zz, float* xz, float* yz, hipStream_t St, int s_i, int e_i, int s_j,
-            int e_j) {
-  const char* W = "fstr_H";
+template <class ST>
+void fstr_common(const char* W, void* zz, void* xz, void* yz, hipStream_t St, int s_i, int e_i,
+                 int s_j, int e_j, const WaveScales& SC) {
+  typedef typename ST::T T;
   if (!awp_grid_ok(0, W)) return;
   const AwpGrid& g = awp_state().grid[0];
   if (s_i > e_i || s_j > e_j) return;
@@ -440,8 +432,78 @@ void fstr_H(float* zz, float* xz, float* yz, hipStream_t St, int s_i, int e_i, i
   F.yline = g.yline;
   F.slice = g.slice;
   dim3 grid(awp_cdiv(e_j - s_j + 1, 64), awp_cdiv(e_i - s_i + 1, 4));
-  hipLaunchKernelGGL(fstr_kernel, grid, dim3(64, 4), 0, St, zz, xz, yz, F);
+  hipLaunchKernelGGL((fstr_kernel<ST>), grid, dim3(64, 4), 0, St, (T*)zz, (T*)xz, (T*)yz, F, SC);
   awp_check_launch(W);
+}
+
+}  // namespace
+
+extern "C" {
+
+void dstrqc_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
+              float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
+              float* w1, float* lam, float* mu, float* qp, float* coeff, float* qs,
+              float* dcrjx, float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St,
+              float* lam_mu, float* vx1, float* vx2, int* ww, float* wwo, int NX, int NPC,
+              int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i) {
+  void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
+  WaveScales SC = {};
+  dstrqc_common<false, StF32>("dstrqc_H", f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt,
+                              nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i,
+                              e_i, s_j, e_j, d_i, SC);
+}
+
+void dstrqc_H_new(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                  float* r1, float* r2, float* r3, float* r4, float* r5, float* r6,
+                  float* u1, float* v1, float* w1, float* lam, float* mu, float* qp,
+                  float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
+                  int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
+                  float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i,
+                  int s_j, int e_j, int d_i) {
+  void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
+  WaveScales SC = {};
+  dstrqc_common<true, StF32>("dstrqc_H_new", f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz,
+                             nyt, nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i,
+                             e_i, s_j, e_j, d_i, SC);
+}
+
+void awp_dstrqc_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* xz,
+                    uint16_t* yz, uint16_t* r1, uint16_t* r2, uint16_t* r3, uint16_t* r4,
+                    uint16_t* r5, uint16_t* r6, uint16_t* u1, uint16_t* v1, uint16_t* w1,
+                    float* lam, float* mu, float* qp, float* coeff, float* qs, float* dcrjx,
+                    float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St, float* lam_mu,
+                    float* vx1, float* vx2, int* ww, float* wwo, int NX, int NPC, int rankx,
+                    int ranky, int s_i, int e_i, int s_j, int e_j, int d_i, int fmt,
+                    const float* scale) {
+  const char* W = "awp_dstrqc_s16";
+  WaveScales SC;
+  if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
+  void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
+  if (fmt == AWP_FMT_BF16)
+    dstrqc_common<false, StBF16>(W, f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt,
+                                 St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i,
+                                 s_j, e_j, d_i, SC);
+  else
+    dstrqc_common<false, StF16>(W, f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt,
+                                St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i,
+                                s_j, e_j, d_i, SC);
+}
+
+void fstr_H(float* zz, float* xz, float* yz, hipStream_t St, int s_i, int e_i, int s_j,
+            int e_j) {
+  WaveScales SC = {};
+  fstr_common<StF32>("fstr_H", zz, xz, yz, St, s_i, e_i, s_j, e_j, SC);
+}
+
+void awp_fstr_s16(uint16_t* zz, uint16_t* xz, uint16_t* yz, hipStream_t St, int s_i, int e_i,
+                  int s_j, int e_j, int fmt, const float* scale) {
+  const char* W = "awp_fstr_s16";
+  WaveScales SC;
+  if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
+  if (fmt == AWP_FMT_BF16)
+    fstr_common<StBF16>(W, zz, xz, yz, St, s_i, e_i, s_j, e_j, SC);
+  else
+    fstr_common<StF16>(W, zz, xz, yz, St, s_i, e_i, s_j, e_j, SC);
 }
 
 }  // extern "C"

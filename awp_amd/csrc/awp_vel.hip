@@ -11,6 +11,7 @@
 // -ffp-contract=off), so results are reproducible bit for bit on any
 // IEEE-754 binary32 implementation of the same operations.
 #include "awp_common.h"
+#include "awp_store.h"
 #include "awp_tuning.h"
 #include "../../include/awp_hip.h"
 
@@ -24,13 +25,15 @@ struct VelLaunch {
   float c1, c2, dth;
 };
 
-template <int BJ>
+template <int BJ, class ST>
 __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
-    float* __restrict__ u1, float* __restrict__ v1, float* __restrict__ w1,
-    const float* __restrict__ xx, const float* __restrict__ yy, const float* __restrict__ zz,
-    const float* __restrict__ xy, const float* __restrict__ xz, const float* __restrict__ yz,
-    const float* __restrict__ dcrjx, const float* __restrict__ dcrjy,
-    const float* __restrict__ dcrjz, const float* __restrict__ d1, const VelLaunch L) {
+    typename ST::T* __restrict__ u1, typename ST::T* __restrict__ v1,
+    typename ST::T* __restrict__ w1, const typename ST::T* __restrict__ xx,
+    const typename ST::T* __restrict__ yy, const typename ST::T* __restrict__ zz,
+    const typename ST::T* __restrict__ xy, const typename ST::T* __restrict__ xz,
+    const typename ST::T* __restrict__ yz, const float* __restrict__ dcrjx,
+    const float* __restrict__ dcrjy, const float* __restrict__ dcrjz,
+    const float* __restrict__ d1, const VelLaunch L, const WaveScales SC) {
   const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
   const int kt = b % L.nkt;
   const int jt = (b / L.nkt) % L.njt;
@@ -47,7 +50,10 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
   const long long Y = L.yline;
   const long long S = L.slice;
   const long long J = (long long)j * Y;
-  const unsigned kb = (unsigned)k * 4u;
+  const unsigned kb = (unsigned)k * 4u;                      // medium (binary32)
+  const unsigned kw = (unsigned)k * (unsigned)sizeof(typename ST::T);  // wavefield
+  const float sxx = SC.s[AWP_SXX], syy = SC.s[AWP_SYY], szz = SC.s[AWP_SZZ];
+  const float sxy = SC.s[AWP_SXY], sxz = SC.s[AWP_SXZ], syz = SC.s[AWP_SYZ];
   const float c1 = L.c1, c2 = L.c2, dth = L.dth;
   const float dy = dcrjy[j];
   const float dz = dcrjz[k];
@@ -55,12 +61,12 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
 #define RP(A, ii, dj, dk) ((A) + (long long)(ii) * S + J + (long long)(dj) * Y + (dk))
 
   // Prime the x pipelines at i = ia.
-  float xx_m2 = awp_ld(RP(xx, ia - 2, 0, 0), kb), xx_m1 = awp_ld(RP(xx, ia - 1, 0, 0), kb),
-        xx_0 = awp_ld(RP(xx, ia, 0, 0), kb);
-  float xy_m1 = awp_ld(RP(xy, ia - 1, 0, 0), kb), xy_0 = awp_ld(RP(xy, ia, 0, 0), kb),
-        xy_p1 = awp_ld(RP(xy, ia + 1, 0, 0), kb);
-  float xz_m1 = awp_ld(RP(xz, ia - 1, 0, 0), kb), xz_0 = awp_ld(RP(xz, ia, 0, 0), kb),
-        xz_p1 = awp_ld(RP(xz, ia + 1, 0, 0), kb);
+  float xx_m2 = ST::ldb(RP(xx, ia - 2, 0, 0), kw, sxx), xx_m1 = ST::ldb(RP(xx, ia - 1, 0, 0), kw, sxx),
+        xx_0 = ST::ldb(RP(xx, ia, 0, 0), kw, sxx);
+  float xy_m1 = ST::ldb(RP(xy, ia - 1, 0, 0), kw, sxy), xy_0 = ST::ldb(RP(xy, ia, 0, 0), kw, sxy),
+        xy_p1 = ST::ldb(RP(xy, ia + 1, 0, 0), kw, sxy);
+  float xz_m1 = ST::ldb(RP(xz, ia - 1, 0, 0), kw, sxz), xz_0 = ST::ldb(RP(xz, ia, 0, 0), kw, sxz),
+        xz_p1 = ST::ldb(RP(xz, ia + 1, 0, 0), kw, sxz);
   // density at plane i: (j,k), (j-1,k), (j,k-1), (j-1,k-1)
   float dc = awp_ld(RP(d1, ia, 0, 0), kb), djm = awp_ld(RP(d1, ia, -1, 0), kb),
         dkm = awp_ld(RP(d1, ia, 0, -1), kb), djk = awp_ld(RP(d1, ia, -1, -1), kb);
@@ -71,33 +77,34 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     // induction pointer; the plane bases stay in scalar registers.
     const int i = __builtin_amdgcn_readfirstlane(ia + s);
     // pipeline leads
-    const float xx_p1 = awp_ld(RP(xx, i + 1, 0, 0), kb);
-    const float xy_p2 = awp_ld(RP(xy, i + 2, 0, 0), kb);
-    const float xz_p2 = awp_ld(RP(xz, i + 2, 0, 0), kb);
+    const float xx_p1 = ST::ldb(RP(xx, i + 1, 0, 0), kw, sxx);
+    const float xy_p2 = ST::ldb(RP(xy, i + 2, 0, 0), kw, sxy);
+    const float xz_p2 = ST::ldb(RP(xz, i + 2, 0, 0), kw, sxz);
     const float ec = awp_ld(RP(d1, i + 1, 0, 0), kb), ejm = awp_ld(RP(d1, i + 1, -1, 0), kb),
                 ekm = awp_ld(RP(d1, i + 1, 0, -1), kb), ejk = awp_ld(RP(d1, i + 1, -1, -1), kb);
 
     // in-plane stencils
-    const float* XY = RP(xy, i, 0, 0);
-    const float* XZ = RP(xz, i, 0, 0);
-    const float* YY = RP(yy, i, 0, 0);
-    const float* YZ = RP(yz, i, 0, 0);
-    const float* ZZ = RP(zz, i, 0, 0);
-    const float xy_jm1 = awp_ld(XY - Y, kb), xy_jp1 = awp_ld(XY + Y, kb),
-                xy_jm2 = awp_ld(XY - 2 * Y, kb);
-    const float xz_km1 = awp_ld(XZ - 1, kb), xz_kp1 = awp_ld(XZ + 1, kb), xz_km2 = awp_ld(XZ - 2, kb);
-    const float yy_0 = awp_ld(YY, kb), yy_jp1 = awp_ld(YY + Y, kb), yy_jp2 = awp_ld(YY + 2 * Y, kb),
-                yy_jm1 = awp_ld(YY - Y, kb);
-    const float yz_0 = awp_ld(YZ, kb), yz_km1 = awp_ld(YZ - 1, kb), yz_kp1 = awp_ld(YZ + 1, kb),
-                yz_km2 = awp_ld(YZ - 2, kb);
-    const float yz_jm1 = awp_ld(YZ - Y, kb), yz_jp1 = awp_ld(YZ + Y, kb),
-                yz_jm2 = awp_ld(YZ - 2 * Y, kb);
-    const float zz_0 = awp_ld(ZZ, kb), zz_kp1 = awp_ld(ZZ + 1, kb), zz_kp2 = awp_ld(ZZ + 2, kb),
-                zz_km1 = awp_ld(ZZ - 1, kb);
-    float* U = RP(u1, i, 0, 0);
-    float* V = RP(v1, i, 0, 0);
-    float* W = RP(w1, i, 0, 0);
-    const float u = awp_ld(U, kb), v = awp_ld(V, kb), w = awp_ld(W, kb);
+    const typename ST::T* XY = RP(xy, i, 0, 0);
+    const typename ST::T* XZ = RP(xz, i, 0, 0);
+    const typename ST::T* YY = RP(yy, i, 0, 0);
+    const typename ST::T* YZ = RP(yz, i, 0, 0);
+    const typename ST::T* ZZ = RP(zz, i, 0, 0);
+    const float xy_jm1 = ST::ldb(XY - Y, kw, sxy), xy_jp1 = ST::ldb(XY + Y, kw, sxy),
+                xy_jm2 = ST::ldb(XY - 2 * Y, kw, sxy);
+    const float xz_km1 = ST::ldb(XZ - 1, kw, sxz), xz_kp1 = ST::ldb(XZ + 1, kw, sxz), xz_km2 = ST::ldb(XZ - 2, kw, sxz);
+    const float yy_0 = ST::ldb(YY, kw, syy), yy_jp1 = ST::ldb(YY + Y, kw, syy), yy_jp2 = ST::ldb(YY + 2 * Y, kw, syy),
+                yy_jm1 = ST::ldb(YY - Y, kw, syy);
+    const float yz_0 = ST::ldb(YZ, kw, syz), yz_km1 = ST::ldb(YZ - 1, kw, syz), yz_kp1 = ST::ldb(YZ + 1, kw, syz),
+                yz_km2 = ST::ldb(YZ - 2, kw, syz);
+    const float yz_jm1 = ST::ldb(YZ - Y, kw, syz), yz_jp1 = ST::ldb(YZ + Y, kw, syz),
+                yz_jm2 = ST::ldb(YZ - 2 * Y, kw, syz);
+    const float zz_0 = ST::ldb(ZZ, kw, szz), zz_kp1 = ST::ldb(ZZ + 1, kw, szz), zz_kp2 = ST::ldb(ZZ + 2, kw, szz),
+                zz_km1 = ST::ldb(ZZ - 1, kw, szz);
+    typename ST::T* U = RP(u1, i, 0, 0);
+    typename ST::T* V = RP(v1, i, 0, 0);
+    typename ST::T* W = RP(w1, i, 0, 0);
+    const float u = ST::ldb(U, kw, SC.s[AWP_SU]), v = ST::ldb(V, kw, SC.s[AWP_SV]),
+                w = ST::ldb(W, kw, SC.s[AWP_SW]);
 
     const float dcrj = dcrjx[i] * dy * dz;
     float f_d1 = 0.25f * (dc + djm + dkm + djk);
@@ -107,15 +114,15 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     f_d2 = dth / f_d2;
     f_d3 = dth / f_d3;
 
-    awp_st(U, kb, (u + f_d1 * (c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) + c1 * (xy_0 - xy_jm1) +
+    ST::stb(U, kw, (u + f_d1 * (c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) + c1 * (xy_0 - xy_jm1) +
                                c2 * (xy_jp1 - xy_jm2) + c1 * (xz_0 - xz_km1) +
-                               c2 * (xz_kp1 - xz_km2))) * dcrj);
-    awp_st(V, kb, (v + f_d2 * (c1 * (xy_p1 - xy_0) + c2 * (xy_p2 - xy_m1) + c1 * (yy_jp1 - yy_0) +
+                               c2 * (xz_kp1 - xz_km2))) * dcrj, SC.inv[AWP_SU]);
+    ST::stb(V, kw, (v + f_d2 * (c1 * (xy_p1 - xy_0) + c2 * (xy_p2 - xy_m1) + c1 * (yy_jp1 - yy_0) +
                                c2 * (yy_jp2 - yy_jm1) + c1 * (yz_0 - yz_km1) +
-                               c2 * (yz_kp1 - yz_km2))) * dcrj);
-    awp_st(W, kb, (w + f_d3 * (c1 * (xz_p1 - xz_0) + c2 * (xz_p2 - xz_m1) + c1 * (yz_0 - yz_jm1) +
+                               c2 * (yz_kp1 - yz_km2))) * dcrj, SC.inv[AWP_SV]);
+    ST::stb(W, kw, (w + f_d3 * (c1 * (xz_p1 - xz_0) + c2 * (xz_p2 - xz_m1) + c1 * (yz_0 - yz_jm1) +
                                c2 * (yz_jp1 - yz_jm2) + c1 * (zz_kp1 - zz_0) +
-                               c2 * (zz_kp2 - zz_km1))) * dcrj);
+                               c2 * (zz_kp2 - zz_km1))) * dcrj, SC.inv[AWP_SW]);
 
     // advance pipelines
     xx_m2 = xx_m1; xx_m1 = xx_0; xx_0 = xx_p1;
@@ -126,26 +133,26 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
 #undef RP
 }
 
-template <int BJ>
-void launch_vel(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz, float* xy,
-                float* xz, float* yz, float* dcrjx, float* dcrjy, float* dcrjz, float* d_1,
-                hipStream_t St, VelLaunch L) {
+template <int BJ, class ST>
+void launch_vel(void* const* f9, float* dcrjx, float* dcrjy, float* dcrjz, float* d_1,
+                hipStream_t St, VelLaunch L, const WaveScales& SC) {
+  typedef typename ST::T T;
   const int nkj = L.nkt * L.njt;
   int nic, ichunk;
   awp_split_i(L.ni, nkj, awp_tuning().vel_ichunk, awp_tuning().vel_target_wgs, &nic, &ichunk);
   L.ichunk = ichunk;
   L.nblocks = nkj * nic;
-  hipLaunchKernelGGL((dvelcx_kernel<BJ>), dim3(L.nblocks), dim3(64, BJ), 0, St, u1, v1, w1, xx,
-                     yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, L);
+  hipLaunchKernelGGL((dvelcx_kernel<BJ, ST>), dim3(L.nblocks), dim3(64, BJ), 0, St, (T*)f9[0],
+                     (T*)f9[1], (T*)f9[2], (const T*)f9[3], (const T*)f9[4], (const T*)f9[5],
+                     (const T*)f9[6], (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz, d_1,
+                     L, SC);
 }
 
-}  // namespace
-
-extern "C" void dvelcx_H_opt(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz,
-                             float* xy, float* xz, float* yz, float* dcrjx, float* dcrjy,
-                             float* dcrjz, float* d_1, int nyt, int nzt, hipStream_t St,
-                             int s_i, int e_i, int d_i, int ngrids) {
-  const char* W = "dvelcx_H_opt";
+// f9: u1 v1 w1 xx yy zz xy xz yz in the storage type of ST.
+template <class ST>
+void dvelcx_common(const char* W, void* const* f9, float* dcrjx, float* dcrjy, float* dcrjz,
+                   float* d_1, int nyt, int nzt, hipStream_t St, int s_i, int e_i, int d_i,
+                   int ngrids, const WaveScales& SC) {
   if (!awp_grid_ok(d_i, W)) return;
   const AwpGrid& g = awp_state().grid[d_i];
   if (nyt != g.nyt || nzt != g.nzt) {
@@ -180,15 +187,41 @@ extern "C" void dvelcx_H_opt(float* u1, float* v1, float* w1, float* xx, float* 
   L.nkt = awp_cdiv(L.k1 - L.kb + 1, 64);
   L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
   if (L.k1 < L.k0) return;
-#define AWP_VEL_LAUNCH(B) \
-  launch_vel<B>(u1, v1, w1, xx, yy, zz, xy, xz, yz, dcrjx, dcrjy, dcrjz, d_1, St, L);
   switch (bj) {
-    case 1: AWP_VEL_LAUNCH(1) break;
-    case 2: AWP_VEL_LAUNCH(2) break;
-    case 8: AWP_VEL_LAUNCH(8) break;
-    case 16: AWP_VEL_LAUNCH(16) break;
-    default: AWP_VEL_LAUNCH(4) break;
+    case 1: launch_vel<1, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 2: launch_vel<2, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 8: launch_vel<8, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 16: launch_vel<16, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    default: launch_vel<4, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
   }
-#undef AWP_VEL_LAUNCH
   awp_check_launch(W);
+}
+
+}  // namespace
+
+extern "C" void dvelcx_H_opt(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz,
+                             float* xy, float* xz, float* yz, float* dcrjx, float* dcrjy,
+                             float* dcrjz, float* d_1, int nyt, int nzt, hipStream_t St,
+                             int s_i, int e_i, int d_i, int ngrids) {
+  void* f9[9] = {u1, v1, w1, xx, yy, zz, xy, xz, yz};
+  WaveScales SC = {};
+  dvelcx_common<StF32>("dvelcx_H_opt", f9, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i,
+                       d_i, ngrids, SC);
+}
+
+extern "C" void awp_dvelcx_s16(uint16_t* u1, uint16_t* v1, uint16_t* w1, uint16_t* xx,
+                               uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* xz,
+                               uint16_t* yz, float* dcrjx, float* dcrjy, float* dcrjz,
+                               float* d_1, int nyt, int nzt, hipStream_t St, int s_i, int e_i,
+                               int d_i, int ngrids, int fmt, const float* scale) {
+  const char* W = "awp_dvelcx_s16";
+  WaveScales SC;
+  if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
+  void* f9[9] = {u1, v1, w1, xx, yy, zz, xy, xz, yz};
+  if (fmt == AWP_FMT_BF16)
+    dvelcx_common<StBF16>(W, f9, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i, ngrids,
+                          SC);
+  else
+    dvelcx_common<StF16>(W, f9, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i, ngrids,
+                         SC);
 }

@@ -19,6 +19,7 @@ from .grid import ALIGN, NGSL, Dims
 from .media import Medium
 from .plasticity import PlasticMedium
 from .stepping import PmclStepOrder
+from .storage import FIELDS16, FORMATS, ScaledStorage, initial_scales
 
 FIELDS = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
           "r1", "r2", "r3", "r4", "r5", "r6")
@@ -312,3 +313,127 @@ class Solver(PmclStepOrder):
     def field(self, name: str, p: int = 0) -> np.ndarray:
         torch.cuda.synchronize(self.device)
         return self.levels[p].f[name].cpu().numpy()
+
+
+class Solver16(ScaledStorage, Solver):
+    """Single-grid NVE = 1 run with the wavefield and memory variables stored
+    in 16 bits (storage "bf16" or "fp16") with per-field power-of-two scales
+    (awp_amd.storage); arithmetic in binary32, medium in binary32."""
+
+    def __init__(self, medium: Medium, sources: PointSources | None = None,
+                 receivers: Receivers | None = None, nt_record: int = 0, device: str = "cuda",
+                 storage: str = "bf16", check_every: int = 8):
+        if isinstance(medium, (list, tuple)):
+            raise ValueError("16-bit storage runs one grid")
+        if storage not in FORMATS:
+            raise ValueError(f"storage must be one of {sorted(FORMATS)}")
+        super().__init__(medium, sources, receivers, nt_record, device)
+        self.storage = storage
+        self.fmt = FORMATS[storage]
+        self.check_every = int(check_every)
+        L = self.levels[0]
+        for name in FIELDS16:
+            L.f[name] = torch.zeros(L.dims.shape, dtype=torch.int16, device=self.device)
+        self.scales = initial_scales(storage, medium, sources)
+        self._absbuf = torch.zeros(len(FIELDS16), dtype=torch.float32, device=self.device)
+        self.rescales = 0
+        self._sync_scales()
+
+    def _sync_scales(self) -> None:
+        self._sc = (C.c_float * len(FIELDS16))(*[self.scales[f] for f in FIELDS16])
+
+    def _absmax_all(self) -> dict:
+        L = self.levels[0]
+        n = L.dims.size
+        base = self._absbuf.data_ptr()
+        for idx, name in enumerate(FIELDS16):
+            self.lib.awp_s16_absmax(_ptr(L.f[name]), n, self.fmt, base + 4 * idx, self.stream)
+        _lib.check()
+        vals = self._absbuf.cpu().numpy()
+        return {name: float(vals[idx]) for idx, name in enumerate(FIELDS16)}
+
+    def _rescale(self, name: str, shift: int) -> None:
+        L = self.levels[0]
+        self.lib.awp_s16_rescale(_ptr(L.f[name]), L.dims.size, self.fmt, int(shift), self.stream)
+
+    def check_scales(self) -> list:
+        changed = super().check_scales()
+        if changed:
+            self._sync_scales()
+        return changed
+
+    def step(self) -> None:
+        super().step()
+        if self.check_every and self.cur_step % self.check_every == 0:
+            self.check_scales()
+
+    def velocity(self, p: int = 0) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
+        s_i, e_i = d.i_range
+        self.lib.awp_dvelcx_s16(*(_ptr(f[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy",
+                                                       "xz", "yz")),
+                                _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]),
+                                _ptr(m["d1"]), d.nyt, d.nzt, self.stream, s_i, e_i, p,
+                                self.ngrids, self.fmt, self._sc)
+
+    def stress_region(self, p: int, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
+        if s_i > e_i or s_j > e_j:
+            return
+        self.lib.awp_dstrqc_s16(
+            *(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                   "r5", "r6", "u1", "v1", "w1")),
+            _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["coeff"]), _ptr(m["qs"]),
+            _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nyt, d.nzt, self.stream,
+            _ptr(m["lam_mu"]), _ptr(m["vx1"]), _ptr(m["vx2"]), _ptr(m["ww"]), _ptr(m["wwo"]),
+            d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, p, self.fmt, self._sc)
+
+    def add_source(self, p: int, sample_step: int) -> None:
+        L = self.levels[p]
+        if L.src is None:
+            return
+        f, s = L.f, L.src
+        axx, ayy, azz, axz, ayz, axy = s["cols"]
+        self.lib.awp_addsrc_s16(sample_step, s["nst"], 3, _ptr(s["psrc"]), s["npsrc"],
+                                self.stream, _ptr(axx), _ptr(ayy), _ptr(azz), _ptr(axz),
+                                _ptr(ayz), _ptr(axy), _ptr(f["xx"]), _ptr(f["yy"]),
+                                _ptr(f["zz"]), _ptr(f["xy"]), _ptr(f["yz"]), _ptr(f["xz"]), p,
+                                self.fmt, self._sc)
+
+    def free_surface(self) -> None:
+        d, f = self.dims, self.f
+        s_i, e_i = d.i_range
+        s_j, e_j = d.j_range
+        self.lib.awp_fstr_s16(_ptr(f["zz"]), _ptr(f["xz"]), _ptr(f["yz"]), self.stream, s_i, e_i,
+                              s_j, e_j, self.fmt, self._sc)
+
+    def record(self, slot: int) -> None:
+        if self.rec is None or slot >= self.rec["out"].shape[0]:
+            return
+        f, r = self.f, self.rec
+        self.lib.awp_receivers_gather_s16(_ptr(f["u1"]), _ptr(f["v1"]), _ptr(f["w1"]),
+                                          _ptr(r["idx"]), r["n"], _ptr(r["out"]), slot,
+                                          self.stream, 0, self.fmt, self._sc)
+
+    def velocity_dm(self, p):
+        raise NotImplementedError("16-bit storage runs one grid")
+
+    stress_dm = interpolate = swap = plastic_calc = plastic_app = velocity_dm
+
+    def raw(self, name: str) -> np.ndarray:
+        """Stored 16-bit values of a field (uint16)."""
+        torch.cuda.synchronize(self.device)
+        return self.levels[0].f[name].cpu().numpy().view(np.uint16)
+
+    def field(self, name: str, p: int = 0) -> np.ndarray:
+        L = self.levels[p]
+        if name not in FIELDS16:
+            return super().field(name, p)
+        out = torch.empty(L.dims.shape, dtype=torch.float32, device=self.device)
+        self.lib.awp_s16_decode(_ptr(L.f[name]), _ptr(out), L.dims.size, self.fmt,
+                                float(self.scales[name]), self.stream)
+        _lib.check()
+        torch.cuda.synchronize(self.device)
+        return out.cpu().numpy()

@@ -34,6 +34,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_VEL = 52.0      # 10 reads + 3 writes of 4 B per cell
 BYTES_STR = 140.0     # 23 reads + 12 writes of 4 B per cell
 BYTES_STEP = BYTES_VEL + BYTES_STR
+# 16-bit wavefield storage: 9 x 2 B + d1 4 B read, 3 x 2 B written (velocity);
+# 15 x 2 B + 8 medium x 4 B read, 12 x 2 B written (stress)
+BYTES16_VEL = 28.0
+BYTES16_STR = 86.0
 
 
 def parse():
@@ -48,6 +52,16 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps with no event timing (for rocprofv3)")
+    ap.add_argument("--workload", choices=("anelastic", "dm"), default="anelastic",
+                    help="anelastic: the headline single-grid workload; dm: discontinuous mesh "
+                         "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz)")
+    ap.add_argument("--nz-fine", type=int, default=96)
+    ap.add_argument("--storage", choices=("f32", "bf16", "fp16"), default="f32",
+                    help="wavefield storage of the anelastic workload (16-bit: per-field "
+                         "power-of-two scales, binary32 arithmetic)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy a BYTES device buffer twice (device-to-device): the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
     return ap.parse_args()
 
 
@@ -105,6 +119,12 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    if args.workload == "dm":
+        bench_dm(args)
+        return
 
     def workload(nx, ny):
         return configs.anelastic(nx=nx, ny=ny, nz=args.nz, nt=max(args.steps + args.warmup, 1))
@@ -113,7 +133,15 @@ def main():
     t0 = time.perf_counter()
     med = w.medium()
     setup_s = time.perf_counter() - t0
-    sol = Solver(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1)
+    if args.storage == "f32":
+        sol = Solver(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1)
+        b_vel, b_str = BYTES_VEL, BYTES_STR
+    else:
+        from awp_amd.solver import Solver16
+        sol = Solver16(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1,
+                       storage=args.storage)
+        b_vel, b_str = BYTES16_VEL, BYTES16_STR
+    check_every = getattr(sol, "check_every", 0)
     del med
     cells = w.dims.ncells
     sol.start()
@@ -149,6 +177,8 @@ def main():
             sol.add_source(0, sol.cur_step + 1)
         sol.free_surface()
         sol.record(sol.cur_step - 1)
+        if check_every and sol.cur_step % check_every == 0:
+            sol.check_scales()
         e[3].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -160,15 +190,15 @@ def main():
     str_avg = float(np.mean(str_ms))
     ms_per_step = wall * 1e3 / args.steps
     value = cells * args.steps / wall
-    str_gbs = BYTES_STR * cells / (str_avg * 1e-3) / 1e9
-    vel_gbs = BYTES_VEL * cells / (vel_avg * 1e-3) / 1e9
-    step_gbs = BYTES_STEP * cells / (ms_per_step * 1e-3) / 1e9
+    str_gbs = b_str * cells / (str_avg * 1e-3) / 1e9
+    vel_gbs = b_vel * cells / (vel_avg * 1e-3) / 1e9
+    step_gbs = (b_vel + b_str) * cells / (ms_per_step * 1e-3) / 1e9
     tr = sol.traces()
     finite = bool(np.isfinite(tr).all())
 
     traffic = None
     tfile = ROOT / "profiles" / "dstrqc_anelastic_512x512x256_traffic.json"
-    if tfile.exists() and (args.nx, args.ny, args.nz) == (512, 512, 256):
+    if tfile.exists() and (args.nx, args.ny, args.nz) == (512, 512, 256) and args.storage == "f32":
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
@@ -185,22 +215,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.storage == "f32" else f"f32 arithmetic, {args.storage} storage",
         "data": "synthetic",
-        "config": {"workload": w.name, "nx": w.dims.nxt, "ny": w.dims.nyt, "nz": w.dims.nzt,
+        "config": {"workload": w.name, "storage": args.storage, "nx": w.dims.nxt, "ny": w.dims.nyt, "nz": w.dims.nzt,
                    "h_m": w.DH, "dt_s": w.DT, "attenuation": "Qs=50vs[km/s], Qp=2Qs, 8 mechanisms",
                    "medium": "layered vs 1500-4500 m/s + 5% von Karman (a=500 m, H=0.5), seed 7",
                    "step": "dvelcx_H_opt, dstrqc_H (+FS ghosts), addsrc_H, fstr_H, receivers"},
         "roofline": {"bound": "hbm", "kernel": "dstrqc_H", "achieved": str_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes_per_cell": BYTES_STR,
+                     "traffic": traffic, "algorithmic_bytes_per_cell": b_str,
                      "cells_per_launch": cells, "avg_launch_ms": str_avg},
         "kernels": {"dvelcx_H_opt": {"avg_ms": vel_avg, "GBps": vel_gbs,
                                      "frac": vel_gbs / HBM_PEAK_GBS},
                     "dstrqc_H": {"avg_ms": str_avg, "GBps": str_gbs,
                                  "frac": str_gbs / HBM_PEAK_GBS},
                     "step_events_avg_ms": float(np.mean(step_ms))},
-        "step_roofline": {"bytes_per_cell": BYTES_STEP, "achieved_GBps": step_gbs,
+        "step_roofline": {"bytes_per_cell": b_vel + b_str, "achieved_GBps": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "traces_finite": finite,
         "setup_s": setup_s,
@@ -211,6 +241,91 @@ def main():
         out["cpu_baseline"] = cpu_baseline(workload, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+def copy_probe(nbytes: int) -> None:
+    """Two device-to-device copies of nbytes: under rocprofv3 --pmc their
+    FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters (nbytes read
+    and nbytes written per copy)."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 2
+    print(json.dumps({"copy_probe_bytes": n * 4, "GBps_read_plus_write": 2 * n * 4 / dt / 1e9}))
+
+
+def bench_dm(args) -> None:
+    """Discontinuous mesh + plasticity: whole pmcl3d steps (both grids, the
+    coupling kernels, NVE = 3 region split, yield factors, free surface,
+    receivers), each bracketed by HIP events on the launch stream.  Sizes
+    default to the 256 x 256 x 192 coarse grid (h = 75 m) under a
+    768 x 768 x 96 fine grid (h = 25 m)."""
+    import numpy as np
+    import torch
+
+    from awp_amd import _lib, configs
+    from awp_amd.solver import Solver
+
+    nx = args.nx if args.nx != 512 else 256
+    ny = args.ny if args.ny != 512 else 256
+    nz = args.nz if args.nz != 256 else 192
+    nt = max(args.steps + args.warmup, 1)
+    w = configs.dm_plastic(nx=nx, ny=ny, nz=nz, nz_fine=args.nz_fine, nt=nt)
+    t0 = time.perf_counter()
+    media = w.medium()
+    pm = w.params["plastic_fn"](media)
+    setup_s = time.perf_counter() - t0
+    print(f"dm setup {setup_s:.1f} s", file=sys.stderr, flush=True)
+    sol = Solver(media, w.sources, w.receivers, nt_record=nt + 1, plastic=pm)
+    cells = sum(m.dims.ncells for m in media)
+    del media, pm
+    sol.start()
+    if args.profile_steps:
+        for _ in range(args.profile_steps):
+            sol.step()
+        torch.cuda.synchronize()
+        _lib.check()
+        print(json.dumps({"profile_steps": args.profile_steps, "workload": w.name}))
+        return
+    for _ in range(args.warmup):
+        sol.step()
+    torch.cuda.synchronize()
+    _lib.check()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in range(args.steps):
+        ev[n][0].record(stream)
+        sol.step()
+        ev[n][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _lib.check()
+    step_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    tr = sol.traces()
+    d0, d1 = sol.grid_dims(0), sol.grid_dims(1)
+    out = {
+        "metric": METRIC, "value": cells * args.steps / wall, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "fine": [d0.nxt, d0.nyt, d0.nzt], "coarse": [d1.nxt, d1.nyt, d1.nzt],
+                   "h_m": [w.params["DH"][0], w.params["DH"][1]], "dt_s": w.DT,
+                   "plasticity": "Drucker-Prager c=5 MPa, mu=0.75, Tv=0.01 s",
+                   "source": "200 double-couple subfaults in the coarse grid"},
+        "step_events_avg_ms": step_ms, "cells_per_step": cells,
+        "traces_finite": bool(np.isfinite(tr).all()), "setup_s": setup_s,
+        "roofline": None, "cpu_baseline": None,
+    }
     print(json.dumps(out))
 
 

@@ -22,6 +22,7 @@
 #define AWP_HIP_H
 
 #include <hip/hip_runtime_api.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -170,6 +171,47 @@ void swap_H(float* xxl, float* yyl, float* zzl, float* xyl, float* xzl, float* y
             float* v1l, float* w1l, float* xxh, float* yyh, float* zzh, float* xyh, float* xzh,
             float* yzh, float* u1h, float* v1h, float* w1h, int nxtl, int nytl, float* buf_L,
             float* buf_R, float* buf_F, float* buf_B, int rank, hipStream_t St, int d_i);
+
+/* ---- 16-bit wavefield storage (no reference counterpart) ----------------
+ * For runs whose nine wavefield components and six memory variables are kept
+ * in HBM as bfloat16 (fmt 1) or IEEE binary16 (fmt 2) with one power-of-two
+ * scale per field: value = stored * scale.  Every load decodes to binary32,
+ * all arithmetic is the binary32 arithmetic of the launcher named, and every
+ * store rounds x / scale to nearest even.  scale is a host array of 15
+ * scales in the order u1 v1 w1 xx yy zz xy xz yz r1 r2 r3 r4 r5 r6.  The
+ * medium stays binary32.  Arguments otherwise mean what they mean for the
+ * binary32 launcher. */
+void awp_dvelcx_s16(uint16_t* u1, uint16_t* v1, uint16_t* w1, uint16_t* xx, uint16_t* yy,
+                    uint16_t* zz, uint16_t* xy, uint16_t* xz, uint16_t* yz, float* dcrjx,
+                    float* dcrjy, float* dcrjz, float* d_1, int nyt, int nzt, hipStream_t St,
+                    int s_i, int e_i, int d_i, int ngrids, int fmt, const float* scale);
+void awp_dstrqc_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* xz,
+                    uint16_t* yz, uint16_t* r1, uint16_t* r2, uint16_t* r3, uint16_t* r4,
+                    uint16_t* r5, uint16_t* r6, uint16_t* u1, uint16_t* v1, uint16_t* w1,
+                    float* lam, float* mu, float* qp, float* coeff, float* qs, float* dcrjx,
+                    float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St, float* lam_mu,
+                    float* vx1, float* vx2, int* ww, float* wwo, int NX, int NPC, int rankx,
+                    int ranky, int s_i, int e_i, int s_j, int e_j, int d_i, int fmt,
+                    const float* scale);
+void awp_fstr_s16(uint16_t* zz, uint16_t* xz, uint16_t* yz, hipStream_t St, int s_i, int e_i,
+                  int s_j, int e_j, int fmt, const float* scale);
+void awp_addsrc_s16(int i, int READ_STEP, int dim, int* psrc, int npsrc, hipStream_t St,
+                    float* axx, float* ayy, float* azz, float* axz, float* ayz, float* axy,
+                    uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* yz,
+                    uint16_t* xz, int d_i, int fmt, const float* scale);
+void awp_receivers_gather_s16(const uint16_t* u1, const uint16_t* v1, const uint16_t* w1,
+                              const int* idx, int npts, float* out, int step_slot,
+                              hipStream_t St, int d_i, int fmt, const float* scale);
+/* Element-wise helpers on n stored values: encode binary32 x (q = rne(x /
+ * scale)), decode (x = q * scale), rescale in place by 2^shift (exact unless
+ * a value leaves the format's range), and the largest |q| (unit scale) into
+ * the device float *out (NaN if any stored value is NaN). */
+void awp_s16_encode(const float* x, uint16_t* q, long long n, int fmt, float scale,
+                    hipStream_t St);
+void awp_s16_decode(const uint16_t* q, float* x, long long n, int fmt, float scale,
+                    hipStream_t St);
+void awp_s16_rescale(uint16_t* q, long long n, int fmt, int shift, hipStream_t St);
+void awp_s16_absmax(const uint16_t* q, long long n, int fmt, float* out, hipStream_t St);
 
 #ifdef __cplusplus
 }

@@ -185,12 +185,15 @@ void dstrqc(T* xx, T* yy, T* zz, T* xy, T* xz, T* yz, T* r1, T* r2, T* r3,
             const T* dcrjy, const T* dcrjz, const T* lam_mu, const T* vx1,
             const T* vx2, const int* ww, const T* wwo, int NX, int NPC,
             int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i,
-            const Grid<T>& g) {
+            const Grid<T>& g, int parts = 3) {
+  // parts: bit 0 the free-surface velocity ghosts (grid 0), bit 1 the stress
+  // sweep; the GPU runs them as two passes, the reference as one kernel.
   const long sl = g.slice, yl = g.yline;
   const int mink = ALIGN + 3;
   const int maxk = (d_i == 0) ? g.nzt + ALIGN - 1 : g.nzt + ALIGN - 3;
-  if (d_i == 0)
+  if (d_i == 0 && (parts & 1))
     fs_ghosts<T, NEWLIT>(u1, v1, w1, lam_mu, g, NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j);
+  if (!(parts & 2)) return;
   const T c1 = g.c1, c2 = g.c2, DT = g.DT, dth = g.dth, dh1 = g.dh1;
 #pragma omp parallel for schedule(static)
   for (int i = s_i; i <= e_i; ++i) {
@@ -775,6 +778,70 @@ void swap3(T* const* L9, const T* const* H9, const Grid<T>& gl, const Grid<T>& g
       }
 }
 
+// ---------------------------------------------------------------------------
+// 16-bit storage codecs: binary32 -> bfloat16 / binary16 with round to
+// nearest even (binary16 with subnormals, overflow to infinity), and back.
+inline uint16_t f32_to_bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+inline float bf16_to_f32(uint16_t q) {
+  const uint32_t u = (uint32_t)q << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
+inline uint16_t f32_to_f16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  const uint32_t sign = (u >> 16) & 0x8000u;
+  const uint32_t a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);   // NaN
+  if (a >= 0x47800000u) return (uint16_t)(sign | 0x7c00u);  // >= 2^16: infinity
+  if (a >= 0x38800000u) {                                   // binary16 normal range
+    const uint32_t mant = a & 0x7fffffu;
+    uint32_t h = ((((a >> 23) - 127u + 15u)) << 10) | (mant >> 13);
+    const uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h += 1u;  // may carry into inf
+    return (uint16_t)(sign | h);
+  }
+  if (a <= 0x33000000u) return (uint16_t)sign;  // <= 2^-25: rounds to zero
+  const uint32_t mant = (a & 0x7fffffu) | 0x800000u;
+  const uint32_t shift = 126u - (a >> 23);       // 14 .. 24
+  uint32_t h = mant >> shift;
+  const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+  if (rem > half || (rem == half && (h & 1u))) h += 1u;
+  return (uint16_t)(sign | h);
+}
+
+inline float f16_to_f32(uint16_t q) {
+  const uint32_t sign = (uint32_t)(q & 0x8000u) << 16;
+  const uint32_t e = (q >> 10) & 0x1fu, m = q & 0x3ffu;
+  uint32_t u;
+  if (e == 0) {
+    if (m == 0) {
+      u = sign;
+    } else {  // subnormal: normalise
+      int sh = 0;
+      uint32_t mm = m;
+      while (!(mm & 0x400u)) { mm <<= 1; ++sh; }
+      u = sign | ((uint32_t)(127 - 15 + 1 - sh) << 23) | ((mm & 0x3ffu) << 13);
+    }
+  } else if (e == 31) {
+    u = sign | 0x7f800000u | (m << 13);
+  } else {
+    u = sign | ((e - 15u + 127u) << 23) | (m << 13);
+  }
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
 }  // namespace
 
 extern "C" {
@@ -812,6 +879,21 @@ AWP_ORACLE_DVELCX(f64, double)
                       g);                                                           \
   }
 AWP_ORACLE_DSTRQC(awp_oracle_dstrqc_f32, float, false)
+
+void awp_oracle_dstrqc_parts_f32(
+    float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1, float* r2,
+    float* r3, float* r4, float* r5, float* r6, float* u1, float* v1, float* w1,
+    const float* lam, const float* mu, const float* qp, const float* coeff, const float* qs,
+    const float* dcrjx, const float* dcrjy, const float* dcrjz, const float* lam_mu,
+    const float* vx1, const float* vx2, const int* ww, const float* wwo, int nxt, int nyt,
+    int nzt, double DH, double DT, int NX, int NPC, int rankx, int ranky, int s_i, int e_i,
+    int s_j, int e_j, int d_i, int parts) {
+  ftz_daz();
+  const Grid<float> g = make_grid<float>(DH, DT, nxt, nyt, nzt);
+  dstrqc<float, false>(xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp,
+                       coeff, qs, dcrjx, dcrjy, dcrjz, lam_mu, vx1, vx2, ww, wwo, NX, NPC,
+                       rankx, ranky, s_i, e_i, s_j, e_j, d_i, g, parts);
+}
 AWP_ORACLE_DSTRQC(awp_oracle_dstrqc_f64, double, false)
 AWP_ORACLE_DSTRQC(awp_oracle_dstrqc_new_f32, float, true)
 
@@ -923,5 +1005,23 @@ void awp_oracle_rotate_principal_f64(double sigma2, double pfluid, const double*
   }
 AWP_ORACLE_DM(f32, float)
 AWP_ORACLE_DM(f64, double)
+
+// q = rne(x * inv) and x = dec(q) * s over n values (fmt 1 bfloat16, 2
+// binary16), with the binary32 arithmetic in flush-to-zero mode like the GPU.
+void awp_oracle_s16_encode(const float* x, uint16_t* q, long long n, int fmt, float inv) {
+  ftz_daz();
+  for (long long i = 0; i < n; ++i) {
+    volatile float y = x[i] * inv;
+    q[i] = fmt == 1 ? f32_to_bf16(y) : f32_to_f16(y);
+  }
+}
+
+void awp_oracle_s16_decode(const uint16_t* q, float* x, long long n, int fmt, float s) {
+  ftz_daz();
+  for (long long i = 0; i < n; ++i) {
+    volatile float y = fmt == 1 ? bf16_to_f32(q[i]) : f16_to_f32(q[i]);
+    x[i] = y * s;
+  }
+}
 
 }  // extern "C"

@@ -54,6 +54,9 @@ def load() -> C.CDLL:
             getattr(_lib, f"awp_oracle_intp3d_{suf}").argtypes = [P, P] + [I] * 6
             getattr(_lib, f"awp_oracle_swap3_{suf}").argtypes = [P, P] + [I] * 6
         _lib.awp_oracle_rotate_principal_f64.argtypes = [D, D, P, P]
+        _lib.awp_oracle_dstrqc_parts_f32.argtypes = [P] * 28 + [I, I, I, D, D] + [I] * 10
+        _lib.awp_oracle_s16_encode.argtypes = [P, P, C.c_longlong, I, C.c_float]
+        _lib.awp_oracle_s16_decode.argtypes = [P, P, C.c_longlong, I, C.c_float]
         _lib.awp_oracle_dstrqc_new_f32.argtypes = [P] * 28 + [I, I, I, D, D] + [I] * 9
         _lib.awp_oracle_set_threads.argtypes = [I]
         _lib.awp_oracle_get_threads.restype = I
@@ -273,3 +276,106 @@ class OracleSolver(PmclStepOrder):
 
     def traces(self) -> np.ndarray:
         return self.rec["out"] if self.rec is not None else np.zeros((0, 0, 3))
+
+
+class OracleSolver16(OracleSolver):
+    """The 16-bit storage mode (awp_amd.storage) on the oracle: the state is
+    the stored 16-bit values; every launch decodes them to binary32, runs the
+    FP32 oracle function and encodes the fields the launch writes.  The
+    free-surface ghosts are a pass of their own before the stress sweep, as
+    on the GPU, so the sweep reads the stored ghosts."""
+
+    def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
+                 storage: str = "bf16", check_every: int = 8):
+        from awp_amd.storage import FIELDS16, FORMATS, initial_scales
+        super().__init__(medium, sources, receivers, nt_record, dtype=np.float32)
+        self.storage = storage
+        self.fmt = FORMATS[storage]
+        self.check_every = int(check_every)
+        self.fields16 = FIELDS16
+        L = self.levels[0]
+        L.q = {n: np.zeros(L.dims.shape, dtype=np.uint16) for n in FIELDS16}
+        self.scales = initial_scales(storage, medium, sources)
+        self.rescales = 0
+
+    # codecs
+    def _dec(self, names):
+        L = self.levels[0]
+        for n in names:
+            self.lib.awp_oracle_s16_decode(_p(L.q[n]), _p(L.f[n]), L.f[n].size, self.fmt,
+                                           float(self.scales[n]))
+
+    def _enc(self, names):
+        L = self.levels[0]
+        for n in names:
+            self.lib.awp_oracle_s16_encode(_p(L.f[n]), _p(L.q[n]), L.f[n].size, self.fmt,
+                                           float(1.0 / self.scales[n]))
+
+    def _absmax_all(self):
+        L = self.levels[0]
+        out = {}
+        tmp = np.empty(L.dims.shape, dtype=np.float32)
+        for n in self.fields16:
+            self.lib.awp_oracle_s16_decode(_p(L.q[n]), _p(tmp), tmp.size, self.fmt, 1.0)
+            out[n] = float(np.abs(tmp).max())
+        return out
+
+    def _rescale(self, name, shift):
+        L = self.levels[0]
+        tmp = np.empty(L.dims.shape, dtype=np.float32)
+        self.lib.awp_oracle_s16_decode(_p(L.q[name]), _p(tmp), tmp.size, self.fmt,
+                                       float(2.0 ** shift))
+        self.lib.awp_oracle_s16_encode(_p(tmp), _p(L.q[name]), tmp.size, self.fmt, 1.0)
+
+    def check_scales(self):
+        from awp_amd.storage import ScaledStorage
+        return ScaledStorage.check_scales(self)
+
+    def step(self):
+        super().step()
+        if self.check_every and self.cur_step % self.check_every == 0:
+            self.check_scales()
+
+    # launches
+    def velocity(self, p=0):
+        self._dec(self.fields16[:9])
+        super().velocity(p)
+        self._enc(("u1", "v1", "w1"))
+
+    def stress_region(self, p, s_i, e_i, s_j, e_j):
+        if s_i > e_i or s_j > e_j:
+            return
+        L = self.levels[p]
+        d, f, m = L.dims, L.f, L.m
+        names = ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4", "r5", "r6", "u1",
+                 "v1", "w1")
+        for parts, out in ((1, ("u1", "v1", "w1")),
+                           (2, ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                "r5", "r6"))):
+            self._dec(self.fields16)
+            self.lib.awp_oracle_dstrqc_parts_f32(
+                *(_p(f[n]) for n in names), _p(m["lam"]), _p(m["mu"]), _p(m["qp"]),
+                _p(m["coeff"]), _p(m["qs"]), _p(m["dcrjx"]), _p(m["dcrjy"]), _p(m["dcrjz"]),
+                _p(m["lam_mu"]), _p(m["vx1"]), _p(m["vx2"]), _p(m["ww"]), _p(m["wwo"]), d.nxt,
+                d.nyt, d.nzt, L.DH, L.DT, d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, p, parts)
+            self._enc(out)
+
+    def add_source(self, p, sample_step):
+        if self.levels[p].src is None:
+            return
+        self._dec(("xx", "yy", "zz", "xy", "xz", "yz"))
+        super().add_source(p, sample_step)
+        self._enc(("xx", "yy", "zz", "xy", "xz", "yz"))
+
+    def free_surface(self):
+        self._dec(("zz", "xz", "yz"))
+        super().free_surface()
+        self._enc(("zz", "xz", "yz"))
+
+    def record(self, slot):
+        self._dec(("u1", "v1", "w1"))
+        super().record(slot)
+
+    def field(self, name):
+        self._dec((name,))
+        return self.levels[0].f[name].copy()
