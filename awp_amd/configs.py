@@ -292,3 +292,47 @@ def dm_plastic(nx: int = 256, ny: int = 256, nz: int = 192, nz_fine: int = 96,
                  dict(cohesion=cohesion, friction=friction, Tv=Tv, M0=M0, dims=dims, DH=DHs))
     w.params["plastic_fn"] = mk_plastic
     return w
+
+
+def adjoint_pair(nx: int = 256, ny: int = 256, nz: int = 128, nt: int = 2000, DH: float = 100.0,
+                 DT: float = 0.005, nsrc: int = 4, nrec_side: int = 8, seed: int = 17,
+                 anomaly: float = 0.10, f0: float = 1.0, M0: float = 1.0e15):
+    """The adjoint-gradient workload: a smooth background (vs rising linearly
+    with depth 1500 -> 4500 m/s, vp = 1.73 vs, Gardner rho, Qs = 50 vs[km/s],
+    Qp = 2 Qs) and the same model with a Gaussian vs anomaly of the given
+    relative size at the centre (width a sixth of the box); nsrc seeded
+    explosive (isotropic) Ricker sources at mid depth; a nrec_side^2 grid of
+    surface receivers.  Returns (background medium, true medium, sources,
+    receivers)."""
+    dims = Dims(nx, ny, nz)
+    rng = np.random.default_rng(seed)
+    z = (np.arange(nz) + 0.5) * DH
+    vs1 = 1500.0 + 3000.0 * z / (nz * DH)
+    x = (np.arange(nx) + 0.5) * DH
+    y = (np.arange(ny) + 0.5) * DH
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    cx, cy, cz = nx * DH / 2, ny * DH / 2, nz * DH / 2
+    w = min(nx, ny, nz) * DH / 6.0
+    gauss = np.exp(-((X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2) / (2 * w * w))
+
+    def medium(vs):
+        vp = 1.73 * vs
+        rho = 310.0 * vp ** 0.25
+        Qs = 50.0 * vs / 1000.0
+        return build_medium(dims, DH, DT, vp, vs, rho, 2.0 * Qs, Qs)
+
+    vs_bg = np.broadcast_to(vs1[None, None, :], (nx, ny, nz))
+    bg = medium(vs_bg)
+    true = medium(vs_bg * (1.0 + anomaly * gauss))
+    t = np.arange(nt) * DT
+    cells = np.stack([rng.integers(nx // 4, 3 * nx // 4 + 1, nsrc),
+                      rng.integers(ny // 4, 3 * ny // 4 + 1, nsrc),
+                      rng.integers(nz // 3, 2 * nz // 3 + 1, nsrc)], axis=1)
+    rates = np.zeros((nsrc, nt, 6))
+    for n in range(nsrc):
+        rates[n, :, 0] = rates[n, :, 1] = rates[n, :, 2] = M0 * ricker(t, f0, t0=1.2 / f0 + 0.05 * n)
+    src = PointSources(cells, rates)
+    gx = np.linspace(max(1, nx // 10), nx - max(1, nx // 10), nrec_side).round().astype(int)
+    gy = np.linspace(max(1, ny // 10), ny - max(1, ny // 10), nrec_side).round().astype(int)
+    rec = Receivers(np.array([[a, b] for a in gx for b in gy]))
+    return bg, true, src, rec

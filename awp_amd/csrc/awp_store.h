@@ -36,12 +36,19 @@ __host__ __device__ __forceinline__ float awp_bf16_to_f32(uint16_t q) {
   return __builtin_bit_cast(float, (uint32_t)q << 16);
 }
 
+// The empty asm keeps the value in a register between the binary32 multiply
+// by the scale and the conversion: fused, the pair becomes v_fma_mix*,
+// which follows the binary32 denormal mode (flush) and so flushes binary16
+// subnormals that v_cvt_f16_f32 / v_cvt_f32_f16 keep.
 __device__ __forceinline__ uint16_t awp_f32_to_f16(float x) {
+  asm volatile("" : "+v"(x));
   return __builtin_bit_cast(uint16_t, (_Float16)x);  // v_cvt_f16_f32, round to nearest even
 }
 
 __device__ __forceinline__ float awp_f16_to_f32(uint16_t q) {
-  return (float)__builtin_bit_cast(_Float16, q);
+  float x = (float)__builtin_bit_cast(_Float16, q);
+  asm volatile("" : "+v"(x));
+  return x;
 }
 
 // Element access by index (ld/st) or by a uniform row pointer plus a 32-bit

@@ -213,6 +213,39 @@ void awp_s16_decode(const uint16_t* q, float* x, long long n, int fmt, float sca
 void awp_s16_rescale(uint16_t* q, long long n, int fmt, int shift, hipStream_t St);
 void awp_s16_absmax(const uint16_t* q, long long n, int fmt, float* out, hipStream_t St);
 
+/* ---- Adjoint and material gradient (no reference counterpart) ----------
+ * For callers that need dJ/d(lambda, mu, rho) of a receiver misfit through
+ * the NVE = 1 step of one grid (awp_amd/adjoint.py drives them).
+ * awp_adjoint_step applies the transpose of step n to the adjoint state
+ * adj[15] (xx yy zz xy xz yz r1..r6 u1 v1 w1, binary32, in place), reading
+ * the forward velocities after step n (fwd_v[3]: u1 v1 w1) and the forward
+ * stresses before it (fwd_s[6]: xx yy zz xy xz yz); med[9] = d1 lam mu qp qs
+ * vx1 vx2 wwo lam_mu; scratch[9] are zero-initialised full-size arrays that
+ * the call reuses; grad[8] (f_d1 f_d2 f_d3 xl xm xmu1 xmu2 xmu3) and
+ * grad_lam_mu accumulate coefficient gradients in binary32; col holds
+ * 3 x (nxt+20) x (nyt+20) floats of per-column scratch. */
+void awp_adjoint_step(float* const* adj, const float* const* fwd_v, const float* const* fwd_s,
+                      const float* const* med, const int* ww, const float* coeff,
+                      const float* dcrjx, const float* dcrjy, const float* dcrjz,
+                      float* const* scratch, float* const* grad, float* grad_lam_mu,
+                      float* col, hipStream_t St);
+/* adds the binary64 residuals res[3*n + c] of receiver n (the awp_receivers_
+ * gather sampling) to the adjoint velocities */
+void awp_adjoint_inject(float* ub, float* vb, float* wb, const int* idx, int npts,
+                        const double* res, hipStream_t St);
+/* g64 += g32; g32 = 0 over n values */
+void awp_adjoint_flush(float* g32, double* g64, long long n, hipStream_t St);
+/* res = d - obs (binary64) and *J = 1/2 sum res^2 over n samples; work holds
+ * 256 doubles */
+void awp_misfit(const float* traces, const float* obs, long long n, double* res, double* J,
+                double* work, hipStream_t St);
+/* maps the binary64 coefficient gradients gcoef[8] and grad_lam_mu to the
+ * gradients in lam (1/lambda), mu (1/mu) and d1 (rho) of every padded cell */
+void awp_adjoint_map(const double* const* gcoef, const double* grad_lam_mu,
+                     const float* const* med, const int* ww, const float* coeff,
+                     const float* dcrjx, const float* dcrjy, const float* dcrjz, double* glam,
+                     double* gmu, double* grho, hipStream_t St);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,149 @@
+"""Material gradient dJ/d(lam, mu, rho) of a receiver misfit.
+
+The CPU reference gradient is reverse-mode differentiation (torch autograd,
+float64) of a vectorised restatement of the forward step (oracle/torch_ref);
+that restatement is pinned to the FP64 oracle's traces, and its gradient to
+central finite differences.  The GPU gradient comes from the hand-written
+adjoint kernels (awp_adjoint_step) with checkpointed recomputation; it is
+checked against the CPU reference gradient field by field, against central
+finite differences of the GPU misfit along seeded random directions, and
+for independence of the checkpoint interval.
+"""
+import numpy as np
+import pytest
+import torch
+
+from awp_amd import configs
+
+NT = 60
+
+
+def _case():
+    return configs.adjoint_pair(nx=20, ny=18, nz=16, nt=NT, DH=100.0, DT=0.005, nsrc=2,
+                                nrec_side=3, f0=4.0)
+
+
+def _arrays(med):
+    return (torch.from_numpy(med.lam.copy()), torch.from_numpy(med.mu.copy()),
+            torch.from_numpy(med.d1.copy()))
+
+
+def _reference(bg, true, src, rec):
+    from oracle.torch_ref import TorchForward
+    obs = TorchForward(true, src, rec, NT).run(*_arrays(true)).detach()
+    tf = TorchForward(bg, src, rec, NT)
+    lam, mu, rho = (a.requires_grad_(True) for a in _arrays(bg))
+    d = tf.run(lam, mu, rho)
+    J = 0.5 * ((d - obs) ** 2).sum()
+    J.backward()
+    return tf, obs, float(J), lam.grad, mu.grad, rho.grad
+
+
+@pytest.fixture(scope="module")
+def case():
+    bg, true, src, rec = _case()
+    return (bg, true, src, rec) + _reference(bg, true, src, rec)
+
+
+def test_torch_reference_matches_oracle(oracle_lib, case):
+    from oracle.oracle import OracleSolver
+    bg, true, src, rec, tf = case[:5]
+    o = OracleSolver(bg, src, rec, nt_record=NT, dtype=np.float64)
+    o.run(NT)
+    d = tf.run(*_arrays(bg)).detach().numpy()
+    assert np.abs(o.traces()).max() > 0
+    assert np.linalg.norm(d - o.traces()) <= 1e-12 * np.linalg.norm(o.traces())
+
+
+def test_reference_gradient_matches_finite_differences(case):
+    bg, true, src, rec, tf, obs, J0, gl, gm, gr = case
+    rng = np.random.default_rng(5)
+    lam0, mu0, rho0 = _arrays(bg)
+    for name, base, g in (("lam", lam0, gl), ("mu", mu0, gm), ("rho", rho0, gr)):
+        dirn = torch.from_numpy(rng.standard_normal(base.shape)) * base
+        eps = 1e-4
+
+        def J(a):
+            args = {"lam": lam0, "mu": mu0, "rho": rho0}
+            args[name] = a
+            d = tf.run(args["lam"], args["mu"], args["rho"]).detach()
+            return float(0.5 * ((d - obs) ** 2).sum())
+        fd = (J(base + eps * dirn) - J(base - eps * dirn)) / (2 * eps)
+        ad = float((g * dirn).sum())
+        assert abs(fd - ad) <= 1e-5 * abs(ad), (name, fd, ad)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return True
+
+
+def _gpu_problem(case, K):
+    from awp_amd.adjoint import AdjointProblem
+    bg, true, src, rec, tf, obs = case[:6]
+    return AdjointProblem(bg, src, rec, NT, obs.numpy().astype(np.float32), checkpoint_every=K)
+
+
+def _rel(a, b):
+    return float(torch.linalg.norm((a - b).flatten()) / torch.linalg.norm(b.flatten()))
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_reference(gpu, case):
+    from awp_amd.adjoint import misfit
+    bg = case[0]
+    J0, gl, gm, gr = case[6:]
+    P = _gpu_problem(case, 7)
+    lam, mu, rho = (a.clone().requires_grad_(True) for a in _arrays(bg))
+    J = misfit(lam, mu, rho, P)
+    J.backward()
+    errs = {n: _rel(a.grad, b) for n, a, b in (("lam", lam, gl), ("mu", mu, gm), ("rho", rho, gr))}
+    print(f"GPU adjoint vs torch-AD FP64 reference: J {float(J):.6e} vs {J0:.6e}; "
+          f"gradient rel-L2 {errs}")
+    assert abs(float(J) / J0 - 1) < 1e-4
+    for n, e in errs.items():
+        assert e < 2e-3, (n, e)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_finite_differences(gpu, case):
+    """Directional derivatives along 5 seeded random directions (relative
+    size 1 of each array's magnitude, all three arrays at once) against
+    central differences of the GPU misfit."""
+    bg = case[0]
+    P = _gpu_problem(case, 10)
+    lam0, mu0, rho0 = _arrays(bg)
+    P.set_medium(lam0, mu0, rho0)
+    P.forward()
+    glam, gmu, grho = (g.cpu() for g in P.backward())
+    rng = np.random.default_rng(9)
+    worst = 0.0
+    for t in range(5):
+        dl, dm, dr = (torch.from_numpy(rng.standard_normal(a.shape)) * a for a in (lam0, mu0, rho0))
+        eps = 2e-3
+
+        def J(s):
+            P.set_medium(lam0 + s * dl, mu0 + s * dm, rho0 + s * dr)
+            return P.forward()
+        fd = (J(eps) - J(-eps)) / (2 * eps)
+        ad = float((glam * dl).sum() + (gmu * dm).sum() + (grho * dr).sum())
+        err = abs(fd - ad) / abs(ad)
+        worst = max(worst, err)
+        print(f"direction {t}: adjoint {ad:.6e} finite difference {fd:.6e} rel {err:.2e}")
+    assert worst < 1e-3
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_independent_of_checkpoint_interval(gpu, case):
+    bg = case[0]
+    out = []
+    for K in (1, 13, NT):
+        P = _gpu_problem(case, K)
+        P.set_medium(*_arrays(bg))
+        P.forward()
+        out.append([g.cpu() for g in P.backward()])
+    for g in out[1:]:
+        for a, b in zip(g, out[0]):
+            assert _rel(a, b) < 1e-6
