@@ -52,13 +52,17 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps with no event timing (for rocprofv3)")
-    ap.add_argument("--workload", choices=("anelastic", "dm"), default="anelastic",
+    ap.add_argument("--workload", choices=("anelastic", "dm", "adjoint"), default="anelastic",
                     help="anelastic: the headline single-grid workload; dm: discontinuous mesh "
                          "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz)")
     ap.add_argument("--nz-fine", type=int, default=96)
     ap.add_argument("--storage", choices=("f32", "bf16", "fp16"), default="f32",
                     help="wavefield storage of the anelastic workload (16-bit: per-field "
                          "power-of-two scales, binary32 arithmetic)")
+    ap.add_argument("--nt", type=int, default=2000, help="adjoint workload: time steps")
+    ap.add_argument("--checkpoint-every", type=int, default=50)
+    ap.add_argument("--fd-directions", type=int, default=5,
+                    help="adjoint workload: random directions of the finite-difference check")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy a BYTES device buffer twice (device-to-device): the "
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -124,6 +128,9 @@ def main():
         return
     if args.workload == "dm":
         bench_dm(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
 
     def workload(nx, ny):
@@ -259,6 +266,87 @@ def copy_probe(nbytes: int) -> None:
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / 2
     print(json.dumps({"copy_probe_bytes": n * 4, "GBps_read_plus_write": 2 * n * 4 / dt / 1e9}))
+
+
+def bench_adjoint(args) -> None:
+    """Adjoint gradient: smooth background vs a 10 % Gaussian vs anomaly,
+    4 explosive sources, 8 x 8 surface receivers, 256 x 256 x 128 cells at
+    h = 100 m unless --nx/--ny/--nz say otherwise.  Times one plain forward
+    (FP32, the pmcl3d step) and one gradient (checkpointed forward with the
+    binary64 misfit, segment recomputation, adjoint sweep, mapping to
+    lam, mu, rho), and checks the gradient against central finite
+    differences of the misfit along seeded random directions."""
+    import numpy as np
+    import torch
+
+    from awp_amd import _lib, configs
+    from awp_amd.adjoint import AdjointProblem
+    from awp_amd.solver import Solver
+
+    nx = args.nx if args.nx != 512 else 256
+    ny = args.ny if args.ny != 512 else 256
+    nz = args.nz if args.nz != 256 else 128
+    nt = args.nt
+    t0 = time.perf_counter()
+    bg, true, src, rec = configs.adjoint_pair(nx=nx, ny=ny, nz=nz, nt=nt)
+    setup_s = time.perf_counter() - t0
+    print(f"adjoint setup {setup_s:.1f} s", file=sys.stderr, flush=True)
+    obs_run = Solver(true, src, rec, nt_record=nt)
+    obs_run.run(nt)
+    obs = obs_run.traces()
+    del obs_run
+    torch.cuda.empty_cache()
+    P = AdjointProblem(bg, src, rec, nt, obs, checkpoint_every=args.checkpoint_every)
+    lam0 = torch.from_numpy(bg.lam.copy())
+    mu0 = torch.from_numpy(bg.mu.copy())
+    rho0 = torch.from_numpy(bg.d1.copy())
+    P.set_medium(lam0, mu0, rho0)
+    # plain forward, as the forward-only workload runs it
+    plain = P.solver
+    P._reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.run(nt)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    print(f"plain forward {t_fwd:.3f} s", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J = P.forward()
+    glam, gmu, grho = P.backward()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    print(f"gradient {t_grad:.3f} s", file=sys.stderr, flush=True)
+    glam, gmu, grho = glam.cpu(), gmu.cpu(), grho.cpu()
+    rng = np.random.default_rng(2024)
+    fd = []
+    for n in range(args.fd_directions):
+        dl, dm, dr = (torch.from_numpy(rng.standard_normal(a.shape)) * a
+                      for a in (lam0, mu0, rho0))
+        eps = 2e-3
+
+        def Jof(s):
+            P.set_medium(lam0 + s * dl, mu0 + s * dm, rho0 + s * dr)
+            return P.forward()
+        f = (Jof(eps) - Jof(-eps)) / (2 * eps)
+        a = float((glam * dl).sum() + (gmu * dm).sum() + (grho * dr).sum())
+        fd.append({"adjoint": a, "finite_difference": f, "rel_err": abs(f - a) / abs(a)})
+        print(f"direction {n}: {fd[-1]}", file=sys.stderr, flush=True)
+    cells = nx * ny * nz
+    out = {
+        "metric": METRIC, "value": t_grad / t_fwd, "unit": "gradient time / plain forward time",
+        "n_gpus": 1, "steps": nt, "warmup": 0, "ms_per_step": t_fwd * 1e3 / nt,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 wavefields, f64 misfit and gradient", "data": "synthetic",
+        "config": {"workload": f"adjoint_{nx}x{ny}x{nz}", "h_m": 100.0, "dt_s": 0.005, "nt": nt,
+                   "checkpoint_every": args.checkpoint_every, "sources": "4 explosive",
+                   "receivers": "8 x 8 surface", "model": "smooth vs(z) + 10 % Gaussian anomaly"},
+        "plain_forward_s": t_fwd, "gradient_s": t_grad,
+        "forward_cell_updates_per_s": cells * nt / t_fwd, "misfit": J, "fd_check": fd,
+        "fd_max_rel_err": max(x["rel_err"] for x in fd) if fd else None,
+        "setup_s": setup_s, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
 
 
 def bench_dm(args) -> None:
