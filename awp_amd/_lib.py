@@ -52,6 +52,20 @@ SIGNATURES: dict[str, list] = {
     "awp_adjoint_flush": [_fp, _fp, C.c_longlong, _s],
     "awp_misfit": [_fp, _fp, C.c_longlong, _fp, _fp, _fp, _s],
     "awp_adjoint_map": [_fp] * 11 + [_s],
+    # awp_topo_t* travels as a plain address (ctypes.byref of awp_amd.topography.AwpTopo)
+    "topo_set_constants": [_fp],
+    "topo_velocity_interior_H": [_fp],
+    "topo_velocity_front_H": [_fp],
+    "topo_velocity_back_H": [_fp],
+    "topo_stress_interior_H": [_fp],
+    "topo_stress_left_H": [_fp],
+    "topo_stress_right_H": [_fp],
+    "awp_topo_velocity_s16": [_fp, _i, _fp],
+    "awp_topo_stress_s16": [_fp, _i, _fp],
+    "awp_topo_geom_custom": [_fp, _i, _i, _i, _f, _s],
+    "awp_topo_metrics_f": [_fp, _i, _i, _f, _s],
+    "awp_topo_geom_no_stretching": [_fp, _i, _f, _s],
+    "awp_topo_metrics_g": [_fp, _i, _f, _s],
 }
 RESTYPES = {
     "awp_last_error": _i,
@@ -70,6 +84,11 @@ def load() -> C.CDLL:
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m awp_amd.build` "
             "(the HIP path has no CPU fallback)")
+    # torch ships its own HIP runtime (libamdhip64.so.7 in torch/lib).  Loading
+    # torch first makes the library bind to that copy by soname, so kernels and
+    # torch's allocations share one runtime; loaded the other way round the
+    # process holds two runtimes and the second reports no device.
+    import torch  # noqa: F401
     lib = C.CDLL(str(LIB_PATH))
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)

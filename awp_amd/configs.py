@@ -10,6 +10,9 @@ configurations of the time-stepping path.
                          von Karman perturbation (a = 500 m, Hurst 0.5),
                          Qs = 50 vs[km/s], Qp = 2 Qs, double-couple Ricker
                          source f0 = 1 Hz.
+  topography_384         384x384x192, h=100 m, dt=5 ms, 1500 steps, the same
+                         medium under a free surface of 6 seeded Gaussian
+                         hills (height <= 1.5 km, widths 2-8 km).
 Every random draw is seeded.  Sizes can be scaled down (``scale``) for parity
 tests; the physics parameters stay the same.
 """
@@ -180,6 +183,24 @@ def anelastic(nx: int = 512, ny: int = 512, nz: int = 256, nt: int = 1000, DH: f
 
     return Workload("anelastic_%dx%dx%d" % (nx, ny, nz), dims, DH, DT, nt, mk, src, rec,
                     dict(f0=f0, M0=M0, seed=seed))
+
+
+def topography_hills(nx: int = 384, ny: int = 384, nz: int = 192, nt: int = 1500,
+                     DH: float = 100.0, DT: float = 0.005, seed: int = 7, nhills: int = 6,
+                     hmax: float = 1500.0, wmin: float = 2000.0, wmax: float = 8000.0,
+                     nrec: int = 16) -> Workload:
+    """Curvilinear topography: the anelastic workload's layered + random
+    medium with Q and double-couple Ricker source on a grid whose free surface
+    is a seeded sum of nhills Gaussian hills (height <= hmax, widths in
+    [wmin, wmax]); surface receivers.  The topography rides in
+    params["topography"]."""
+    from .topography import Topography
+    w = anelastic(nx=nx, ny=ny, nz=nz, nt=nt, DH=DH, DT=DT, seed=seed, nrec=nrec)
+    w.name = "topography_%dx%dx%d" % (nx, ny, nz)
+    w.params["topography"] = Topography.gaussian_hills(nx, ny, DH, n=nhills, seed=seed + 3,
+                                                       hmax=hmax, wmin=wmin, wmax=wmax)
+    w.params.update(hmax=hmax, wmin=wmin, wmax=wmax, nhills=nhills)
+    return w
 
 
 def line_source(dims: Dims, nsub: int, nt: int, DT: float, f0: float, M0: float, seed: int,

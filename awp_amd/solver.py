@@ -119,7 +119,7 @@ class Solver(PmclStepOrder):
                  sources: PointSources | Sequence | None = None,
                  receivers: Receivers | None = None, nt_record: int = 0,
                  device: str = "cuda", variant: str = "dstrqc", NPC: int = 0,
-                 plastic: PlasticMedium | Sequence | None = None):
+                 plastic: PlasticMedium | Sequence | None = None, topography=None):
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a GPU: the HIP path has no CPU fallback")
         self.lib = _lib.load()
@@ -152,6 +152,16 @@ class Solver(PmclStepOrder):
         self.lib.SetDeviceConstValue(DH, C.c_float(media[0].DT), nxt, nyt, nzt, G, fmaj, fmin,
                                      Rzc, RzTc)
         _lib.check()
+        # curvilinear topography replaces grid 0's velocity and stress updates
+        # (pmcl3d.c:1661-1694, 1843-1856)
+        self.topo = None
+        if topography is not None:
+            if G != 1 or plastic is not None:
+                raise ValueError("topography runs one grid without plasticity")
+            from .topography import TopoDevice
+            self.topo = TopoDevice(self.lib, self.levels[0], topography, self.device,
+                                   self.stream)
+            _lib.check()
         # receivers on grid 0
         self.rec = None
         d0 = self.levels[0].dims
@@ -193,6 +203,9 @@ class Solver(PmclStepOrder):
 
     # -- the launchers --
     def velocity(self, p: int = 0) -> None:
+        if p == 0 and self.topo is not None:
+            self.topo.velocity(self.stream)
+            return
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         s_i, e_i = d.i_range
@@ -230,6 +243,10 @@ class Solver(PmclStepOrder):
         self.stress_region(p, s_i, e_i, s_j, e_j)
 
     def stress_region(self, p: int, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
+        if p == 0 and self.topo is not None:
+            # the curvilinear update covers its own bounds (topography.c:377-420)
+            self.topo.stress(self.stream)
+            return
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:
@@ -322,12 +339,12 @@ class Solver16(ScaledStorage, Solver):
 
     def __init__(self, medium: Medium, sources: PointSources | None = None,
                  receivers: Receivers | None = None, nt_record: int = 0, device: str = "cuda",
-                 storage: str = "bf16", check_every: int = 8):
+                 storage: str = "bf16", check_every: int = 8, topography=None):
         if isinstance(medium, (list, tuple)):
             raise ValueError("16-bit storage runs one grid")
         if storage not in FORMATS:
             raise ValueError(f"storage must be one of {sorted(FORMATS)}")
-        super().__init__(medium, sources, receivers, nt_record, device)
+        super().__init__(medium, sources, receivers, nt_record, device, topography=topography)
         self.storage = storage
         self.fmt = FORMATS[storage]
         self.check_every = int(check_every)
@@ -338,6 +355,8 @@ class Solver16(ScaledStorage, Solver):
         self._absbuf = torch.zeros(len(FIELDS16), dtype=torch.float32, device=self.device)
         self.rescales = 0
         self._sync_scales()
+        if self.topo is not None:
+            self.topo.bind()
 
     def _sync_scales(self) -> None:
         self._sc = (C.c_float * len(FIELDS16))(*[self.scales[f] for f in FIELDS16])
@@ -368,6 +387,9 @@ class Solver16(ScaledStorage, Solver):
             self.check_scales()
 
     def velocity(self, p: int = 0) -> None:
+        if p == 0 and self.topo is not None:
+            self.topo.velocity_s16(self.stream, self.fmt, self._sc)
+            return
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         s_i, e_i = d.i_range
@@ -378,6 +400,9 @@ class Solver16(ScaledStorage, Solver):
                                 self.ngrids, self.fmt, self._sc)
 
     def stress_region(self, p: int, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
+        if p == 0 and self.topo is not None:
+            self.topo.stress_s16(self.stream, self.fmt, self._sc)
+            return
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:

@@ -246,6 +246,79 @@ void awp_adjoint_map(const double* const* gcoef, const double* grad_lam_mu,
                      const float* dcrjx, const float* dcrjy, const float* dcrjz, double* glam,
                      double* gmu, double* grho, hipStream_t St);
 
+/* ---- Curvilinear topography (grid 0) -------------------------------------
+ * The reference passes a topo_t (include/topography/topography.h:136-273) to
+ * its launchers; awp_topo_t keeps the members those launchers read, under
+ * the same names, with device pointers only.
+ *
+ * Metric arrays (device): the ten f* arrays are (nx+20) x (ny+84) floats,
+ * element (i, j) of topographic coordinates i in [-10, nx+10),
+ * j in [-42, ny+42) at (j+42) + (i+10)*(ny+84); f holds the geometry on
+ * [-8, nx+8) x [-8, ny+8) and the builders fill the others on
+ * [-6, n+6)^2.  The four g* arrays hold nz+64 floats, level k at k+32.
+ * Wavefield and medium arrays are the 3-D fields of the Cartesian kernels
+ * (topographic (i, j, k) is array element (i+10, j+10, k+32)). */
+typedef struct {
+  float *d_f, *d_f_1, *d_f_2, *d_f_c, *d_f1_1, *d_f1_2, *d_f1_c, *d_f2_1, *d_f2_2, *d_f2_c;
+} awp_f_grid_t; /* f_grid_t, include/topography/metrics/metrics.h */
+
+typedef struct {
+  float *d_g, *d_g3, *d_g3_c, *d_g_c;
+} awp_g_grid_t; /* g_grid_t, include/topography/metrics/metrics.h */
+
+typedef struct {
+  int use;
+  /* neighbour ranks, -1 on the domain boundary; on one GPU all four are -1 */
+  int x_rank_l, x_rank_r, y_rank_b, y_rank_f;
+  int nx, ny, nz;
+  float dth, timestep, gridspacing; /* dt/h (single precision), 1, h */
+  float *rho, *lami, *mui;
+  float *xx, *yy, *zz, *xy, *xz, *yz;
+  float *r1, *r2, *r3, *r4, *r5, *r6;
+  float *qpi, *qsi;
+  int* ww;
+  float *wwo, *vx1, *vx2, *coeff;
+  float *u1, *v1, *w1;
+  float *dcrjx, *dcrjy, *dcrjz;
+  awp_f_grid_t metrics_f;
+  awp_g_grid_t metrics_g;
+  hipStream_t stream_i;
+  /* written by topo_set_constants (the reference keeps them in __constant__
+   * memory, src/topography/kernels/stress_attenuation.cu:24-72) */
+  float c_DT, c_dth, c_dh1;
+  int constants_set;
+} awp_topo_t;
+
+/* topo_set_constants, src/topography/stress_attenuation.cu:19-23 */
+void topo_set_constants(awp_topo_t* T);
+/* include/topography/velocity.cuh:16-18: curvilinear velocity on levels
+ * 3 <= k < nz (interior stencils below nz-6, one-sided rows above).  front and
+ * back compute the halo-send strips of a multi-GPU run: with no neighbour
+ * (-1) they return, as the reference's do; a neighbour is an error here. */
+void topo_velocity_interior_H(awp_topo_t* T);
+void topo_velocity_front_H(awp_topo_t* T);
+void topo_velocity_back_H(awp_topo_t* T);
+/* include/topography/stress.cuh:17-19: curvilinear stress + Q on levels
+ * 3 <= k < nz of the stress bounds (topography.c:377-420) */
+void topo_stress_interior_H(awp_topo_t* T);
+void topo_stress_left_H(awp_topo_t* T);
+void topo_stress_right_H(awp_topo_t* T);
+/* no reference counterpart: the same updates with 16-bit wavefield storage
+ * (the wavefield pointers of T address uint16_t; fmt and scale as for
+ * awp_dvelcx_s16) */
+void awp_topo_velocity_s16(awp_topo_t* T, int fmt, const float* scale);
+void awp_topo_stress_s16(awp_topo_t* T, int fmt, const float* scale);
+/* geom_custom, src/topography/geometry/geometry.c:188-210: f = 1 + elevation
+ * / (h (nz-2)) in place on the f grid (f holds the elevation on entry) */
+void awp_topo_geom_custom(float* f, int nx, int ny, int nz, float h, hipStream_t St);
+/* metrics_build_f (src/topography/metrics/metrics.c:35-41, 141-170): the
+ * nine interpolants and derivatives of f */
+void awp_topo_metrics_f(const awp_f_grid_t* mf, int nx, int ny, float h, hipStream_t St);
+/* geom_no_grid_stretching, geometry.c:33-42: g(k) = h k below the top ghost */
+void awp_topo_geom_no_stretching(float* g, int nz, float h, hipStream_t St);
+/* metrics_build_g (metrics.c:223-229, 293-318): g_c, g3, g3_c from g */
+void awp_topo_metrics_g(const awp_g_grid_t* mg, int nz, float h, hipStream_t St);
+
 #ifdef __cplusplus
 }
 #endif

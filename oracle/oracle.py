@@ -58,6 +58,15 @@ def load() -> C.CDLL:
         _lib.awp_oracle_s16_encode.argtypes = [P, P, C.c_longlong, I, C.c_float]
         _lib.awp_oracle_s16_decode.argtypes = [P, P, C.c_longlong, I, C.c_float]
         _lib.awp_oracle_dstrqc_new_f32.argtypes = [P] * 28 + [I, I, I, D, D] + [I] * 9
+        for suf in ("f32", "f64"):
+            T = C.c_float if suf == "f32" else C.c_double
+            getattr(_lib, f"awp_oracle_topo_geom_{suf}").argtypes = [P, I, I, I, D]
+            getattr(_lib, f"awp_oracle_topo_metrics_f_{suf}").argtypes = [P, I, I, D]
+            getattr(_lib, f"awp_oracle_topo_metrics_g_{suf}").argtypes = [P] * 4 + [I, D, I]
+            getattr(_lib, f"awp_oracle_topo_vel_{suf}").argtypes = (
+                [P] * 15 + [T, T] + [I] * 7)
+            getattr(_lib, f"awp_oracle_topo_str_{suf}").argtypes = (
+                [P] * 29 + [I, I, I, D, D, I, I, I, I])
         _lib.awp_oracle_set_threads.argtypes = [I]
         _lib.awp_oracle_get_threads.restype = I
     return _lib
@@ -115,12 +124,71 @@ class _Level:
         return arr
 
 
+class TopoHost:
+    """Metric arrays of grid 0 built by the oracle's restatement of
+    geom_custom, metrics_build_f and metrics_build_g."""
+
+    def __init__(self, lib, topo, dims, DH: float, DT: float, dtype):
+        from awp_amd.topography import F_NAMES, G_NAMES, f_shape, g_size
+        topo.check(dims.nxt, dims.nyt)
+        self.lib = lib
+        self.suf = "f32" if np.dtype(dtype) == np.float32 else "f64"
+        self.dims = dims
+        self.DH, self.DT = DH, DT
+        self.mf = {n: np.zeros(f_shape(dims.nxt, dims.nyt), dtype=dtype) for n in F_NAMES}
+        self.mg = {n: np.zeros(g_size(dims.nzt), dtype=dtype) for n in G_NAMES}
+        self.mf["f"][...] = topo.f_input(dtype)
+        getattr(lib, f"awp_oracle_topo_geom_{self.suf}")(_p(self.mf["f"]), dims.nxt, dims.nyt,
+                                                         dims.nzt, DH)
+        getattr(lib, f"awp_oracle_topo_metrics_f_{self.suf}")(
+            (C.c_void_p * 10)(*[_p(self.mf[n]) for n in F_NAMES]), dims.nxt, dims.nyt, DH)
+        getattr(lib, f"awp_oracle_topo_metrics_g_{self.suf}")(
+            _p(self.mg["g"]), _p(self.mg["g_c"]), _p(self.mg["g3"]), _p(self.mg["g3_c"]),
+            dims.nzt, DH, 1)
+        self.pf = (C.c_void_p * 10)(*[_p(self.mf[n]) for n in F_NAMES])
+        self.pg = (C.c_void_p * 4)(*[_p(self.mg[n]) for n in G_NAMES])
+        # topo_t.dth = dt/h and timestep = 1 in _prec (topography.c:71-72)
+        T = np.float32 if self.suf == "f32" else np.float64
+        self.nu = float(T(DT) / T(DH))
+
+    def velocity(self, f, m):
+        d = self.dims
+        getattr(self.lib, f"awp_oracle_topo_vel_{self.suf}")(
+            _p(f["u1"]), _p(f["v1"]), _p(f["w1"]), _p(m["dcrjx"]), _p(m["dcrjy"]),
+            _p(m["dcrjz"]), self.pf, self.pg, _p(m["d1"]), _p(f["xx"]), _p(f["xy"]),
+            _p(f["xz"]), _p(f["yy"]), _p(f["yz"]), _p(f["zz"]), 1.0, self.nu, d.nxt, d.nyt,
+            d.nzt, 0, 0, d.nxt, d.nyt)
+
+    def stress_bounds(self):
+        """Array-index stress bounds of one rank without neighbours
+        (topography.c:377-420 plus the ngsl+2 shift)."""
+        d = self.dims
+        half, shift = NGSL_ORACLE // 2, NGSL_ORACLE + 2
+        return half + shift, d.nxt - half + shift, half + shift, d.nyt - half + shift
+
+    def stress(self, f, m):
+        d = self.dims
+        s_i, e_i, s_j, e_j = self.stress_bounds()
+        getattr(self.lib, f"awp_oracle_topo_str_{self.suf}")(
+            *(_p(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                 "r5", "r6", "u1", "v1", "w1")),
+            self.pf, self.pg, _p(m["lam"]), _p(m["mu"]), _p(m["qp"]), _p(m["coeff"]),
+            _p(m["qs"]), _p(m["dcrjx"]), _p(m["dcrjy"]), _p(m["dcrjz"]), _p(m["vx1"]),
+            _p(m["vx2"]), _p(m["ww"]), _p(m["wwo"]), d.nxt, d.nyt, d.nzt, self.DH, self.DT,
+            s_i, e_i, s_j, e_j)
+
+
+NGSL_ORACLE = 8
+
+
 class OracleSolver(PmclStepOrder):
     """medium: one Medium or a per-grid list (grid 0 on top); sources and
-    plastic: one entry (grid 0) or per-grid lists; receivers on grid 0."""
+    plastic: one entry (grid 0) or per-grid lists; receivers on grid 0;
+    topography: curvilinear grid 0 (one grid, no plasticity)."""
 
     def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
-                 dtype=np.float32, variant: str = "dstrqc", NPC: int = 0, plastic=None):
+                 dtype=np.float32, variant: str = "dstrqc", NPC: int = 0, plastic=None,
+                 topography=None):
         self.lib = load()
         self.dtype = np.dtype(dtype)
         self.suf = "f32" if self.dtype == np.float32 else "f64"
@@ -141,6 +209,12 @@ class OracleSolver(PmclStepOrder):
                             out=np.zeros((max(nt_record, 1), len(receivers.xy), 3),
                                          dtype=self.dtype))
         self.cur_step = 0
+        self.topo = None
+        if topography is not None:
+            if G != 1 or plastic is not None:
+                raise ValueError("topography runs one grid without plasticity")
+            L0 = self.levels[0]
+            self.topo = TopoHost(self.lib, topography, L0.dims, L0.DH, L0.DT, self.dtype)
 
     @property
     def dims(self):
@@ -166,6 +240,9 @@ class OracleSolver(PmclStepOrder):
         return s["nst"] if s is not None else 0
 
     def velocity(self, p: int = 0) -> None:
+        if p == 0 and self.topo is not None:
+            self.topo.velocity(self.levels[0].f, self.levels[0].m)
+            return
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         s_i, e_i = d.i_range
@@ -200,6 +277,9 @@ class OracleSolver(PmclStepOrder):
         self.stress_region(p, s_i, e_i, s_j, e_j)
 
     def stress_region(self, p, s_i, e_i, s_j, e_j) -> None:
+        if p == 0 and self.topo is not None:
+            self.topo.stress(self.levels[0].f, self.levels[0].m)
+            return
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:
@@ -286,9 +366,10 @@ class OracleSolver16(OracleSolver):
     on the GPU, so the sweep reads the stored ghosts."""
 
     def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
-                 storage: str = "bf16", check_every: int = 8):
+                 storage: str = "bf16", check_every: int = 8, topography=None):
         from awp_amd.storage import FIELDS16, FORMATS, initial_scales
-        super().__init__(medium, sources, receivers, nt_record, dtype=np.float32)
+        super().__init__(medium, sources, receivers, nt_record, dtype=np.float32,
+                         topography=topography)
         self.storage = storage
         self.fmt = FORMATS[storage]
         self.check_every = int(check_every)
@@ -343,6 +424,11 @@ class OracleSolver16(OracleSolver):
         self._enc(("u1", "v1", "w1"))
 
     def stress_region(self, p, s_i, e_i, s_j, e_j):
+        if p == 0 and self.topo is not None:
+            self._dec(self.fields16)
+            self.topo.stress(self.levels[0].f, self.levels[0].m)
+            self._enc(("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4", "r5", "r6"))
+            return
         if s_i > e_i or s_j > e_j:
             return
         L = self.levels[p]

@@ -1,0 +1,139 @@
+"""Curvilinear topography on the GPU, against the oracle (bit for bit):
+the metric builders, one call of each launcher on random fields over a hilly
+surface, whole runs (FP32 and 16-bit storage) and the launchers' error
+paths."""
+import numpy as np
+import pytest
+
+from awp_amd import configs
+from awp_amd.topography import F_NAMES, G_NAMES, Topography
+
+NX, NY, NZ = 40, 36, 32
+
+
+def _workload(nt=40):
+    w = configs.topography_hills(nx=NX, ny=NY, nz=NZ, nt=nt, DH=100.0, DT=0.005, nhills=4,
+                                 hmax=600.0, wmin=700.0, wmax=1800.0, nrec=8)
+    return w
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from awp_amd import _lib
+    return torch, _lib
+
+
+def test_neighbour_ranks_are_rejected():
+    """The halo-strip launchers have nothing to do without neighbours and
+    refuse a neighbour (no GPU call is made)."""
+    import ctypes as C
+    from awp_amd import _lib
+    from awp_amd.topography import AwpTopo
+    lib = _lib.load()
+    T = AwpTopo()
+    T.use = 1
+    T.x_rank_l = T.x_rank_r = T.y_rank_f = T.y_rank_b = -1
+    lib.awp_clear_error()
+    for fn in (lib.topo_velocity_front_H, lib.topo_velocity_back_H, lib.topo_stress_left_H,
+               lib.topo_stress_right_H):
+        fn(C.byref(T))
+    assert lib.awp_last_error() == 0
+    T.y_rank_f = 3
+    lib.topo_velocity_front_H(C.byref(T))
+    assert lib.awp_last_error() == 1
+    lib.awp_clear_error()
+    T.use = 1
+    lib.topo_stress_interior_H(C.byref(T))  # no constants, no arrays
+    assert lib.awp_last_error() != 0
+    lib.awp_clear_error()
+
+
+@pytest.mark.gpu
+def test_gpu_metrics_match_oracle(gpu, oracle_lib):
+    from awp_amd.solver import Solver
+    from oracle.oracle import OracleSolver
+    w = _workload()
+    med = w.medium()
+    topo = w.params["topography"]
+    g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    got = g.topo.metrics()
+    for n in F_NAMES:
+        assert np.array_equal(got[n].view(np.uint32), o.topo.mf[n].view(np.uint32)), n
+    for n in G_NAMES:
+        assert np.array_equal(got[n].view(np.uint32), o.topo.mg[n].view(np.uint32)), n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("which", ["velocity", "stress"])
+def test_gpu_launcher_one_call_bit_exact(gpu, oracle_lib, which):
+    """Random O(1) fields (memory variables too) over the hilly surface."""
+    torch, _lib = gpu
+    from awp_amd.solver import FIELDS, Solver
+    from oracle.oracle import OracleSolver
+    w = _workload()
+    med = w.medium()
+    topo = w.params["topography"]
+    g = Solver(med, None, None, topography=topo)
+    o = OracleSolver(med, None, None, topography=topo)
+    rng = np.random.default_rng(21)
+    for n in FIELDS:
+        a = rng.standard_normal(o.f[n].shape).astype(np.float32)
+        o.f[n][...] = a
+        g.f[n].copy_(torch.from_numpy(a))
+    if which == "velocity":
+        g.velocity(0)
+        o.velocity(0)
+    else:
+        d = g.dims
+        g.stress_region(0, *d.i_range, *d.j_range)
+        o.stress_region(0, *d.i_range, *d.j_range)
+    _lib.check()
+    for n in FIELDS:
+        a, b = g.field(n), o.f[n]
+        bad = np.argwhere(a.view(np.uint32) != b.view(np.uint32))
+        assert len(bad) == 0 or np.array_equal(a[tuple(bad.T)], b[tuple(bad.T)]), \
+            (n, len(bad), tuple(bad[0]), a[tuple(bad[0])], b[tuple(bad[0])])
+
+
+@pytest.mark.gpu
+def test_gpu_topography_run_bit_exact(gpu, oracle_lib):
+    """40 steps of the pmcl3d order with the curvilinear updates on grid 0
+    (source, free-surface pass, receivers): wavefield and traces."""
+    torch, _lib = gpu
+    from awp_amd.solver import FIELDS, Solver
+    from oracle.oracle import OracleSolver
+    w = _workload()
+    med = w.medium()
+    topo = w.params["topography"]
+    g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    g.run(w.nt, check_every=10)
+    o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    o.run(w.nt)
+    assert np.array_equal(g.traces(), o.traces())
+    assert np.abs(o.traces()).max() > 0
+    for n in FIELDS:
+        assert np.array_equal(g.field(n), o.f[n]), n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["bf16", "fp16"])
+def test_gpu_topography_16bit_bit_exact(gpu, oracle_lib, storage):
+    from awp_amd.solver import Solver16
+    from awp_amd.storage import FIELDS16
+    from oracle.oracle import OracleSolver16
+    w = _workload(nt=30)
+    med = w.medium()
+    topo = w.params["topography"]
+    g = Solver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage, topography=topo)
+    g.run(w.nt, check_every=10)
+    o = OracleSolver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage,
+                       topography=topo)
+    o.run(w.nt)
+    assert g.scales == o.scales and g.rescales == o.rescales
+    assert np.array_equal(g.traces(), o.traces())
+    for n in FIELDS16:
+        assert np.array_equal(g.raw(n), o.levels[0].q[n]), n
