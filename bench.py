@@ -52,9 +52,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps with no event timing (for rocprofv3)")
-    ap.add_argument("--workload", choices=("anelastic", "dm", "adjoint"), default="anelastic",
+    ap.add_argument("--workload", choices=("anelastic", "dm", "adjoint", "topo"),
+                    default="anelastic",
                     help="anelastic: the headline single-grid workload; dm: discontinuous mesh "
-                         "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz)")
+                         "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz); "
+                         "adjoint: gradient cost and finite-difference check; topo: curvilinear "
+                         "topography (384 x 384 x 192 unless --nx/--ny/--nz say otherwise)")
+    ap.add_argument("--save-traces", default="",
+                    help="write the receiver traces (numpy .npy) of the timed run here")
     ap.add_argument("--nz-fine", type=int, default=96)
     ap.add_argument("--storage", choices=("f32", "bf16", "fp16"), default="f32",
                     help="wavefield storage of the anelastic workload (16-bit: per-field "
@@ -131,6 +136,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "topo":
+        bench_topo(args)
         return
 
     def workload(nx, ny):
@@ -345,6 +353,104 @@ def bench_adjoint(args) -> None:
         "forward_cell_updates_per_s": cells * nt / t_fwd, "misfit": J, "fd_check": fd,
         "fd_max_rel_err": max(x["rel_err"] for x in fd) if fd else None,
         "setup_s": setup_s, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def bench_topo(args) -> None:
+    """Curvilinear topography: the pmcl3d step with the topo_* velocity and
+    stress launchers on grid 0 (source, free-surface pass, receivers), HIP
+    events around each launcher on the launch stream.  FP32 or 16-bit
+    storage.  Algorithmic bytes per cell as for the Cartesian kernels (the
+    metrics are 2-D and 1-D arrays)."""
+    import numpy as np
+    import torch
+
+    from awp_amd import _lib, configs
+    from awp_amd.solver import Solver, Solver16
+
+    nx = args.nx if args.nx != 512 else 384
+    ny = args.ny if args.ny != 512 else 384
+    nz = args.nz if args.nz != 256 else 192
+    nt = max(args.steps + args.warmup, 1)
+    w = configs.topography_hills(nx=nx, ny=ny, nz=nz, nt=nt)
+    topo = w.params["topography"]
+    t0 = time.perf_counter()
+    med = w.medium()
+    setup_s = time.perf_counter() - t0
+    if args.storage == "f32":
+        sol = Solver(med, w.sources, w.receivers, nt_record=nt + 1, topography=topo)
+        b_vel, b_str = BYTES_VEL, BYTES_STR
+    else:
+        sol = Solver16(med, w.sources, w.receivers, nt_record=nt + 1, storage=args.storage,
+                       topography=topo)
+        b_vel, b_str = BYTES16_VEL, BYTES16_STR
+    check_every = getattr(sol, "check_every", 0)
+    del med
+    cells = w.dims.ncells
+    sol.start()
+    if args.profile_steps:
+        for _ in range(args.profile_steps):
+            sol.step()
+        torch.cuda.synchronize()
+        _lib.check()
+        print(json.dumps({"profile_steps": args.profile_steps, "workload": w.name}))
+        return
+    for _ in range(args.warmup):
+        sol.step()
+    torch.cuda.synchronize()
+    _lib.check()
+    stream = torch.cuda.current_stream()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in range(args.steps):
+        e = ev[n]
+        sol.cur_step += 1
+        e[0].record(stream)
+        sol.velocity()
+        e[1].record(stream)
+        sol.stress()
+        e[2].record(stream)
+        if sol.cur_step < sol.source_steps(0):
+            sol.add_source(0, sol.cur_step + 1)
+        sol.free_surface()
+        sol.record(sol.cur_step - 1)
+        if check_every and sol.cur_step % check_every == 0:
+            sol.check_scales()
+        e[3].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _lib.check()
+    vel_avg = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    str_avg = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    ms_per_step = wall * 1e3 / args.steps
+    str_gbs = b_str * cells / (str_avg * 1e-3) / 1e9
+    vel_gbs = b_vel * cells / (vel_avg * 1e-3) / 1e9
+    tr = sol.traces()
+    if args.save_traces:
+        np.save(args.save_traces, tr)
+    out = {
+        "metric": METRIC, "value": cells * args.steps / wall, "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.storage == "f32" else f"f32 arithmetic, {args.storage} storage",
+        "data": "synthetic",
+        "config": {"workload": w.name, "storage": args.storage, "nx": nx, "ny": ny, "nz": nz,
+                   "h_m": w.DH, "dt_s": w.DT,
+                   "topography": "6 seeded Gaussian hills, height <= 1.5 km, widths 2-8 km",
+                   "medium": "layered vs 1500-4500 m/s + 5% von Karman, Q",
+                   "step": "topo velocity + stress (Q) on grid 0, addsrc_H, fstr_H, receivers"},
+        "roofline": {"bound": "hbm", "kernel": "topo_stress_interior_H", "achieved": str_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_cell": b_str,
+                     "cells_per_launch": cells, "avg_launch_ms": str_avg},
+        "kernels": {"topo_velocity_interior_H": {"avg_ms": vel_avg, "GBps": vel_gbs,
+                                                 "frac": vel_gbs / HBM_PEAK_GBS},
+                    "topo_stress_interior_H": {"avg_ms": str_avg, "GBps": str_gbs,
+                                               "frac": str_gbs / HBM_PEAK_GBS}},
+        "traces_finite": bool(np.isfinite(tr).all()), "trace_absmax": float(np.abs(tr).max()),
+        "setup_s": setup_s, "cpu_baseline": None,
     }
     print(json.dumps(out))
 

@@ -84,3 +84,32 @@ def test_binding_arity_matches_header():
     from awp_amd import _lib
     for name, n in _header_arg_counts().items():
         assert len(_lib.SIGNATURES[name]) == n, (name, len(_lib.SIGNATURES[name]), n)
+
+
+def test_topo_struct_layout_matches_c():
+    """awp_topo_t: the ctypes mirror's size and member offsets equal the C
+    compiler's (the header compiled on the host with g++)."""
+    import shutil
+    import subprocess
+    import tempfile
+    import ctypes as C
+    from awp_amd.topography import AwpTopo
+    gxx = shutil.which("g++")
+    inc = Path("/opt/rocm/include")
+    if gxx is None or not (inc / "hip" / "hip_runtime_api.h").exists():
+        import pytest
+        pytest.skip("no host compiler or HIP headers")
+    src = ("#define __HIP_PLATFORM_AMD__\n#include \"awp_hip.h\"\n#include <cstdio>\n"
+           "#include <cstddef>\nint main(){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(awp_topo_t),"
+           " offsetof(awp_topo_t, rho), offsetof(awp_topo_t, metrics_f),"
+           " offsetof(awp_topo_t, stream_i), offsetof(awp_topo_t, constants_set));}\n")
+    with tempfile.TemporaryDirectory() as td:
+        cpp, exe = Path(td) / "t.cpp", Path(td) / "t"
+        cpp.write_text(src)
+        subprocess.run([gxx, f"-I{ROOT / 'include'}", f"-I{inc}", str(cpp), "-o", str(exe)],
+                       check=True, capture_output=True)
+        got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                              text=True).stdout.split()]
+    want = [C.sizeof(AwpTopo), AwpTopo.rho.offset, AwpTopo.metrics_f.offset,
+            AwpTopo.stream_i.offset, AwpTopo.constants_set.offset]
+    assert got == want
