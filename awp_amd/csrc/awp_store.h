@@ -24,30 +24,24 @@ struct WaveScales {
   float inv[AWP_NSCALE];  // encode: q = rne(x * inv)
 };
 
-// binary32 -> bfloat16, round to nearest even; NaN stays a quiet NaN.
-__host__ __device__ __forceinline__ uint16_t awp_f32_to_bf16(float x) {
-  uint32_t u = __builtin_bit_cast(uint32_t, x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x0040u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
 __host__ __device__ __forceinline__ float awp_bf16_to_f32(uint16_t q) {
   return __builtin_bit_cast(float, (uint32_t)q << 16);
 }
 
-// The empty asm keeps the value in a register between the binary32 multiply
-// by the scale and the conversion: fused, the pair becomes v_fma_mix*,
-// which follows the binary32 denormal mode (flush) and so flushes binary16
-// subnormals that v_cvt_f16_f32 / v_cvt_f32_f16 keep.
+// binary32 <-> binary16 with the hardware conversions (round to nearest even;
+// binary16 subnormals kept, as the binary16 denormal mode is on), each issued
+// as an instruction of its own.  Left to the compiler, a conversion fuses
+// with the neighbouring scale multiply into v_fma_mix*, which follows the
+// binary32 denormal mode (flush) and so loses binary16 subnormals.
 __device__ __forceinline__ uint16_t awp_f32_to_f16(float x) {
-  asm volatile("" : "+v"(x));
-  return __builtin_bit_cast(uint16_t, (_Float16)x);  // v_cvt_f16_f32, round to nearest even
+  uint32_t r;
+  asm("v_cvt_f16_f32 %0, %1" : "=v"(r) : "v"(x));
+  return (uint16_t)r;
 }
 
 __device__ __forceinline__ float awp_f16_to_f32(uint16_t q) {
-  float x = (float)__builtin_bit_cast(_Float16, q);
-  asm volatile("" : "+v"(x));
+  float x;
+  asm("v_cvt_f32_f16 %0, %1" : "=v"(x) : "v"((uint32_t)q));
   return x;
 }
 
@@ -97,10 +91,15 @@ struct St16 {
   }
 };
 
+// binary32 -> bfloat16 is gfx950's v_cvt_pk_bf16_f32: round to nearest even,
+// overflow to infinity, NaN stays NaN.  Encoder inputs are products x * inv
+// computed with binary32 denormals flushed, so they are never subnormal.
 struct CodecBF16 {
   static constexpr int fmt = AWP_FMT_BF16;
   __device__ static __forceinline__ float to_f32(uint16_t q) { return awp_bf16_to_f32(q); }
-  __device__ static __forceinline__ uint16_t from_f32(float x) { return awp_f32_to_bf16(x); }
+  __device__ static __forceinline__ uint16_t from_f32(float x) {
+    return __builtin_bit_cast(uint16_t, (__bf16)x);
+  }
 };
 struct CodecF16 {
   static constexpr int fmt = AWP_FMT_F16;

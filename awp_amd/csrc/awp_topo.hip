@@ -22,6 +22,7 @@
 #include "awp_common.h"
 #include "awp_store.h"
 #include "awp_topo_coef.h"
+#include "awp_tuning.h"
 #include "../../include/awp_hip.h"
 
 namespace {
@@ -87,231 +88,276 @@ __device__ __forceinline__ float dbl_quot(float num, float den) {
 }
 
 // ---------------------------------------------------------------------------
-// Velocity, interior levels: dtopo_vel_111_unroll (velocity_unroll.cu:1-461).
+// Velocity: dtopo_vel_111_unroll (velocity_unroll.cu:1-461) on levels
+// 3 <= k < nz-6 and dtopo_vel_112 (optimized_velocity.cu:1040-1627) on the top
+// six, in one kernel.
+//
+// A wave owns one (j, 64-level) column segment and marches along i.  At every
+// plane each lane forms, at its own level, the six horizontal 4-point
+// interpolants whose z derivatives the metric terms need, and publishes them
+// (with its own s13, s23, s33) in LDS; the 7-point (9-point on the top rows)
+// z sums then read the neighbouring levels from LDS instead of recomputing
+// 7 x 4 loads per term.  The x neighbours travel in register windows.  Lanes
+// 3..60 of a segment produce output, so segments start every 58 levels; the
+// last segment of a column ends at level nz-1 and also takes the one-sided top
+// rows (table row kk = nz-1-k), as a divergent branch of its wave.
+#define VT_BJ 4      // waves (j columns) per workgroup
+#define VT_NL 9      // LDS arrays per wave
+
+struct VelTiles {
+  int nint;   // interior segments (58 outputs each, from level 3)
+  int ktop0;  // first level of the top segment (nz - 64, may be negative)
+  int wtop0;  // first level the top segment writes
+  int ichunk; // planes per march
+};
+
 template <class ST>
-__global__ __launch_bounds__(256) void topo_vel_int_kernel(const TopoVelArgs<ST> A,
-                                                           const TGeom G) {
-  const int k = (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = G.bj + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = G.bi + (int)blockIdx.z;
-  if (k < 3 || k >= G.nz - 6 || j >= G.ej || i >= G.ei) return;
+__global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<ST> A,
+                                                              const TGeom G, const VelTiles V) {
+  __shared__ float lds[2][VT_BJ][VT_NL][64];
+  const int lane = (int)threadIdx.x;
+  const int w = (int)threadIdx.y;
+  const int t = (int)blockIdx.x;
+  const bool top = t == V.nint;
+  const int k0 = top ? V.ktop0 : 58 * t;
+  const int k = k0 + lane;
+  const int nz = G.nz;
+  // levels this lane writes
+  const int w0 = top ? V.wtop0 : 58 * t + 3;
+  const int w1 = top ? nz : (58 * t + 61 < V.wtop0 ? 58 * t + 61 : V.wtop0);
+  bool out = k >= w0 && k < w1;
+  const bool bnd = k >= nz - 6;  // one-sided rows (top segment only)
+  const int kk = nz - 1 - k;
+  // every wave of the workgroup runs the same planes (uniform barrier count);
+  // a wave past the last column works on a copy of it and stores nothing
+  const int jw = G.bj + (int)(blockIdx.y * VT_BJ) + w;
+  const bool jok = jw < G.ej;
+  const int j = __builtin_amdgcn_readfirstlane(jok ? jw : G.ej - 1);
+  const int ib = (int)blockIdx.z * V.ichunk;
+  const int ie = ib + V.ichunk < G.ei ? ib + V.ichunk : G.ei;
+  if (ib >= ie) return;  // uniform over the workgroup
+  out = out && jok;
   const int sl = G.sl, yl = G.yl, fr = G.frow;
-  const int p = (k + AWP_ALIGN) + yl * (j + AWP_NGSL + 2) + sl * (i + AWP_NGSL + 2);
-  const int q = (j + AWP_ALIGN + AWP_NGSL + 2) + (i + AWP_NGSL + 2) * fr;
-  const int kg = k + AWP_ALIGN;
+  // every lane reads at its own level; the top segment of a grid with
+  // nz < 64 starts below level 0, and those lanes (never outputs, never read
+  // by an output's stencil) read level 0 instead
+  const int kr = k < 0 ? 0 : k;
+  const int kg = kr + AWP_ALIGN;
+  const int col0 = yl * (j + AWP_NGSL + 2) + AWP_ALIGN;   // level 0 of column (i = -10)
   const TMetrics& M = A.M;
-  const float* rho = A.rho;
 
-  const float c = 0.25f;
-  const float rho1 = c * (rho[p - 1] + rho[p - yl - 1]) + c * (rho[p] + rho[p - yl]);
-  const float rho2 = c * (rho[p - 1] + rho[p - sl - 1]) + c * (rho[p] + rho[p - sl]);
-  const float rho3 = c * (rho[p] + rho[p - sl]) + c * (rho[p - yl] + rho[p - sl - yl]);
-  float Ai1 = M.f_1[q] * M.g3_c[kg] * rho1;
-  Ai1 = dbl_quot(A.nu, Ai1);
-  float Ai2 = M.f_2[q] * M.g3_c[kg] * rho2;
-  Ai2 = dbl_quot(A.nu, Ai2);
-  float Ai3 = M.f_c[q] * M.g3[kg] * rho3;
-  Ai3 = dbl_quot(A.nu, Ai3);
-  const float f_dcrj =
-      A.dcrjx[i + AWP_NGSL + 2] * A.dcrjy[j + AWP_NGSL + 2] * A.dcrjz[kg];
-
-  // z derivatives of horizontal 4-point interpolants over k-3..k+3, terms in
-  // the stencil order 3, 0, 1, 2, 4, 5, 6 (dhpz4 = dphz4: [3] is +0).
   constexpr float dhpz4[7] = {(float)TV_DHPZ4[0], (float)TV_DHPZ4[1], (float)TV_DHPZ4[2],
                               (float)TV_DHPZ4[3], (float)TV_DHPZ4[4], (float)TV_DHPZ4[5],
                               (float)TV_DHPZ4[6]};
   constexpr float dphz4[7] = {(float)TV_DPHZ4[0], (float)TV_DPHZ4[1], (float)TV_DPHZ4[2],
                               (float)TV_DPHZ4[3], (float)TV_DPHZ4[4], (float)TV_DPHZ4[5],
                               (float)TV_DPHZ4[6]};
-  float z11 = 0.f, z12 = 0.f, z12x = 0.f, z22 = 0.f, z13 = 0.f, z23 = 0.f;
+  // per-level weights of the z sums (interior rows); the top rows form theirs
+  // from the tables inside their branch
+  float cz[7], czp[7];
 #pragma unroll
-  for (int t = 0; t < 7; ++t) {
-    const int m = t == 0 ? 3 : (t <= 3 ? t - 1 : t);
-    const int o = p + m - 3, kk = kg + m - 3;
-    const float cz = dhpz4[m] * M.g_c[kk];
-    const float czp = dphz4[m] * M.g[kk];
-    const float t11 = cz * (P4F(2) * A.s11[o] + P4F(0) * A.s11[o - 2 * sl] +
-                            P4F(1) * A.s11[o - sl] + P4F(3) * A.s11[o + sl]);
-    const float t12 = cz * (P4F(2) * A.s12[o] + P4F(0) * A.s12[o - 2 * yl] +
-                            P4F(1) * A.s12[o - yl] + P4F(3) * A.s12[o + yl]);
-    const float t12x = cz * (P4F(1) * A.s12[o] + P4F(0) * A.s12[o - sl] +
-                             P4F(2) * A.s12[o + sl] + P4F(3) * A.s12[o + 2 * sl]);
-    const float t22 = cz * (P4F(1) * A.s22[o] + P4F(0) * A.s22[o - yl] +
-                            P4F(2) * A.s22[o + yl] + P4F(3) * A.s22[o + 2 * yl]);
-    const float t13 = czp * (P4F(1) * A.s13[o] + P4F(0) * A.s13[o - sl] +
-                             P4F(2) * A.s13[o + sl] + P4F(3) * A.s13[o + 2 * sl]);
-    const float t23 = czp * (P4F(2) * A.s23[o] + P4F(0) * A.s23[o - 2 * yl] +
-                             P4F(1) * A.s23[o - yl] + P4F(3) * A.s23[o + yl]);
-    if (t == 0) {
-      z11 = t11; z12 = t12; z12x = t12x; z22 = t22; z13 = t13; z23 = t23;
-    } else {
-      z11 = z11 + t11; z12 = z12 + t12; z12x = z12x + t12x; z22 = z22 + t22;
-      z13 = z13 + t13; z23 = z23 + t23;
-    }
+  for (int m = 0; m < 7; ++m) {
+    const int kk2 = kg + m - 3;  // >= align - 3
+    cz[m] = dhpz4[m] * M.g_c[kk2];
+    czp[m] = dphz4[m] * M.g[kk2];
   }
-  const float g3c = M.g3_c[kg], g3 = M.g3[kg];
-  float s;
-  // u1
-  s = D4F(2) * M.f_c[q] * g3c * A.s11[p];
-  s = s + D4F(0) * M.f_c[q - 2 * fr] * g3c * A.s11[p - 2 * sl];
-  s = s + D4F(1) * M.f_c[q - fr] * g3c * A.s11[p - sl];
-  s = s + D4F(3) * M.f_c[q + fr] * g3c * A.s11[p + sl];
-  s = s + D4F(2) * M.f[q] * g3c * A.s12[p];
-  s = s + D4F(0) * M.f[q - 2] * g3c * A.s12[p - 2 * yl];
-  s = s + D4F(1) * M.f[q - 1] * g3c * A.s12[p - yl];
-  s = s + D4F(3) * M.f[q + 1] * g3c * A.s12[p + yl];
-  s = s + D4F(2) * A.s13[p];
-  s = s + D4F(0) * A.s13[p - 2];
-  s = s + D4F(1) * A.s13[p - 1];
-  s = s + D4F(3) * A.s13[p + 1];
-  s = s - M.f1_1[q] * z11;
-  s = s - M.f2_1[q] * z12;
-  const float v1 = (A.a * ST::ld(A.u1, p, A.su1) + Ai1 * s) * f_dcrj;
-  // u2
-  s = D4F(2) * A.s23[p];
-  s = s + D4F(0) * A.s23[p - 2];
-  s = s + D4F(1) * A.s23[p - 1];
-  s = s + D4F(3) * A.s23[p + 1];
-  s = s + D4F(1) * M.f[q] * g3c * A.s12[p];
-  s = s + D4F(0) * M.f[q - fr] * g3c * A.s12[p - sl];
-  s = s + D4F(2) * M.f[q + fr] * g3c * A.s12[p + sl];
-  s = s + D4F(3) * M.f[q + 2 * fr] * g3c * A.s12[p + 2 * sl];
-  s = s + D4F(1) * M.f_c[q] * g3c * A.s22[p];
-  s = s + D4F(0) * M.f_c[q - 1] * g3c * A.s22[p - yl];
-  s = s + D4F(2) * M.f_c[q + 1] * g3c * A.s22[p + yl];
-  s = s + D4F(3) * M.f_c[q + 2] * g3c * A.s22[p + 2 * yl];
-  s = s - M.f1_2[q] * z12x;
-  s = s - M.f2_2[q] * z22;
-  const float v2 = (A.a * ST::ld(A.u2, p, A.su2) + Ai2 * s) * f_dcrj;
-  // u3
-  s = D4F(2) * M.f_2[q] * g3 * A.s23[p];
-  s = s + D4F(0) * M.f_2[q - 2] * g3 * A.s23[p - 2 * yl];
-  s = s + D4F(1) * M.f_2[q - 1] * g3 * A.s23[p - yl];
-  s = s + D4F(3) * M.f_2[q + 1] * g3 * A.s23[p + yl];
-  s = s + D4F(1) * M.f_1[q] * g3 * A.s13[p];
-  s = s + D4F(0) * M.f_1[q - fr] * g3 * A.s13[p - sl];
-  s = s + D4F(2) * M.f_1[q + fr] * g3 * A.s13[p + sl];
-  s = s + D4F(3) * M.f_1[q + 2 * fr] * g3 * A.s13[p + 2 * sl];
-  s = s + D4F(1) * A.s33[p];
-  s = s + D4F(0) * A.s33[p - 1];
-  s = s + D4F(2) * A.s33[p + 1];
-  s = s + D4F(3) * A.s33[p + 2];
-  s = s - M.f1_c[q] * z13;
-  s = s - M.f2_c[q] * z23;
-  const float v3 = (A.a * ST::ld(A.u3, p, A.su3) + Ai3 * s) * f_dcrj;
-  ST::st(A.u1, p, v1, A.iu1);
-  ST::st(A.u2, p, v2, A.iu2);
-  ST::st(A.u3, p, v3, A.iu3);
-}
+  const float g3c = M.g3_c[kg], g3v = M.g3[kg];
+  const float dcz = A.dcrjz[kg];
+  const float dcy = A.dcrjy[j + AWP_NGSL + 2];
 
-// Velocity, top six levels k = nz-1-kk: dtopo_vel_112
-// (optimized_velocity.cu:1040-1627); one lane per (kk, j).
-template <class ST>
-__global__ __launch_bounds__(192) void topo_vel_top_kernel(const TopoVelArgs<ST> A,
-                                                           const TGeom G) {
-  const int kk = (int)threadIdx.x;  // 0..5
-  const int j = G.bj + (int)(blockIdx.y * 32 + threadIdx.y);
-  const int i = G.bi + (int)blockIdx.z;
-  if (kk >= 6 || j >= G.ej || i >= G.ei) return;
-  const int nz = G.nz;
-  const int sl = G.sl, yl = G.yl, fr = G.frow;
-  const int k = nz - 1 - kk;
-  const int col = yl * (j + AWP_NGSL + 2) + sl * (i + AWP_NGSL + 2) + AWP_ALIGN;  // level 0
-  const int p = col + k;
-  const int q = (j + AWP_ALIGN + AWP_NGSL + 2) + (i + AWP_NGSL + 2) * fr;
-  const int kg = k + AWP_ALIGN;
-  const TMetrics& M = A.M;
-  const float* rho = A.rho;
-  const int kb = col + nz - kk - 2;
-
-  const float rho1 = (float)(0.25 * (double)(rho[kb] + rho[kb - yl]) +
-                             0.25 * (double)(rho[kb + 1] + rho[kb + 1 - yl]));
-  const float rho2 = (float)(0.25 * (double)(rho[kb] + rho[kb - sl]) +
-                             0.25 * (double)(rho[kb + 1] + rho[kb + 1 - sl]));
-  const float rho3 = (float)(0.25 * (double)(rho[kb + 1] + rho[kb + 1 - sl]) +
-                             0.25 * (double)(rho[kb + 1 - yl] + rho[kb + 1 - sl - yl]));
-  float Ai1 = M.f_1[q] * M.g3_c[kg] * rho1;
-  Ai1 = dbl_quot(A.nu, Ai1);
-  float Ai2 = M.f_2[q] * M.g3_c[kg] * rho2;
-  Ai2 = dbl_quot(A.nu, Ai2);
-  float Ai3 = M.f_c[q] * M.g3[kg] * rho3;
-  Ai3 = dbl_quot(A.nu, Ai3);
-  const float f_dcrj =
-      A.dcrjx[i + AWP_NGSL + 2] * A.dcrjy[j + AWP_NGSL + 2] * A.dcrjz[kg];
-
-  float z11 = 0.f, z12 = 0.f, z12x = 0.f, z22 = 0.f, z13 = 0.f, z23 = 0.f;
+  // x windows at level k: s11(i-2..i+1), s12(i-1..i+2), s13(i-1..i+2)
+  auto P = [&](int i) { return col0 + sl * (i + AWP_NGSL + 2) + kr; };
+  float s11w[4], s12x[4], s13x[4];
 #pragma unroll
-  for (int m = 8; m >= 0; --m) {
-    const int o = col + nz - 1 - m, lg = AWP_ALIGN + nz - 1 - m;
-    const float cz = c_vel_dhpz4r.v[kk][m] * M.g_c[lg];
-    const float czp = c_vel_dphz4r.v[kk][m] * M.g[lg];
-    const float t11 = cz * (P4F(2) * A.s11[o] + P4F(0) * A.s11[o - 2 * sl] +
-                            P4F(1) * A.s11[o - sl] + P4F(3) * A.s11[o + sl]);
-    const float t12 = cz * (P4F(2) * A.s12[o] + P4F(0) * A.s12[o - 2 * yl] +
-                            P4F(1) * A.s12[o - yl] + P4F(3) * A.s12[o + yl]);
-    const float t12x = cz * (P4F(1) * A.s12[o] + P4F(0) * A.s12[o - sl] +
-                             P4F(2) * A.s12[o + sl] + P4F(3) * A.s12[o + 2 * sl]);
-    const float t22 = cz * (P4F(1) * A.s22[o] + P4F(0) * A.s22[o - yl] +
-                            P4F(2) * A.s22[o + yl] + P4F(3) * A.s22[o + 2 * yl]);
-    const float t13 = czp * (P4F(1) * A.s13[o] + P4F(0) * A.s13[o - sl] +
-                             P4F(2) * A.s13[o + sl] + P4F(3) * A.s13[o + 2 * sl]);
-    const float t23 = czp * (P4F(2) * A.s23[o] + P4F(0) * A.s23[o - 2 * yl] +
-                             P4F(1) * A.s23[o - yl] + P4F(3) * A.s23[o + yl]);
-    if (m == 8) {
-      z11 = t11; z12 = t12; z12x = t12x; z22 = t22; z13 = t13; z23 = t23;
-    } else {
-      z11 = z11 + t11; z12 = z12 + t12; z12x = z12x + t12x; z22 = z22 + t22;
-      z13 = z13 + t13; z23 = z23 + t23;
-    }
+  for (int q = 0; q < 3; ++q) {
+    s11w[q + 1] = A.s11[P(ib - 2 + q)];
+    s12x[q + 1] = A.s12[P(ib - 1 + q)];
+    s13x[q + 1] = A.s13[P(ib - 1 + q)];
   }
-  const float g3c = M.g3_c[kg], g3 = M.g3[kg];
-  float s;
-  // u1
-  s = D4F(2) * M.f_c[q] * g3c * A.s11[p];
-  s = s + D4F(0) * M.f_c[q - 2 * fr] * g3c * A.s11[p - 2 * sl];
-  s = s + D4F(1) * M.f_c[q - fr] * g3c * A.s11[p - sl];
-  s = s + D4F(3) * M.f_c[q + fr] * g3c * A.s11[p + sl];
-  s = s + D4F(2) * M.f[q] * g3c * A.s12[p];
-  s = s + D4F(0) * M.f[q - 2] * g3c * A.s12[p - 2 * yl];
-  s = s + D4F(1) * M.f[q - 1] * g3c * A.s12[p - yl];
-  s = s + D4F(3) * M.f[q + 1] * g3c * A.s12[p + yl];
+  // density at plane i-1: (j, k-1), (j, k), (j-1, k)
+  float rm_k1 = A.rho[P(ib - 1) - 1], rm_k = A.rho[P(ib - 1)], rm_j = A.rho[P(ib - 1) - yl];
+
+  for (int i = ib; i < ie; ++i) {
+    const int par = i & 1;
+    const int p = P(i);
 #pragma unroll
-  for (int m = 7; m >= 0; --m) s = s + c_vel_dhz4r.v[kk][m] * A.s13[col + nz - 1 - m];
-  s = s - M.f1_1[q] * z11;
-  s = s - M.f2_1[q] * z12;
-  const float v1 = (A.a * ST::ld(A.u1, p, A.su1) + Ai1 * s) * f_dcrj;
-  // u2: z terms first
-  s = c_vel_dhz4r.v[kk][7] * A.s23[col + nz - 8];
+    for (int q = 0; q < 3; ++q) {
+      s11w[q] = s11w[q + 1];
+      s12x[q] = s12x[q + 1];
+      s13x[q] = s13x[q + 1];
+    }
+    s11w[3] = A.s11[p + sl];
+    s12x[3] = A.s12[p + 2 * sl];
+    s13x[3] = A.s13[p + 2 * sl];
+    const float s12m2 = A.s12[p - 2 * yl], s12m1 = A.s12[p - yl], s12p1 = A.s12[p + yl];
+    const float s22m1 = A.s22[p - yl], s22o = A.s22[p], s22p1 = A.s22[p + yl],
+                s22p2 = A.s22[p + 2 * yl];
+    const float s23m2 = A.s23[p - 2 * yl], s23m1 = A.s23[p - yl], s23o = A.s23[p],
+                s23p1 = A.s23[p + yl];
+    const float s33o = A.s33[p];
+    const float s12o = s12x[1], s13o = s13x[1];
+    // interpolants at this level, operands in the reference's order
+    const float I11 = P4F(2) * s11w[2] + P4F(0) * s11w[0] + P4F(1) * s11w[1] + P4F(3) * s11w[3];
+    const float I12 = P4F(2) * s12o + P4F(0) * s12m2 + P4F(1) * s12m1 + P4F(3) * s12p1;
+    const float I12x = P4F(1) * s12x[1] + P4F(0) * s12x[0] + P4F(2) * s12x[2] + P4F(3) * s12x[3];
+    const float I22 = P4F(1) * s22o + P4F(0) * s22m1 + P4F(2) * s22p1 + P4F(3) * s22p2;
+    const float I13 = P4F(1) * s13x[1] + P4F(0) * s13x[0] + P4F(2) * s13x[2] + P4F(3) * s13x[3];
+    const float I23 = P4F(2) * s23o + P4F(0) * s23m2 + P4F(1) * s23m1 + P4F(3) * s23p1;
+    float(*L)[64] = lds[par][w];
+    L[0][lane] = I11;
+    L[1][lane] = I12;
+    L[2][lane] = I12x;
+    L[3][lane] = I22;
+    L[4][lane] = I13;
+    L[5][lane] = I23;
+    L[6][lane] = s13o;
+    L[7][lane] = s23o;
+    L[8][lane] = s33o;
+    // density at plane i
+    const float r_k1 = A.rho[p - 1], r_k = A.rho[p], r_j = A.rho[p - yl],
+                r_jk1 = A.rho[p - yl - 1];
+    __syncthreads();
+
+    if (out) {
+      const int q = (j + AWP_ALIGN + AWP_NGSL + 2) + (i + AWP_NGSL + 2) * fr;
+      float rho1, rho2, rho3;
+      if (!bnd) {
+        const float c = 0.25f;
+        rho1 = c * (r_k1 + r_jk1) + c * (r_k + r_j);
+        rho2 = c * (r_k1 + rm_k1) + c * (r_k + rm_k);
+        rho3 = c * (r_k + rm_k) + c * (r_j + rm_j);
+      } else {
+        rho1 = (float)(0.25 * (double)(r_k1 + r_jk1) + 0.25 * (double)(r_k + r_j));
+        rho2 = (float)(0.25 * (double)(r_k1 + rm_k1) + 0.25 * (double)(r_k + rm_k));
+        rho3 = (float)(0.25 * (double)(r_k + rm_k) + 0.25 * (double)(r_j + rm_j));
+      }
+      float Ai1 = M.f_1[q] * g3c * rho1;
+      Ai1 = dbl_quot(A.nu, Ai1);
+      float Ai2 = M.f_2[q] * g3c * rho2;
+      Ai2 = dbl_quot(A.nu, Ai2);
+      float Ai3 = M.f_c[q] * g3v * rho3;
+      Ai3 = dbl_quot(A.nu, Ai3);
+      const float f_dcrj = A.dcrjx[i + AWP_NGSL + 2] * dcy * dcz;
+      float z11, z12, z12x, z22, z13, z23;
+      float zs13, zs23, zs33;  // plain z derivatives of s13 (u1), s23 (u2), s33 (u3)
+      if (!bnd) {
+        // order 3, 0, 1, 2, 4, 5, 6
 #pragma unroll
-  for (int m = 6; m >= 0; --m) s = s + c_vel_dhz4r.v[kk][m] * A.s23[col + nz - 1 - m];
-  s = s + D4F(1) * M.f[q] * g3c * A.s12[p];
-  s = s + D4F(0) * M.f[q - fr] * g3c * A.s12[p - sl];
-  s = s + D4F(2) * M.f[q + fr] * g3c * A.s12[p + sl];
-  s = s + D4F(3) * M.f[q + 2 * fr] * g3c * A.s12[p + 2 * sl];
-  s = s + D4F(1) * M.f_c[q] * g3c * A.s22[p];
-  s = s + D4F(0) * M.f_c[q - 1] * g3c * A.s22[p - yl];
-  s = s + D4F(2) * M.f_c[q + 1] * g3c * A.s22[p + yl];
-  s = s + D4F(3) * M.f_c[q + 2] * g3c * A.s22[p + 2 * yl];
-  s = s - M.f1_2[q] * z12x;
-  s = s - M.f2_2[q] * z22;
-  const float v2 = (A.a * ST::ld(A.u2, p, A.su2) + Ai2 * s) * f_dcrj;
-  // u3
-  s = D4F(2) * M.f_2[q] * g3 * A.s23[p];
-  s = s + D4F(0) * M.f_2[q - 2] * g3 * A.s23[p - 2 * yl];
-  s = s + D4F(1) * M.f_2[q - 1] * g3 * A.s23[p - yl];
-  s = s + D4F(3) * M.f_2[q + 1] * g3 * A.s23[p + yl];
-  s = s + D4F(1) * M.f_1[q] * g3 * A.s13[p];
-  s = s + D4F(0) * M.f_1[q - fr] * g3 * A.s13[p - sl];
-  s = s + D4F(2) * M.f_1[q + fr] * g3 * A.s13[p + sl];
-  s = s + D4F(3) * M.f_1[q + 2 * fr] * g3 * A.s13[p + 2 * sl];
+        for (int tt = 0; tt < 7; ++tt) {
+          const int m = tt == 0 ? 3 : (tt <= 3 ? tt - 1 : tt);
+          const int o = lane + m - 3;
+          const float t11 = cz[m] * L[0][o], t12 = cz[m] * L[1][o];
+          const float t12x = cz[m] * L[2][o], t22 = cz[m] * L[3][o];
+          const float t13 = czp[m] * L[4][o], t23 = czp[m] * L[5][o];
+          if (tt == 0) {
+            z11 = t11; z12 = t12; z12x = t12x; z22 = t22; z13 = t13; z23 = t23;
+          } else {
+            z11 = z11 + t11; z12 = z12 + t12; z12x = z12x + t12x; z22 = z22 + t22;
+            z13 = z13 + t13; z23 = z23 + t23;
+          }
+        }
+        zs13 = 0.f;
+        zs23 = 0.f;
+        zs33 = 0.f;
+      } else {
+        // levels nz-9 .. nz-1 live in lanes 55 .. 63 of the top segment
+        const int lb = nz - 1 - k0;  // lane of level nz-1
 #pragma unroll
-  for (int m = 6; m >= 0; --m) s = s + c_vel_dz4r.v[kk][m] * A.s33[col + nz - 1 - m];
-  s = s - M.f1_c[q] * z13;
-  s = s - M.f2_c[q] * z23;
-  const float v3 = (A.a * ST::ld(A.u3, p, A.su3) + Ai3 * s) * f_dcrj;
-  ST::st(A.u1, p, v1, A.iu1);
-  ST::st(A.u2, p, v2, A.iu2);
-  ST::st(A.u3, p, v3, A.iu3);
+        for (int m = 8; m >= 0; --m) {
+          const int o = lb - m;
+          const float cr = c_vel_dhpz4r.v[kk][m] * M.g_c[AWP_ALIGN + nz - 1 - m];
+          const float crp = c_vel_dphz4r.v[kk][m] * M.g[AWP_ALIGN + nz - 1 - m];
+          const float t11 = cr * L[0][o], t12 = cr * L[1][o];
+          const float t12x = cr * L[2][o], t22 = cr * L[3][o];
+          const float t13 = crp * L[4][o], t23 = crp * L[5][o];
+          if (m == 8) {
+            z11 = t11; z12 = t12; z12x = t12x; z22 = t22; z13 = t13; z23 = t23;
+          } else {
+            z11 = z11 + t11; z12 = z12 + t12; z12x = z12x + t12x; z22 = z22 + t22;
+            z13 = z13 + t13; z23 = z23 + t23;
+          }
+        }
+        zs13 = zs23 = zs33 = 0.f;
+      }
+      float s;
+      // u1
+      s = D4F(2) * M.f_c[q] * g3c * s11w[2];
+      s = s + D4F(0) * M.f_c[q - 2 * fr] * g3c * s11w[0];
+      s = s + D4F(1) * M.f_c[q - fr] * g3c * s11w[1];
+      s = s + D4F(3) * M.f_c[q + fr] * g3c * s11w[3];
+      s = s + D4F(2) * M.f[q] * g3c * s12o;
+      s = s + D4F(0) * M.f[q - 2] * g3c * s12m2;
+      s = s + D4F(1) * M.f[q - 1] * g3c * s12m1;
+      s = s + D4F(3) * M.f[q + 1] * g3c * s12p1;
+      if (!bnd) {
+        s = s + D4F(2) * L[6][lane];
+        s = s + D4F(0) * L[6][lane - 2];
+        s = s + D4F(1) * L[6][lane - 1];
+        s = s + D4F(3) * L[6][lane + 1];
+      } else {
+        const int lb = nz - 1 - k0;
+#pragma unroll
+        for (int m = 7; m >= 0; --m) s = s + c_vel_dhz4r.v[kk][m] * L[6][lb - m];
+      }
+      s = s - M.f1_1[q] * z11;
+      s = s - M.f2_1[q] * z12;
+      const float v1 = (A.a * ST::ld(A.u1, p, A.su1) + Ai1 * s) * f_dcrj;
+      // u2
+      if (!bnd) {
+        s = D4F(2) * s23o;
+        s = s + D4F(0) * L[7][lane - 2];
+        s = s + D4F(1) * L[7][lane - 1];
+        s = s + D4F(3) * L[7][lane + 1];
+      } else {
+        const int lb = nz - 1 - k0;
+        s = c_vel_dhz4r.v[kk][7] * L[7][lb - 7];
+#pragma unroll
+        for (int m = 6; m >= 0; --m) s = s + c_vel_dhz4r.v[kk][m] * L[7][lb - m];
+      }
+      s = s + D4F(1) * M.f[q] * g3c * s12x[1];
+      s = s + D4F(0) * M.f[q - fr] * g3c * s12x[0];
+      s = s + D4F(2) * M.f[q + fr] * g3c * s12x[2];
+      s = s + D4F(3) * M.f[q + 2 * fr] * g3c * s12x[3];
+      s = s + D4F(1) * M.f_c[q] * g3c * s22o;
+      s = s + D4F(0) * M.f_c[q - 1] * g3c * s22m1;
+      s = s + D4F(2) * M.f_c[q + 1] * g3c * s22p1;
+      s = s + D4F(3) * M.f_c[q + 2] * g3c * s22p2;
+      s = s - M.f1_2[q] * z12x;
+      s = s - M.f2_2[q] * z22;
+      const float v2 = (A.a * ST::ld(A.u2, p, A.su2) + Ai2 * s) * f_dcrj;
+      // u3
+      s = D4F(2) * M.f_2[q] * g3v * s23o;
+      s = s + D4F(0) * M.f_2[q - 2] * g3v * s23m2;
+      s = s + D4F(1) * M.f_2[q - 1] * g3v * s23m1;
+      s = s + D4F(3) * M.f_2[q + 1] * g3v * s23p1;
+      s = s + D4F(1) * M.f_1[q] * g3v * s13x[1];
+      s = s + D4F(0) * M.f_1[q - fr] * g3v * s13x[0];
+      s = s + D4F(2) * M.f_1[q + fr] * g3v * s13x[2];
+      s = s + D4F(3) * M.f_1[q + 2 * fr] * g3v * s13x[3];
+      if (!bnd) {
+        s = s + D4F(1) * s33o;
+        s = s + D4F(0) * L[8][lane - 1];
+        s = s + D4F(2) * L[8][lane + 1];
+        s = s + D4F(3) * L[8][lane + 2];
+      } else {
+        const int lb = nz - 1 - k0;
+#pragma unroll
+        for (int m = 6; m >= 0; --m) s = s + c_vel_dz4r.v[kk][m] * L[8][lb - m];
+      }
+      s = s - M.f1_c[q] * z13;
+      s = s - M.f2_c[q] * z23;
+      const float v3 = (A.a * ST::ld(A.u3, p, A.su3) + Ai3 * s) * f_dcrj;
+      (void)zs13;
+      (void)zs23;
+      (void)zs33;
+      ST::st(A.u1, p, v1, A.iu1);
+      ST::st(A.u2, p, v2, A.iu2);
+      ST::st(A.u3, p, v3, A.iu3);
+    }
+    rm_k1 = r_k1;
+    rm_k = r_k;
+    rm_j = r_j;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -452,185 +498,259 @@ __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int po
   ST::st(A.yz, pos, (sv + DT * f_rtmp) * f_dcrj, W.inv[AWP_SYZ]);
 }
 
-// Interior stress levels align+3 <= k < align+nz-6 (array indices), one lane
-// per k; i, j are array indices in [s_i, e_i) x [s_j, e_j).
-template <class ST>
-__global__ __launch_bounds__(256) void topo_str_int_kernel(const TopoStrArgs<ST> A,
-                                                           const TGeom G) {
-  const int k = (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = G.bj + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = G.bi + (int)blockIdx.z;
-  if (k < AWP_ALIGN + 3 || k >= AWP_ALIGN + G.nz - 6 || j >= G.ej || i >= G.ei) return;
-  const int sl = G.sl, yl = G.yl, fr = G.frow;
-  const int pos = k + yl * j + sl * i;
-  const int q = j + AWP_ALIGN + i * fr;
-  const TMetrics& M = A.M;
+// Stress: dtopo_str_111 (stress_attenuation.cu:74-668) on levels
+// 3 <= k < nz-6 and dtopo_str_112 (:670-1410) on the top six, in one kernel
+// with the velocity kernel's lane map (wave = (j, 64-level segment), march
+// along i, 58 output levels per segment, the top rows in the last segment).
+//
+// The metric terms need 7-point (top rows: 9-point table) z sums of
+// velocity columns at x neighbours (i-2 .. i+2) and y neighbours (j-2 .. j+2).
+// The x-neighbour sums ride in register windows: one new column's sum per
+// field and plane.  The y-neighbour sums and the own column's z differences
+// read velocity columns that each plane stages in LDS (15 columns per wave).
+#define ST_BJ 4
+#define ST_NC 15
+enum {
+  C_UN, C_UO, C_UM1, C_UP1, C_UP2,   // u1 at (i+2, j), (i, j), (i, j-1), (i, j+1), (i, j+2)
+  C_VN, C_VO, C_VM2, C_VM1, C_VP1,   // v1 at (i+1, j), (i, j), (i, j-2), (i, j-1), (i, j+1)
+  C_WN, C_WO, C_WM1, C_WP1, C_WP2    // w1 at (i+1, j), (i, j), (i, j-1), (i, j+1), (i, j+2)
+};
 
-  // 7-point z sum of a column at offset o, stencil c (the stress tables'
-  // [3] is -0), terms in ascending order (ASC) or 3, 0, 1, 2, 4, 5, 6.
+template <class ST>
+__global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<ST> A,
+                                                              const TGeom G, const VelTiles V) {
+  __shared__ float lds[2][ST_BJ][ST_NC][64];
+  const int lane = (int)threadIdx.x;
+  const int w = (int)threadIdx.y;
+  const int t = (int)blockIdx.x;
+  const bool top = t == V.nint;
+  const int k0 = top ? V.ktop0 : 58 * t;
+  const int k = k0 + lane;  // topographic level
+  const int nz = G.nz;
+  const int w0 = top ? V.wtop0 : 58 * t + 3;
+  const int w1 = top ? nz : (58 * t + 61 < V.wtop0 ? 58 * t + 61 : V.wtop0);
+  const bool bnd = k >= nz - 6;
+  const int kb = nz - 1 - k;  // table row of a top level
+  const int jw = G.bj + (int)(blockIdx.y * ST_BJ) + w;
+  const bool jok = jw < G.ej;
+  const int j = __builtin_amdgcn_readfirstlane(jok ? jw : G.ej - 1);
+  const int ib = G.bi + (int)blockIdx.z * V.ichunk;
+  const int ie = ib + V.ichunk < G.ei ? ib + V.ichunk : G.ei;
+  if (ib >= ie) return;  // uniform over the workgroup
+  const bool out = k >= w0 && k < w1 && jok;
+  const int sl = G.sl, yl = G.yl, fr = G.frow;
+  const int kr = k < 0 ? 0 : k;          // read level (see topo_vel_kernel)
+  const int K = kr + AWP_ALIGN;          // array level
+  const TMetrics& M = A.M;
+  const int lb = nz - 1 - k0;            // lane of level nz-1 (top segment)
+
   constexpr float phdz4[7] = {(float)TS_PHDZ4[0], (float)TS_PHDZ4[1], (float)TS_PHDZ4[2],
                               (float)TS_PHDZ4[3], (float)TS_PHDZ4[4], (float)TS_PHDZ4[5],
                               (float)TS_PHDZ4[6]};
   constexpr float pdhz4[7] = {(float)TS_PDHZ4[0], (float)TS_PDHZ4[1], (float)TS_PDHZ4[2],
                               (float)TS_PDHZ4[3], (float)TS_PDHZ4[4], (float)TS_PDHZ4[5],
                               (float)TS_PDHZ4[6]};
-  auto zs_asc = [&](const TField<ST>& fld, int o) {
-    float acc = phdz4[0] * fld[o - 3];
+  // z sums of an LDS column at this lane's level
+  auto zasc = [&](const float* c) {  // phdz4, ascending (normal stresses)
+    float acc = phdz4[0] * c[lane - 3];
 #pragma unroll
-    for (int m = 1; m < 7; ++m) acc = acc + phdz4[m] * fld[o + m - 3];
+    for (int m = 1; m < 7; ++m) acc = acc + phdz4[m] * c[lane + m - 3];
     return acc;
   };
-  auto zs_mid = [&](const float (&c)[7], const TField<ST>& fld, int o) {
-    float acc = c[3] * fld[o];
-    acc = acc + c[0] * fld[o - 3];
-    acc = acc + c[1] * fld[o - 2];
-    acc = acc + c[2] * fld[o - 1];
-    acc = acc + c[4] * fld[o + 1];
-    acc = acc + c[5] * fld[o + 2];
-    acc = acc + c[6] * fld[o + 3];
+  auto zmid = [&](const float(&cf)[7], const float* c) {  // order 3, 0, 1, 2, 4, 5, 6
+    float acc = cf[3] * c[lane];
+    acc = acc + cf[0] * c[lane - 3];
+    acc = acc + cf[1] * c[lane - 2];
+    acc = acc + cf[2] * c[lane - 1];
+    acc = acc + cf[4] * c[lane + 1];
+    acc = acc + cf[5] * c[lane + 2];
+    acc = acc + cf[6] * c[lane + 3];
     return acc;
   };
-  const float f_dcrj = A.dcrjx[i] * A.dcrjy[j] * A.dcrjz[k];
-  float acc;
-  // xx, yy, zz
-  float Jii = M.f_c[q] * M.g3_c[k];
-  Jii = (float)(1.0 * 1.0 / (double)Jii);
-  acc = P4F(0) * M.f1_1[q - fr] * zs_asc(A.u1, pos - sl);
-  acc = acc + P4F(1) * M.f1_1[q] * zs_asc(A.u1, pos);
-  acc = acc + P4F(2) * M.f1_1[q + fr] * zs_asc(A.u1, pos + sl);
-  acc = acc + P4F(3) * M.f1_1[q + 2 * fr] * zs_asc(A.u1, pos + 2 * sl);
-  const float vs1 = D4F(1) * A.u1[pos] + D4F(0) * A.u1[pos - sl] + D4F(2) * A.u1[pos + sl] +
-                    D4F(3) * A.u1[pos + 2 * sl] - Jii * M.g_c[k] * acc;
-  acc = P4F(2) * M.f2_2[q] * zs_asc(A.v1, pos);
-  acc = acc + P4F(0) * M.f2_2[q - 2] * zs_asc(A.v1, pos - 2 * yl);
-  acc = acc + P4F(1) * M.f2_2[q - 1] * zs_asc(A.v1, pos - yl);
-  acc = acc + P4F(3) * M.f2_2[q + 1] * zs_asc(A.v1, pos + yl);
-  const float vs2 = D4F(2) * A.v1[pos] + D4F(0) * A.v1[pos - 2 * yl] + D4F(1) * A.v1[pos - yl] +
-                    D4F(3) * A.v1[pos + yl] - Jii * M.g_c[k] * acc;
-  const float vs3 = Jii * (D4F(2) * A.w1[pos] + D4F(0) * A.w1[pos - 2] +
-                           D4F(1) * A.w1[pos - 1] + D4F(3) * A.w1[pos + 1]);
-  // xy (the interior kernel reads g3_c six levels up, stress_attenuation.cu:473)
-  float J12i = M.f[q] * M.g3_c[k + 6];
-  J12i = (float)(1.0 / (double)J12i);
-  acc = P4F(1) * M.f2_1[q] * zs_mid(phdz4, A.u1, pos);
-  acc = acc + P4F(0) * M.f2_1[q - 1] * zs_mid(phdz4, A.u1, pos - yl);
-  acc = acc + P4F(2) * M.f2_1[q + 1] * zs_mid(phdz4, A.u1, pos + yl);
-  acc = acc + P4F(3) * M.f2_1[q + 2] * zs_mid(phdz4, A.u1, pos + 2 * yl);
-  const float xyv1 = D4F(1) * A.u1[pos] + D4F(0) * A.u1[pos - yl] + D4F(2) * A.u1[pos + yl] +
-                     D4F(3) * A.u1[pos + 2 * yl] - J12i * M.g_c[k] * acc;
-  acc = P4F(2) * M.f1_2[q] * zs_mid(phdz4, A.v1, pos);
-  acc = acc + P4F(0) * M.f1_2[q - 2 * fr] * zs_mid(phdz4, A.v1, pos - 2 * sl);
-  acc = acc + P4F(1) * M.f1_2[q - fr] * zs_mid(phdz4, A.v1, pos - sl);
-  acc = acc + P4F(3) * M.f1_2[q + fr] * zs_mid(phdz4, A.v1, pos + sl);
-  const float xyv2 = D4F(2) * A.v1[pos] + D4F(0) * A.v1[pos - 2 * sl] +
-                     D4F(1) * A.v1[pos - sl] + D4F(3) * A.v1[pos + sl] - J12i * M.g_c[k] * acc;
-  // xz
-  float J13i = M.f_1[q] * M.g3[k];
-  J13i = (float)(1.0 * 1.0 / (double)J13i);
-  const float xzv1 = J13i * (D4F(1) * A.u1[pos] + D4F(0) * A.u1[pos - 1] +
-                             D4F(2) * A.u1[pos + 1] + D4F(3) * A.u1[pos + 2]);
-  acc = P4F(2) * M.f1_c[q] * zs_mid(pdhz4, A.w1, pos);
-  acc = acc + P4F(0) * M.f1_c[q - 2 * fr] * zs_mid(pdhz4, A.w1, pos - 2 * sl);
-  acc = acc + P4F(1) * M.f1_c[q - fr] * zs_mid(pdhz4, A.w1, pos - sl);
-  acc = acc + P4F(3) * M.f1_c[q + fr] * zs_mid(pdhz4, A.w1, pos + sl);
-  const float xzv2 = D4F(2) * A.w1[pos] + D4F(0) * A.w1[pos - 2 * sl] +
-                     D4F(1) * A.w1[pos - sl] + D4F(3) * A.w1[pos + sl] - J13i * M.g[k] * acc;
-  // yz
-  float J23i = M.f_2[q] * M.g3[k];
-  J23i = (float)(1.0 * 1.0 / (double)J23i);
-  const float yzv1 = J23i * (D4F(1) * A.v1[pos] + D4F(0) * A.v1[pos - 1] +
-                             D4F(2) * A.v1[pos + 1] + D4F(3) * A.v1[pos + 2]);
-  acc = P4F(1) * M.f2_c[q] * zs_mid(pdhz4, A.w1, pos);
-  acc = acc + P4F(0) * M.f2_c[q - 1] * zs_mid(pdhz4, A.w1, pos - yl);
-  acc = acc + P4F(2) * M.f2_c[q + 1] * zs_mid(pdhz4, A.w1, pos + yl);
-  acc = acc + P4F(3) * M.f2_c[q + 2] * zs_mid(pdhz4, A.w1, pos + 2 * yl);
-  const float yzv2 = D4F(1) * A.w1[pos] + D4F(0) * A.w1[pos - yl] + D4F(2) * A.w1[pos + yl] +
-                     D4F(3) * A.w1[pos + 2 * yl] - J23i * M.g[k] * acc;
-  topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2, yzv1, yzv2);
-}
+  auto zrow = [&](const FTab<6, 9>& tab, const float* c) {  // top rows, m = 8 .. 0
+    float acc = tab.v[kb][8] * c[lb - 8];
+#pragma unroll
+    for (int m = 7; m >= 0; --m) acc = acc + tab.v[kb][m] * c[lb - m];
+    return acc;
+  };
+  // the three x-group sums of this lane for a staged column (non-output
+  // lanes compute nothing: their neighbours may lie outside the segment)
+  auto zu_x = [&](const float* c) { return !out ? 0.f : (bnd ? zrow(c_str_phdz4r, c) : zasc(c)); };
+  auto zv_x = [&](const float* c) { return !out ? 0.f : (bnd ? zrow(c_str_phdz4r, c) : zmid(phdz4, c)); };
+  auto zw = [&](const float* c) { return !out ? 0.f : (bnd ? zrow(c_str_pdhz4r, c) : zmid(pdhz4, c)); };
 
-// Top six stress levels k = align+nz-1-kb, kb = 0..5 (dtopo_str_112).
-template <class ST>
-__global__ __launch_bounds__(192) void topo_str_top_kernel(const TopoStrArgs<ST> A,
-                                                           const TGeom G) {
-  const int kb = (int)threadIdx.x;
-  const int j = G.bj + (int)(blockIdx.y * 32 + threadIdx.y);
-  const int i = G.bi + (int)blockIdx.z;
-  if (kb >= 6 || j >= G.ej || i >= G.ei) return;
-  const int nz = G.nz;
-  const int sl = G.sl, yl = G.yl, fr = G.frow;
-  const int k = AWP_ALIGN + nz - 1 - kb;
-  const int col = yl * j + sl * i + AWP_ALIGN;  // level 0 of the column
-  const int pos = col + nz - 1 - kb;
-  const int q = j + AWP_ALIGN + i * fr;
-  const TMetrics& M = A.M;
+  const int colj = yl * j + K;  // level K of column (i = 0, j)
+  const typename ST::T* u1 = A.u1.p;
+  (void)u1;
+  // raw x windows at this level: u (i-1 .. i+2), v and w (i-2 .. i+1), and
+  // the x-group z sums Zu (i-1 .. i+2), Zv, Zw (i-2 .. i+1)
+  float ux[4], vx[4], wx[4], Zu[4], Zv[4], Zw[4];
+  // prologue: stage the columns the first plane's windows need
+  for (int q = 0; q < 3; ++q) {
+    const int iu = ib - 1 + q, iv = ib - 2 + q;
+    float(*L)[64] = lds[q & 1][w];
+    ux[q + 1] = A.u1[colj + sl * iu];
+    vx[q + 1] = A.v1[colj + sl * iv];
+    wx[q + 1] = A.w1[colj + sl * iv];
+    L[C_UN][lane] = ux[q + 1];
+    L[C_VN][lane] = vx[q + 1];
+    L[C_WN][lane] = wx[q + 1];
+    __syncthreads();
+    Zu[q + 1] = zu_x(L[C_UN]);
+    Zv[q + 1] = zv_x(L[C_VN]);
+    Zw[q + 1] = zw(L[C_WN]);
+  }
 
-  // column sums over levels nz-9 .. nz-1, table row kb, terms m = 8 .. 0
-  auto zr = [&](const FTab<6, 9>& tab, const TField<ST>& fld, int o) {
-    float acc = tab.v[kb][8] * fld[o + nz - 9];
+  const float g_ck = M.g_c[K], g_k = M.g[K], g3c_k = M.g3_c[K], g3_k = M.g3[K];
+  const float g3c_k6 = M.g3_c[bnd ? K : K + 6];  // dtopo_str_111 reads six levels up
+  const float dcz = A.dcrjz[K];
+  const float dcy = A.dcrjy[j];
+
+  for (int i = ib; i < ie; ++i) {
+    const int par = (i + 1) & 1;  // the prologue used parities 0, 1, 0
+    float(*L)[64] = lds[par][w];
+    const int pos = colj + sl * i;
 #pragma unroll
-    for (int m = 7; m >= 0; --m) acc = acc + tab.v[kb][m] * fld[o + nz - 1 - m];
-    return acc;
-  };
-  const float f_dcrj = A.dcrjx[i] * A.dcrjy[j] * A.dcrjz[k];
-  float acc;
-  float Jii = M.f_c[q] * M.g3_c[k];
-  Jii = (float)(1.0 * 1.0 / (double)Jii);
-  acc = P4F(1) * M.f1_1[q] * zr(c_str_phdz4r, A.u1, col);
-  acc = acc + P4F(0) * M.f1_1[q - fr] * zr(c_str_phdz4r, A.u1, col - sl);
-  acc = acc + P4F(2) * M.f1_1[q + fr] * zr(c_str_phdz4r, A.u1, col + sl);
-  acc = acc + P4F(3) * M.f1_1[q + 2 * fr] * zr(c_str_phdz4r, A.u1, col + 2 * sl);
-  const float vs1 = D4F(1) * A.u1[pos] + D4F(0) * A.u1[pos - sl] + D4F(2) * A.u1[pos + sl] +
-                    D4F(3) * A.u1[pos + 2 * sl] - Jii * M.g_c[k] * acc;
-  acc = P4F(2) * M.f2_2[q] * zr(c_str_phdz4r, A.v1, col);
-  acc = acc + P4F(0) * M.f2_2[q - 2] * zr(c_str_phdz4r, A.v1, col - 2 * yl);
-  acc = acc + P4F(1) * M.f2_2[q - 1] * zr(c_str_phdz4r, A.v1, col - yl);
-  acc = acc + P4F(3) * M.f2_2[q + 1] * zr(c_str_phdz4r, A.v1, col + yl);
-  const float vs2 = D4F(2) * A.v1[pos] + D4F(0) * A.v1[pos - 2 * yl] + D4F(1) * A.v1[pos - yl] +
-                    D4F(3) * A.v1[pos + yl] - Jii * M.g_c[k] * acc;
-  float zs = c_str_dhz4r.v[kb][7] * A.w1[col + nz - 8];
+    for (int q = 0; q < 3; ++q) {
+      ux[q] = ux[q + 1];
+      vx[q] = vx[q + 1];
+      wx[q] = wx[q + 1];
+      Zu[q] = Zu[q + 1];
+      Zv[q] = Zv[q + 1];
+      Zw[q] = Zw[q + 1];
+    }
+    ux[3] = A.u1[pos + 2 * sl];
+    vx[3] = A.v1[pos + sl];
+    wx[3] = A.w1[pos + sl];
+    const float um1 = A.u1[pos - yl], up1 = A.u1[pos + yl], up2 = A.u1[pos + 2 * yl];
+    const float vm2 = A.v1[pos - 2 * yl], vm1 = A.v1[pos - yl], vp1 = A.v1[pos + yl];
+    const float wm1 = A.w1[pos - yl], wp1 = A.w1[pos + yl], wp2 = A.w1[pos + 2 * yl];
+    const float uo = ux[1], vo = vx[2], wo = wx[2];
+    L[C_UN][lane] = ux[3];
+    L[C_UO][lane] = uo;
+    L[C_UM1][lane] = um1;
+    L[C_UP1][lane] = up1;
+    L[C_UP2][lane] = up2;
+    L[C_VN][lane] = vx[3];
+    L[C_VO][lane] = vo;
+    L[C_VM2][lane] = vm2;
+    L[C_VM1][lane] = vm1;
+    L[C_VP1][lane] = vp1;
+    L[C_WN][lane] = wx[3];
+    L[C_WO][lane] = wo;
+    L[C_WM1][lane] = wm1;
+    L[C_WP1][lane] = wp1;
+    L[C_WP2][lane] = wp2;
+    __syncthreads();
+    Zu[3] = zu_x(L[C_UN]);
+    Zv[3] = zv_x(L[C_VN]);
+    Zw[3] = zw(L[C_WN]);
+    if (out) {
+      const int q = j + AWP_ALIGN + i * fr;
+      const float f_dcrj = A.dcrjx[i] * dcy * dcz;
+      float acc;
+      // xx, yy, zz
+      float Jii = M.f_c[q] * g3c_k;
+      Jii = (float)(1.0 * 1.0 / (double)Jii);
+      if (!bnd) {
+        acc = P4F(0) * M.f1_1[q - fr] * Zu[0];
+        acc = acc + P4F(1) * M.f1_1[q] * Zu[1];
+      } else {
+        acc = P4F(1) * M.f1_1[q] * Zu[1];
+        acc = acc + P4F(0) * M.f1_1[q - fr] * Zu[0];
+      }
+      acc = acc + P4F(2) * M.f1_1[q + fr] * Zu[2];
+      acc = acc + P4F(3) * M.f1_1[q + 2 * fr] * Zu[3];
+      const float vs1 = D4F(1) * ux[1] + D4F(0) * ux[0] + D4F(2) * ux[2] + D4F(3) * ux[3] -
+                        Jii * g_ck * acc;
+      const float zv_o = bnd ? zrow(c_str_phdz4r, L[C_VO]) : zasc(L[C_VO]);
+      const float zv_m2 = bnd ? zrow(c_str_phdz4r, L[C_VM2]) : zasc(L[C_VM2]);
+      const float zv_m1 = bnd ? zrow(c_str_phdz4r, L[C_VM1]) : zasc(L[C_VM1]);
+      const float zv_p1 = bnd ? zrow(c_str_phdz4r, L[C_VP1]) : zasc(L[C_VP1]);
+      acc = P4F(2) * M.f2_2[q] * zv_o;
+      acc = acc + P4F(0) * M.f2_2[q - 2] * zv_m2;
+      acc = acc + P4F(1) * M.f2_2[q - 1] * zv_m1;
+      acc = acc + P4F(3) * M.f2_2[q + 1] * zv_p1;
+      const float vs2 = D4F(2) * vo + D4F(0) * vm2 + D4F(1) * vm1 + D4F(3) * vp1 -
+                        Jii * g_ck * acc;
+      float vs3;
+      if (!bnd) {
+        const float* c = L[C_WO];
+        vs3 = Jii * (D4F(2) * c[lane] + D4F(0) * c[lane - 2] + D4F(1) * c[lane - 1] +
+                     D4F(3) * c[lane + 1]);
+      } else {
+        const float* c = L[C_WO];
+        float zs = c_str_dhz4r.v[kb][7] * c[lb - 7];
 #pragma unroll
-  for (int m = 6; m >= 0; --m) zs = zs + c_str_dhz4r.v[kb][m] * A.w1[col + nz - 1 - m];
-  const float vs3 = Jii * zs;
-  float J12i = M.f[q] * M.g3_c[k];
-  J12i = (float)(1.0 * 1.0 / (double)J12i);
-  acc = P4F(1) * M.f2_1[q] * zr(c_str_phdz4r, A.u1, col);
-  acc = acc + P4F(0) * M.f2_1[q - 1] * zr(c_str_phdz4r, A.u1, col - yl);
-  acc = acc + P4F(2) * M.f2_1[q + 1] * zr(c_str_phdz4r, A.u1, col + yl);
-  acc = acc + P4F(3) * M.f2_1[q + 2] * zr(c_str_phdz4r, A.u1, col + 2 * yl);
-  const float xyv1 = D4F(1) * A.u1[pos] + D4F(0) * A.u1[pos - yl] + D4F(2) * A.u1[pos + yl] +
-                     D4F(3) * A.u1[pos + 2 * yl] - J12i * M.g_c[k] * acc;
-  acc = P4F(2) * M.f1_2[q] * zr(c_str_phdz4r, A.v1, col);
-  acc = acc + P4F(0) * M.f1_2[q - 2 * fr] * zr(c_str_phdz4r, A.v1, col - 2 * sl);
-  acc = acc + P4F(1) * M.f1_2[q - fr] * zr(c_str_phdz4r, A.v1, col - sl);
-  acc = acc + P4F(3) * M.f1_2[q + fr] * zr(c_str_phdz4r, A.v1, col + sl);
-  const float xyv2 = D4F(2) * A.v1[pos] + D4F(0) * A.v1[pos - 2 * sl] +
-                     D4F(1) * A.v1[pos - sl] + D4F(3) * A.v1[pos + sl] - J12i * M.g_c[k] * acc;
-  float J13i = M.f_1[q] * M.g3[k];
-  J13i = (float)(1.0 * 1.0 / (double)J13i);
-  zs = c_str_dz4r.v[kb][6] * A.u1[col + nz - 7];
+        for (int m = 6; m >= 0; --m) zs = zs + c_str_dhz4r.v[kb][m] * c[lb - m];
+        vs3 = Jii * zs;
+      }
+      // xy
+      float J12i = M.f[q] * g3c_k6;
+      J12i = (float)(1.0 / (double)J12i);
+      const float zu_o = bnd ? zrow(c_str_phdz4r, L[C_UO]) : zmid(phdz4, L[C_UO]);
+      const float zu_m1 = bnd ? zrow(c_str_phdz4r, L[C_UM1]) : zmid(phdz4, L[C_UM1]);
+      const float zu_p1 = bnd ? zrow(c_str_phdz4r, L[C_UP1]) : zmid(phdz4, L[C_UP1]);
+      const float zu_p2 = bnd ? zrow(c_str_phdz4r, L[C_UP2]) : zmid(phdz4, L[C_UP2]);
+      acc = P4F(1) * M.f2_1[q] * zu_o;
+      acc = acc + P4F(0) * M.f2_1[q - 1] * zu_m1;
+      acc = acc + P4F(2) * M.f2_1[q + 1] * zu_p1;
+      acc = acc + P4F(3) * M.f2_1[q + 2] * zu_p2;
+      const float xyv1 = D4F(1) * uo + D4F(0) * um1 + D4F(2) * up1 + D4F(3) * up2 -
+                         J12i * g_ck * acc;
+      acc = P4F(2) * M.f1_2[q] * Zv[2];
+      acc = acc + P4F(0) * M.f1_2[q - 2 * fr] * Zv[0];
+      acc = acc + P4F(1) * M.f1_2[q - fr] * Zv[1];
+      acc = acc + P4F(3) * M.f1_2[q + fr] * Zv[3];
+      const float xyv2 = D4F(2) * vx[2] + D4F(0) * vx[0] + D4F(1) * vx[1] + D4F(3) * vx[3] -
+                         J12i * g_ck * acc;
+      // xz
+      float J13i = M.f_1[q] * g3_k;
+      J13i = (float)(1.0 * 1.0 / (double)J13i);
+      float xzv1;
+      if (!bnd) {
+        const float* c = L[C_UO];
+        xzv1 = J13i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
+                       D4F(3) * c[lane + 2]);
+      } else {
+        const float* c = L[C_UO];
+        float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
 #pragma unroll
-  for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * A.u1[col + nz - 1 - m];
-  const float xzv1 = J13i * zs;
-  acc = P4F(2) * M.f1_c[q] * zr(c_str_pdhz4r, A.w1, col);
-  acc = acc + P4F(0) * M.f1_c[q - 2 * fr] * zr(c_str_pdhz4r, A.w1, col - 2 * sl);
-  acc = acc + P4F(1) * M.f1_c[q - fr] * zr(c_str_pdhz4r, A.w1, col - sl);
-  acc = acc + P4F(3) * M.f1_c[q + fr] * zr(c_str_pdhz4r, A.w1, col + sl);
-  const float xzv2 = D4F(2) * A.w1[pos] + D4F(0) * A.w1[pos - 2 * sl] +
-                     D4F(1) * A.w1[pos - sl] + D4F(3) * A.w1[pos + sl] - J13i * M.g[k] * acc;
-  float J23i = M.f_2[q] * M.g3[k];
-  J23i = (float)(1.0 * 1.0 / (double)J23i);
-  zs = c_str_dz4r.v[kb][6] * A.v1[col + nz - 7];
+        for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
+        xzv1 = J13i * zs;
+      }
+      acc = P4F(2) * M.f1_c[q] * Zw[2];
+      acc = acc + P4F(0) * M.f1_c[q - 2 * fr] * Zw[0];
+      acc = acc + P4F(1) * M.f1_c[q - fr] * Zw[1];
+      acc = acc + P4F(3) * M.f1_c[q + fr] * Zw[3];
+      const float xzv2 = D4F(2) * wx[2] + D4F(0) * wx[0] + D4F(1) * wx[1] + D4F(3) * wx[3] -
+                         J13i * g_k * acc;
+      // yz
+      float J23i = M.f_2[q] * g3_k;
+      J23i = (float)(1.0 * 1.0 / (double)J23i);
+      float yzv1;
+      if (!bnd) {
+        const float* c = L[C_VO];
+        yzv1 = J23i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
+                       D4F(3) * c[lane + 2]);
+      } else {
+        const float* c = L[C_VO];
+        float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
 #pragma unroll
-  for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * A.v1[col + nz - 1 - m];
-  const float yzv1 = J23i * zs;
-  acc = P4F(1) * M.f2_c[q] * zr(c_str_pdhz4r, A.w1, col);
-  acc = acc + P4F(0) * M.f2_c[q - 1] * zr(c_str_pdhz4r, A.w1, col - yl);
-  acc = acc + P4F(2) * M.f2_c[q + 1] * zr(c_str_pdhz4r, A.w1, col + yl);
-  acc = acc + P4F(3) * M.f2_c[q + 2] * zr(c_str_pdhz4r, A.w1, col + 2 * yl);
-  const float yzv2 = D4F(1) * A.w1[pos] + D4F(0) * A.w1[pos - yl] + D4F(2) * A.w1[pos + yl] +
-                     D4F(3) * A.w1[pos + 2 * yl] - J23i * M.g[k] * acc;
-  topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2, yzv1, yzv2);
+        for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
+        yzv1 = J23i * zs;
+      }
+      const float zw_m1 = zw(L[C_WM1]), zw_p1 = zw(L[C_WP1]), zw_p2 = zw(L[C_WP2]);
+      acc = P4F(1) * M.f2_c[q] * Zw[2];
+      acc = acc + P4F(0) * M.f2_c[q - 1] * zw_m1;
+      acc = acc + P4F(2) * M.f2_c[q + 1] * zw_p1;
+      acc = acc + P4F(3) * M.f2_c[q + 2] * zw_p2;
+      const float yzv2 = D4F(1) * wo + D4F(0) * wm1 + D4F(2) * wp1 + D4F(3) * wp2 -
+                         J23i * g_k * acc;
+      topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2, yzv1,
+                          yzv2);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -841,18 +961,17 @@ void topo_velocity_launch(awp_topo_t* T, const WaveScales& W, const char* where)
   G.bj = 0;
   G.ej = T->ny;
   hipStream_t st = T->stream_i;
-  {
-    dim3 block(64, 4, 1);
-    dim3 grid(awp_cdiv(T->nz - 6, 64), awp_cdiv(T->ny, 4), T->nx);
-    hipLaunchKernelGGL(topo_vel_int_kernel<ST>, grid, block, 0, st, A, G);
-    if (!awp_check_launch(where)) return;
-  }
-  {
-    dim3 block(6, 32, 1);
-    dim3 grid(1, awp_cdiv(T->ny, 32), T->nx);
-    hipLaunchKernelGGL(topo_vel_top_kernel<ST>, grid, block, 0, st, A, G);
-    awp_check_launch(where);
-  }
+  // segments: interior ones every 58 levels from level 3, then the top one
+  // (levels nz-64 .. nz-1) writing from nz-61 (or 3) up
+  VelTiles V;
+  V.ktop0 = T->nz - 64;
+  V.wtop0 = V.ktop0 + 3 > 3 ? V.ktop0 + 3 : 3;
+  V.nint = V.wtop0 > 3 ? awp_cdiv(V.wtop0 - 3, 58) : 0;
+  V.ichunk = awp_tuning().topo_vel_ichunk;
+  dim3 block(64, VT_BJ, 1);
+  dim3 grid(V.nint + 1, awp_cdiv(T->ny, VT_BJ), awp_cdiv(T->nx, V.ichunk));
+  hipLaunchKernelGGL(topo_vel_kernel<ST>, grid, block, 0, st, A, G, V);
+  awp_check_launch(where);
 }
 
 template <class ST>
@@ -921,18 +1040,15 @@ void topo_stress_launch(awp_topo_t* T, const WaveScales& W, const char* where) {
   G.ej = y1 + shift;
   if (G.ei <= G.bi || G.ej <= G.bj) return;
   hipStream_t st = T->stream_i;
-  {
-    dim3 block(64, 4, 1);
-    dim3 grid(awp_cdiv(AWP_ALIGN + T->nz - 6, 64), awp_cdiv(G.ej - G.bj, 4), G.ei - G.bi);
-    hipLaunchKernelGGL(topo_str_int_kernel<ST>, grid, block, 0, st, A, G);
-    if (!awp_check_launch(where)) return;
-  }
-  {
-    dim3 block(6, 32, 1);
-    dim3 grid(1, awp_cdiv(G.ej - G.bj, 32), G.ei - G.bi);
-    hipLaunchKernelGGL(topo_str_top_kernel<ST>, grid, block, 0, st, A, G);
-    awp_check_launch(where);
-  }
+  VelTiles V;
+  V.ktop0 = T->nz - 64;
+  V.wtop0 = V.ktop0 + 3 > 3 ? V.ktop0 + 3 : 3;
+  V.nint = V.wtop0 > 3 ? awp_cdiv(V.wtop0 - 3, 58) : 0;
+  V.ichunk = awp_tuning().topo_str_ichunk;
+  dim3 block(64, ST_BJ, 1);
+  dim3 grid(V.nint + 1, awp_cdiv(G.ej - G.bj, ST_BJ), awp_cdiv(G.ei - G.bi, V.ichunk));
+  hipLaunchKernelGGL(topo_str_kernel<ST>, grid, block, 0, st, A, G, V);
+  awp_check_launch(where);
 }
 
 bool neighbourless(int rank, const char* where) {

@@ -10,6 +10,8 @@ struct AwpTuning {
   int str_ichunk;   // >0 forces the stress i-chunk length
   int xcd_remap;    // 1: XCD-aware tile order, 0: plain order
   int kalign;       // 1: start k tiles on a 32-float (128 B) line boundary
+  int topo_vel_ichunk;  // planes per march of the curvilinear velocity kernel
+  int topo_str_ichunk;  // same, curvilinear stress kernel
 };
 
 AwpTuning& awp_tuning();

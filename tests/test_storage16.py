@@ -47,7 +47,13 @@ def _test_floats(rng, n=200000):
     ties16 = ((h.astype(np.float64) + h_next) / 2).astype(np.float32)
     b = (rng.integers(0, 0x7f80, 5000).astype(np.uint32) << 16)
     ties_bf = (b | 0x8000).view(np.float32)
-    return np.concatenate([x, h, ties16, -ties16, ties_bf, -ties_bf,
+    # every bfloat16 pattern of the normal range with the low halves at and
+    # around the tie point, and the largest values (which round to infinity)
+    allb = np.arange(0x0080, 0x7f80, dtype=np.uint32) << 16
+    near_bf = (allb[:, None] | np.uint32([0x7fff, 0x8000, 0x8001, 0xffff])[None, :])
+    near_bf = near_bf.ravel().view(np.float32)
+    return np.concatenate([x, h, ties16, -ties16, ties_bf, -ties_bf, near_bf, -near_bf,
+                           np.uint32([0x7f7f7fff, 0x7f7f8000, 0x7f7fffff]).view(np.float32),
                            np.float32([0.0, -0.0, 65504, 65519.99, 65520, 1e10, 6e-8, 3e-8])])
 
 
@@ -57,8 +63,10 @@ def _bf16_reference(x):
     lo = (b & 0xFFFF0000).astype(np.uint32)
     hi = lo + 0x10000
     xl = lo.view(np.float32).astype(np.float64)
-    xh = hi.view(np.float32).astype(np.float64)
     xv = x.astype(np.float64)
+    # above the largest bfloat16 the next candidate is 2^128 (infinity)
+    xh = np.where((hi & 0x7FFFFFFF) == 0x7F800000, np.copysign(2.0 ** 128, xv),
+                  hi.view(np.float32).astype(np.float64))
     dl, dh = np.abs(xv - xl), np.abs(xh - xv)
     pick_hi = (dh < dl) | ((dh == dl) & (((lo >> 16) & 1) == 1))
     return np.where(pick_hi, hi >> 16, lo >> 16).astype(np.uint16)
