@@ -8,9 +8,12 @@ follow from dJ/dlambda = -lam^2 dJ/dlam (moduli_gradients).
 
 The forward keeps the full state every `checkpoint_every` steps (the memory
 variables make the step irreversible); the backward sweep recomputes one
-segment at a time, storing its velocities and stresses, and runs the adjoint
-steps of the segment in reverse (awp_adjoint_step).  All of it runs in
-libawp_hip.so; torch holds the device buffers.
+segment at a time and runs the adjoint steps of the segment in reverse
+(awp_adjoint_step).  The recompute steps out of place (awp_dvelcx_oop,
+awp_dstrqc_oop) from ring slot n-1 into slot n, so every velocity and stress
+state of the segment stays in HBM without copies; the memory variables are
+updated in place.  All of it runs in libawp_hip.so; torch holds the device
+buffers.
 """
 from __future__ import annotations
 
@@ -20,6 +23,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .grid import ALIGN, NGSL
 from .media import Medium
 from .solver import FIELDS, Solver
 
@@ -67,6 +71,22 @@ class AdjointProblem:
         self.col = torch.zeros((3,) + l2, **f32)
         self.checkpoints = {}
         self.forward_done = False
+        if self.solver.variant != "dstrqc":
+            raise ValueError("the adjoint transposes the production stress update (dstrqc_H)")
+        # The out-of-place recompute keeps the cells no kernel writes at their
+        # initial zeros; a source there would accumulate in place forward but
+        # not in the ring, so sources must sit in the stress update region.
+        if self.solver.levels[0].src is not None:
+            psrc = self.solver.levels[0].src["psrc"].cpu().numpy().reshape(-1, 3)
+            (xs, xe), (ys, ye) = d.i_range, d.j_range
+            ii, jj = psrc[:, 0] + 1 + NGSL, psrc[:, 1] + 1 + NGSL  # tpsrc -> array index
+            kk = psrc[:, 2] + ALIGN - 1
+            ok = ((ii >= xs) & (ii <= xe) & (jj >= ys) & (jj <= ye) & (kk >= ALIGN + 3)
+                  & (kk <= d.ktop))
+            if not ok.all():
+                raise ValueError("adjoint sources must lie in the stress update region")
+        # ring of K+1 velocity + stress states (slot 0 holds the checkpoint)
+        self.ring = [{x: torch.zeros(sh, **f32) for x in VEL + STRESS} for _ in range(self.K + 1)]
 
     # -- medium --
     def set_medium(self, lam, mu, rho) -> None:
@@ -124,6 +144,35 @@ class AdjointProblem:
     def traces(self) -> np.ndarray:
         return self.solver.traces()
 
+    def _recompute_step(self, fin: dict, fout: dict) -> None:
+        """One forward step (the pmcl3d order of Solver.step for one grid,
+        NVE = 1) from state fin into fout; memory variables in place."""
+        s, lib = self.solver, self.lib
+        L = s.levels[0]
+        d, m, st = L.dims, L.m, s.stream
+        (s_i, e_i), (s_j, e_j) = d.i_range, d.j_range
+        s.cur_step += 1
+        p = lambda t: t.data_ptr()  # noqa: E731
+        lib.awp_dvelcx_oop(*(p(fout[x]) for x in VEL), *(p(fin[x]) for x in VEL),
+                           *(p(fin[x]) for x in STRESS), p(m["dcrjx"]), p(m["dcrjy"]),
+                           p(m["dcrjz"]), p(m["d1"]), d.nyt, d.nzt, st, s_i, e_i, 0, 1)
+        s0 = _pp([fin[x] for x in STRESS])
+        lib.awp_dstrqc_oop(*(p(fout[x]) for x in STRESS), s0,
+                           *(p(L.f[x]) for x in ("r1", "r2", "r3", "r4", "r5", "r6")),
+                           *(p(fout[x]) for x in VEL), p(m["lam"]), p(m["mu"]), p(m["qp"]),
+                           p(m["coeff"]), p(m["qs"]), p(m["dcrjx"]), p(m["dcrjy"]),
+                           p(m["dcrjz"]), d.nyt, d.nzt, st, p(m["lam_mu"]), p(m["vx1"]),
+                           p(m["vx2"]), p(m["ww"]), p(m["wwo"]), d.nxt, s.NPC, 0, 0, s_i, e_i,
+                           s_j, e_j, 0)
+        src = L.src
+        nst = s.source_steps(0)
+        if nst and s.cur_step < nst:
+            axx, ayy, azz, axz, ayz, axy = src["cols"]
+            lib.addsrc_H(s.cur_step + 1, nst, 3, p(src["psrc"]), src["npsrc"], st, p(axx),
+                         p(ayy), p(azz), p(axz), p(ayz), p(axy), p(fout["xx"]), p(fout["yy"]),
+                         p(fout["zz"]), p(fout["xy"]), p(fout["yz"]), p(fout["xz"]), 0)
+        lib.fstr_H(p(fout["zz"]), p(fout["xz"]), p(fout["yz"]), st, s_i, e_i, s_j, e_j)
+
     # -- backward --
     def backward(self):
         """(dJ/dlam, dJ/dmu, dJ/drho) as binary64 device tensors on the padded
@@ -147,27 +196,23 @@ class AdjointProblem:
         starts = sorted(self.checkpoints)
         for n0 in reversed(starts):
             n1 = min(n0 + self.K, self.nt)
-            # recompute the segment from its checkpoint, keeping v and sigma
-            f = s.f
+            # recompute the segment from its checkpoint into the ring
             for x, t in zip(FIELDS, self.checkpoints[n0]):
-                f[x].copy_(t)
+                (self.ring[0][x] if x in self.ring[0] else s.f[x]).copy_(t)
             s.cur_step = n0
-            keep = {n0: [f[x].clone() for x in VEL + STRESS]}
             for n in range(n0 + 1, n1 + 1):
-                s.step()
-                keep[n] = [f[x].clone() for x in VEL + STRESS]
+                self._recompute_step(self.ring[n - 1 - n0], self.ring[n - n0])
             for n in range(n1, n0, -1):
                 res_n = self.res[n - 1]
                 lib.awp_adjoint_inject(self.adj["u1"].data_ptr(), self.adj["v1"].data_ptr(),
                                        self.adj["w1"].data_ptr(), rec["idx"].data_ptr(), npts,
                                        res_n.data_ptr(), st)
-                fv = _pp(keep[n][:3])
-                fs = _pp(keep[n - 1][3:])
+                fv = _pp([self.ring[n - n0][x] for x in VEL])
+                fs = _pp([self.ring[n - 1 - n0][x] for x in STRESS])
                 lib.awp_adjoint_step(adj, fv, fs, med, m["ww"].data_ptr(), m["coeff"].data_ptr(),
                                      m["dcrjx"].data_ptr(), m["dcrjy"].data_ptr(),
                                      m["dcrjz"].data_ptr(), scratch, g32,
                                      self.gl32.data_ptr(), self.col.data_ptr(), st)
-            del keep
             for a, b in zip(self.g32, self.g64):
                 lib.awp_adjoint_flush(a.data_ptr(), b.data_ptr(), a.numel(), st)
             lib.awp_adjoint_flush(self.gl32.data_ptr(), self.gl64.data_ptr(), self.gl32.numel(), st)

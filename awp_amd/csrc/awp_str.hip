@@ -51,11 +51,19 @@ __device__ __forceinline__ float qw_new(float q, float ca, float cb, float wwo, 
   return w / wwo;
 }
 
-template <int BJ, bool NEWLIT, class ST>
+// OOP: the stresses before the step are read from s0 (xx yy zz xy xz yz) and
+// the updated ones written to xx..yz (distinct arrays); otherwise xx..yz are
+// updated in place and s0 is unused.  The memory variables are always
+// updated in place.
+struct CStress6 {
+  const void* f[6];
+};
+
+template <int BJ, bool NEWLIT, bool OOP, class ST>
 __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     typename ST::T* __restrict__ xx, typename ST::T* __restrict__ yy,
     typename ST::T* __restrict__ zz, typename ST::T* __restrict__ xy,
-    typename ST::T* __restrict__ xz, typename ST::T* __restrict__ yz,
+    typename ST::T* __restrict__ xz, typename ST::T* __restrict__ yz, const CStress6 S0,
     typename ST::T* __restrict__ r1, typename ST::T* __restrict__ r2,
     typename ST::T* __restrict__ r3, typename ST::T* __restrict__ r4,
     typename ST::T* __restrict__ r5, typename ST::T* __restrict__ r6,
@@ -89,6 +97,14 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
   const float c1 = L.c1, c2 = L.c2, dth = L.dth, dh1 = L.dh1, DT = L.DT;
   const float dy = dcrjy[j];
   const float dz = dcrjz[k];
+
+  typedef typename ST::T T;
+  const T* const xx_in = OOP ? (const T*)S0.f[0] : xx;
+  const T* const yy_in = OOP ? (const T*)S0.f[1] : yy;
+  const T* const zz_in = OOP ? (const T*)S0.f[2] : zz;
+  const T* const xy_in = OOP ? (const T*)S0.f[3] : xy;
+  const T* const xz_in = OOP ? (const T*)S0.f[4] : xz;
+  const T* const yz_in = OOP ? (const T*)S0.f[5] : yz;
 
   const float su = SC.s[AWP_SU], sv = SC.s[AWP_SV], sw = SC.s[AWP_SW];
   const long long p0 = (long long)ia * S + o;
@@ -207,21 +223,21 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
 
     f_r = ST::ld(r1, p, SC.s[AWP_SR1]);
     f_rtmp = -h * (vs2 + vs3) + a1;
-    t = ST::ld(xx, p, SC.s[AWP_SXX]) + tmp - xm * (vs2 + vs3) + vx1p * f_r;
+    t = ST::ld(xx_in, p, SC.s[AWP_SXX]) + tmp - xm * (vs2 + vs3) + vx1p * f_r;
     ST::st(r1, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR1]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
     ST::st(xx, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SXX]);
 
     f_r = ST::ld(r2, p, SC.s[AWP_SR2]);
     f_rtmp = -h * (vs1 + vs3) + a1;
-    t = (ST::ld(yy, p, SC.s[AWP_SYY]) + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
+    t = (ST::ld(yy_in, p, SC.s[AWP_SYY]) + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
     ST::st(r2, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR2]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
     ST::st(yy, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SYY]);
 
     f_r = ST::ld(r3, p, SC.s[AWP_SR3]);
     f_rtmp = -h * (vs1 + vs2) + a1;
-    t = (ST::ld(zz, p, SC.s[AWP_SZZ]) + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
+    t = (ST::ld(zz_in, p, SC.s[AWP_SZZ]) + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
     ST::st(r3, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR3]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
     ST::st(zz, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SZZ]);
@@ -230,7 +246,7 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     vs2 = c1 * (v_0 - v_m1) + c2 * (v_p1 - v_m2);
     f_r = ST::ld(r4, p, SC.s[AWP_SR4]);
     f_rtmp = h1 * (vs1 + vs2);
-    t = ST::ld(xy, p, SC.s[AWP_SXY]) + xmu1 * (vs1 + vs2) + vx1p * f_r;
+    t = ST::ld(xy_in, p, SC.s[AWP_SXY]) + xmu1 * (vs1 + vs2) + vx1p * f_r;
     ST::st(r4, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR4]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
     ST::st(xy, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SXY]);
@@ -239,7 +255,7 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     vs2 = c1 * (w_0 - w_m1) + c2 * (w_p1 - w_m2);
     f_r = ST::ld(r5, p, SC.s[AWP_SR5]);
     f_rtmp = h2 * (vs1 + vs2);
-    t = ST::ld(xz, p, SC.s[AWP_SXZ]) + xmu2 * (vs1 + vs2) + vx1p * f_r;
+    t = ST::ld(xz_in, p, SC.s[AWP_SXZ]) + xmu2 * (vs1 + vs2) + vx1p * f_r;
     ST::st(r5, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR5]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
     ST::st(xz, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SXZ]);
@@ -248,7 +264,7 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     vs2 = c1 * (w_jp1 - w_0) + c2 * (w_jp2 - w_jm1);
     f_r = ST::ld(r6, p, SC.s[AWP_SR6]);
     f_rtmp = h3 * (vs1 + vs2);
-    t = ST::ld(yz, p, SC.s[AWP_SYZ]) + xmu3 * (vs1 + vs2) + vx1p * f_r;
+    t = ST::ld(yz_in, p, SC.s[AWP_SYZ]) + xmu3 * (vs1 + vs2) + vx1p * f_r;
     ST::st(r6, p, f_vx2 * f_r + f_wwo * f_rtmp, SC.inv[AWP_SR6]);
     f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
     ST::st(yz, p, (t + DT * f_rtmp) * f_dcrj, SC.inv[AWP_SYZ]);
@@ -312,9 +328,11 @@ __global__ __launch_bounds__(256) void fs_ghost_kernel(typename ST::T* __restric
          SC.inv[AWP_SW]);
 }
 
-// f15: xx yy zz xy xz yz r1..r6 u1 v1 w1 in the storage type of ST.
-template <bool NEWLIT, class ST>
-void dstrqc_common(const char* W, void* const* f15, float* lam, float* mu, float* qp,
+// f15: xx yy zz xy xz yz r1..r6 u1 v1 w1 in the storage type of ST; with OOP,
+// S0 holds the stresses before the step and f15's stresses receive the update.
+template <bool NEWLIT, bool OOP, class ST>
+void dstrqc_common(const char* W, void* const* f15, const CStress6& S0, float* lam, float* mu,
+                   float* qp,
                    float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
                    int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
                    float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i, int s_j,
@@ -369,15 +387,19 @@ void dstrqc_common(const char* W, void* const* f15, float* lam, float* mu, float
   L.ichunk = ichunk;
   L.nblocks = L.nkt * L.njt * nic;
 #define AWP_STR_ARGS                                                                          \
-  A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], A[8], A[9], A[10], A[11], A[12], A[13],    \
+  A[0], A[1], A[2], A[3], A[4], A[5], S0, A[6], A[7], A[8], A[9], A[10], A[11], A[12], A[13], \
       A[14], lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, vx1, vx2, ww, wwo, L, SC
+#define AWP_STR_LAUNCH(B)                                                                     \
+  hipLaunchKernelGGL((dstrqc_kernel<B, NEWLIT, OOP, ST>), dim3(L.nblocks), dim3(64, B), 0, St, \
+                     AWP_STR_ARGS)
   switch (bj) {
-    case 1: hipLaunchKernelGGL((dstrqc_kernel<1, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 1), 0, St, AWP_STR_ARGS); break;
-    case 2: hipLaunchKernelGGL((dstrqc_kernel<2, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 2), 0, St, AWP_STR_ARGS); break;
-    case 4: hipLaunchKernelGGL((dstrqc_kernel<4, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 4), 0, St, AWP_STR_ARGS); break;
-    case 8: hipLaunchKernelGGL((dstrqc_kernel<8, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 8), 0, St, AWP_STR_ARGS); break;
-    default: hipLaunchKernelGGL((dstrqc_kernel<16, NEWLIT, ST>), dim3(L.nblocks), dim3(64, 16), 0, St, AWP_STR_ARGS); break;
+    case 1: AWP_STR_LAUNCH(1); break;
+    case 2: AWP_STR_LAUNCH(2); break;
+    case 4: AWP_STR_LAUNCH(4); break;
+    case 8: AWP_STR_LAUNCH(8); break;
+    default: AWP_STR_LAUNCH(16); break;
   }
+#undef AWP_STR_LAUNCH
 #undef AWP_STR_ARGS
   awp_check_launch(W);
 }
@@ -448,9 +470,9 @@ void dstrqc_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, 
               int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i) {
   void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
   WaveScales SC = {};
-  dstrqc_common<false, StF32>("dstrqc_H", f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt,
-                              nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i,
-                              e_i, s_j, e_j, d_i, SC);
+  dstrqc_common<false, false, StF32>("dstrqc_H", f15, CStress6{}, lam, mu, qp, coeff, qs, dcrjx,
+                                     dcrjy, dcrjz, nyt, nzt, St, lam_mu, vx1, vx2, ww, wwo, NX,
+                                     NPC, rankx, ranky, s_i, e_i, s_j, e_j, d_i, SC);
 }
 
 void dstrqc_H_new(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
@@ -462,9 +484,9 @@ void dstrqc_H_new(float* xx, float* yy, float* zz, float* xy, float* xz, float* 
                   int s_j, int e_j, int d_i) {
   void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
   WaveScales SC = {};
-  dstrqc_common<true, StF32>("dstrqc_H_new", f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz,
-                             nyt, nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i,
-                             e_i, s_j, e_j, d_i, SC);
+  dstrqc_common<true, false, StF32>("dstrqc_H_new", f15, CStress6{}, lam, mu, qp, coeff, qs,
+                                    dcrjx, dcrjy, dcrjz, nyt, nzt, St, lam_mu, vx1, vx2, ww, wwo,
+                                    NX, NPC, rankx, ranky, s_i, e_i, s_j, e_j, d_i, SC);
 }
 
 void awp_dstrqc_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* xz,
@@ -480,13 +502,37 @@ void awp_dstrqc_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint
   if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
   void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
   if (fmt == AWP_FMT_BF16)
-    dstrqc_common<false, StBF16>(W, f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt,
-                                 St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i,
-                                 s_j, e_j, d_i, SC);
+    dstrqc_common<false, false, StBF16>(W, f15, CStress6{}, lam, mu, qp, coeff, qs, dcrjx, dcrjy,
+                                        dcrjz, nyt, nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC,
+                                        rankx, ranky, s_i, e_i, s_j, e_j, d_i, SC);
   else
-    dstrqc_common<false, StF16>(W, f15, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt, nzt,
-                                St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky, s_i, e_i,
-                                s_j, e_j, d_i, SC);
+    dstrqc_common<false, false, StF16>(W, f15, CStress6{}, lam, mu, qp, coeff, qs, dcrjx, dcrjy,
+                                       dcrjz, nyt, nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC,
+                                       rankx, ranky, s_i, e_i, s_j, e_j, d_i, SC);
+}
+
+void awp_dstrqc_oop(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                    const float* const* s0, float* r1, float* r2, float* r3, float* r4,
+                    float* r5, float* r6, float* u1, float* v1, float* w1, float* lam, float* mu,
+                    float* qp, float* coeff, float* qs, float* dcrjx, float* dcrjy,
+                    float* dcrjz, int nyt, int nzt, hipStream_t St, float* lam_mu, float* vx1,
+                    float* vx2, int* ww, float* wwo, int NX, int NPC, int rankx, int ranky,
+                    int s_i, int e_i, int s_j, int e_j, int d_i) {
+  const char* W = "awp_dstrqc_oop";
+  void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
+  CStress6 S0;
+  for (int c = 0; c < 6; ++c) {
+    S0.f[c] = s0 ? s0[c] : nullptr;
+    if (!S0.f[c] || S0.f[c] == f15[c]) {
+      awp_set_error(AWP_ERR_ARG, "%s: the stresses before and after the step must be distinct "
+                    "arrays (dstrqc_H updates in place)", W);
+      return;
+    }
+  }
+  WaveScales SC = {};
+  dstrqc_common<false, true, StF32>(W, f15, S0, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt,
+                                    nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky,
+                                    s_i, e_i, s_j, e_j, d_i, SC);
 }
 
 void fstr_H(float* zz, float* xz, float* yz, hipStream_t St, int s_i, int e_i, int s_j,

@@ -25,10 +25,15 @@ struct VelLaunch {
   float c1, c2, dth;
 };
 
-template <int BJ, class ST>
+// OOP: the velocities before the step are read from u0/v0/w0 and the
+// updated ones written to u1/v1/w1 (distinct arrays); otherwise u1/v1/w1 are
+// updated in place and u0/v0/w0 are unused.
+template <int BJ, bool OOP, class ST>
 __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     typename ST::T* __restrict__ u1, typename ST::T* __restrict__ v1,
-    typename ST::T* __restrict__ w1, const typename ST::T* __restrict__ xx,
+    typename ST::T* __restrict__ w1, const typename ST::T* __restrict__ u0,
+    const typename ST::T* __restrict__ v0, const typename ST::T* __restrict__ w0,
+    const typename ST::T* __restrict__ xx,
     const typename ST::T* __restrict__ yy, const typename ST::T* __restrict__ zz,
     const typename ST::T* __restrict__ xy, const typename ST::T* __restrict__ xz,
     const typename ST::T* __restrict__ yz, const float* __restrict__ dcrjx,
@@ -103,8 +108,9 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     typename ST::T* U = RP(u1, i, 0, 0);
     typename ST::T* V = RP(v1, i, 0, 0);
     typename ST::T* W = RP(w1, i, 0, 0);
-    const float u = ST::ldb(U, kw, SC.s[AWP_SU]), v = ST::ldb(V, kw, SC.s[AWP_SV]),
-                w = ST::ldb(W, kw, SC.s[AWP_SW]);
+    const float u = ST::ldb(OOP ? RP(u0, i, 0, 0) : U, kw, SC.s[AWP_SU]),
+                v = ST::ldb(OOP ? RP(v0, i, 0, 0) : V, kw, SC.s[AWP_SV]),
+                w = ST::ldb(OOP ? RP(w0, i, 0, 0) : W, kw, SC.s[AWP_SW]);
 
     const float dcrj = dcrjx[i] * dy * dz;
     float f_d1 = 0.25f * (dc + djm + dkm + djk);
@@ -134,25 +140,33 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
 }
 
 template <int BJ, class ST>
-void launch_vel(void* const* f9, float* dcrjx, float* dcrjy, float* dcrjz, float* d_1,
-                hipStream_t St, VelLaunch L, const WaveScales& SC) {
+void launch_vel(void* const* f9, void* const* v0, float* dcrjx, float* dcrjy, float* dcrjz,
+                float* d_1, hipStream_t St, VelLaunch L, const WaveScales& SC) {
   typedef typename ST::T T;
   const int nkj = L.nkt * L.njt;
   int nic, ichunk;
   awp_split_i(L.ni, nkj, awp_tuning().vel_ichunk, awp_tuning().vel_target_wgs, &nic, &ichunk);
   L.ichunk = ichunk;
   L.nblocks = nkj * nic;
-  hipLaunchKernelGGL((dvelcx_kernel<BJ, ST>), dim3(L.nblocks), dim3(64, BJ), 0, St, (T*)f9[0],
-                     (T*)f9[1], (T*)f9[2], (const T*)f9[3], (const T*)f9[4], (const T*)f9[5],
-                     (const T*)f9[6], (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz, d_1,
-                     L, SC);
+  if (v0)
+    hipLaunchKernelGGL((dvelcx_kernel<BJ, true, ST>), dim3(L.nblocks), dim3(64, BJ), 0, St,
+                       (T*)f9[0], (T*)f9[1], (T*)f9[2], (const T*)v0[0], (const T*)v0[1],
+                       (const T*)v0[2], (const T*)f9[3], (const T*)f9[4], (const T*)f9[5],
+                       (const T*)f9[6], (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz,
+                       d_1, L, SC);
+  else
+    hipLaunchKernelGGL((dvelcx_kernel<BJ, false, ST>), dim3(L.nblocks), dim3(64, BJ), 0, St,
+                       (T*)f9[0], (T*)f9[1], (T*)f9[2], nullptr, nullptr, nullptr,
+                       (const T*)f9[3], (const T*)f9[4], (const T*)f9[5], (const T*)f9[6],
+                       (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz, d_1, L, SC);
 }
 
-// f9: u1 v1 w1 xx yy zz xy xz yz in the storage type of ST.
+// f9: u1 v1 w1 xx yy zz xy xz yz in the storage type of ST; v0: null (u1 v1
+// w1 updated in place) or the velocities before the step (u0 v0 w0).
 template <class ST>
-void dvelcx_common(const char* W, void* const* f9, float* dcrjx, float* dcrjy, float* dcrjz,
-                   float* d_1, int nyt, int nzt, hipStream_t St, int s_i, int e_i, int d_i,
-                   int ngrids, const WaveScales& SC) {
+void dvelcx_common(const char* W, void* const* f9, void* const* v0, float* dcrjx, float* dcrjy,
+                   float* dcrjz, float* d_1, int nyt, int nzt, hipStream_t St, int s_i, int e_i,
+                   int d_i, int ngrids, const WaveScales& SC) {
   if (!awp_grid_ok(d_i, W)) return;
   const AwpGrid& g = awp_state().grid[d_i];
   if (nyt != g.nyt || nzt != g.nzt) {
@@ -188,11 +202,11 @@ void dvelcx_common(const char* W, void* const* f9, float* dcrjx, float* dcrjy, f
   L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
   if (L.k1 < L.k0) return;
   switch (bj) {
-    case 1: launch_vel<1, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 2: launch_vel<2, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 8: launch_vel<8, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 16: launch_vel<16, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    default: launch_vel<4, ST>(f9, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 1: launch_vel<1, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 2: launch_vel<2, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 8: launch_vel<8, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 16: launch_vel<16, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    default: launch_vel<4, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
   }
   awp_check_launch(W);
 }
@@ -205,8 +219,8 @@ extern "C" void dvelcx_H_opt(float* u1, float* v1, float* w1, float* xx, float* 
                              int s_i, int e_i, int d_i, int ngrids) {
   void* f9[9] = {u1, v1, w1, xx, yy, zz, xy, xz, yz};
   WaveScales SC = {};
-  dvelcx_common<StF32>("dvelcx_H_opt", f9, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i,
-                       d_i, ngrids, SC);
+  dvelcx_common<StF32>("dvelcx_H_opt", f9, nullptr, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St,
+                       s_i, e_i, d_i, ngrids, SC);
 }
 
 extern "C" void awp_dvelcx_s16(uint16_t* u1, uint16_t* v1, uint16_t* w1, uint16_t* xx,
@@ -219,9 +233,27 @@ extern "C" void awp_dvelcx_s16(uint16_t* u1, uint16_t* v1, uint16_t* w1, uint16_
   if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
   void* f9[9] = {u1, v1, w1, xx, yy, zz, xy, xz, yz};
   if (fmt == AWP_FMT_BF16)
-    dvelcx_common<StBF16>(W, f9, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i, ngrids,
-                          SC);
+    dvelcx_common<StBF16>(W, f9, nullptr, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i,
+                          ngrids, SC);
   else
-    dvelcx_common<StF16>(W, f9, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i, ngrids,
-                         SC);
+    dvelcx_common<StF16>(W, f9, nullptr, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i,
+                         ngrids, SC);
+}
+
+extern "C" void awp_dvelcx_oop(float* u1, float* v1, float* w1, const float* u0,
+                               const float* v0, const float* w0, float* xx, float* yy,
+                               float* zz, float* xy, float* xz, float* yz, float* dcrjx,
+                               float* dcrjy, float* dcrjz, float* d_1, int nyt, int nzt,
+                               hipStream_t St, int s_i, int e_i, int d_i, int ngrids) {
+  const char* W = "awp_dvelcx_oop";
+  if (u1 == u0 || v1 == v0 || w1 == w0) {
+    awp_set_error(AWP_ERR_ARG, "%s: the velocities before and after the step must be distinct "
+                  "arrays (dvelcx_H_opt updates in place)", W);
+    return;
+  }
+  void* f9[9] = {u1, v1, w1, xx, yy, zz, xy, xz, yz};
+  void* vin[3] = {(void*)u0, (void*)v0, (void*)w0};
+  WaveScales SC = {};
+  dvelcx_common<StF32>(W, f9, vin, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i, ngrids,
+                       SC);
 }

@@ -246,6 +246,27 @@ void awp_adjoint_map(const double* const* gcoef, const double* grad_lam_mu,
                      const float* dcrjx, const float* dcrjy, const float* dcrjz, double* glam,
                      double* gmu, double* grho, hipStream_t St);
 
+/* Out-of-place forms of dvelcx_H_opt and dstrqc_H (production Q weights) for
+ * the adjoint's segment recompute, which keeps every state of a checkpoint
+ * segment: the field before the step is read from u0 v0 w0 (s0[6]: xx yy zz
+ * xy xz yz) and the updated field written to u1 v1 w1 (xx..yz), which must be
+ * distinct arrays; memory variables r1..r6 are updated in place.  The free-
+ * surface velocity ghosts are written into u1 v1 w1 as dstrqc_H does.  Every
+ * cell the in-place launcher would leave untouched is left untouched in the
+ * output array, so outputs whose untouched cells hold the input's values give
+ * the in-place results bit for bit. */
+void awp_dvelcx_oop(float* u1, float* v1, float* w1, const float* u0, const float* v0,
+                    const float* w0, float* xx, float* yy, float* zz, float* xy, float* xz,
+                    float* yz, float* dcrjx, float* dcrjy, float* dcrjz, float* d_1, int nyt,
+                    int nzt, hipStream_t St, int s_i, int e_i, int d_i, int ngrids);
+void awp_dstrqc_oop(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                    const float* const* s0, float* r1, float* r2, float* r3, float* r4,
+                    float* r5, float* r6, float* u1, float* v1, float* w1, float* lam, float* mu,
+                    float* qp, float* coeff, float* qs, float* dcrjx, float* dcrjy,
+                    float* dcrjz, int nyt, int nzt, hipStream_t St, float* lam_mu, float* vx1,
+                    float* vx2, int* ww, float* wwo, int NX, int NPC, int rankx, int ranky,
+                    int s_i, int e_i, int s_j, int e_j, int d_i);
+
 /* ---- Curvilinear topography (grid 0) -------------------------------------
  * The reference passes a topo_t (include/topography/topography.h:136-273) to
  * its launchers; awp_topo_t keeps the members those launchers read, under

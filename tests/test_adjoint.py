@@ -147,3 +147,39 @@ def test_gpu_gradient_independent_of_checkpoint_interval(gpu, case):
     for g in out[1:]:
         for a, b in zip(g, out[0]):
             assert _rel(a, b) < 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_out_of_place_recompute_matches_in_place(gpu, case):
+    """The segment recompute (awp_dvelcx_oop / awp_dstrqc_oop into the ring
+    of states) reproduces the in-place forward (dvelcx_H_opt, dstrqc_H) bit
+    for bit: velocities and stresses of every step, memory variables after
+    the segment; one segment from the start, one from a mid-run checkpoint."""
+    from awp_amd.adjoint import STRESS, VEL
+    from awp_amd.solver import FIELDS
+    K = 13
+    P = _gpu_problem(case, K)
+    P.set_medium(*_arrays(case[0]))
+    P.forward()
+    s = P.solver
+    R = ("r1", "r2", "r3", "r4", "r5", "r6")
+    for n0 in (0, 26):
+        for x, t in zip(FIELDS, P.checkpoints[n0]):
+            s.f[x].copy_(t)
+        s.cur_step = n0
+        ref = []
+        for _ in range(K):
+            s.step()
+            ref.append({x: s.f[x].clone() for x in VEL + STRESS})
+        ref_r = {x: s.f[x].clone() for x in R}
+        for x, t in zip(FIELDS, P.checkpoints[n0]):
+            (P.ring[0][x] if x in P.ring[0] else s.f[x]).copy_(t)
+        s.cur_step = n0
+        for n in range(K):
+            P._recompute_step(P.ring[n], P.ring[n + 1])
+        torch.cuda.synchronize()
+        for n in range(K):
+            for x in VEL + STRESS:
+                assert torch.equal(P.ring[n + 1][x], ref[n][x]), (n0, n, x)
+        for x in R:
+            assert torch.equal(s.f[x], ref_r[x]), (n0, x)
