@@ -49,23 +49,61 @@ __global__ __launch_bounds__(256) void decode_kernel(const uint16_t* __restrict_
     x[i] = ST::dec(q[i], s);
 }
 
+// Eight 16-bit values per lane and 16-byte access where the array allows it
+// (n8 > 0 only for a 16-byte aligned q); the remainder one value per lane.
+// q <- q * 2^shift, rounded to the format: the scaling is v_ldexp_f32, exact
+// like a multiply by 2^shift (flushes included) and never fused with the
+// binary16 conversion into a v_fma_mix* (which would flush binary16
+// subnormals).
 template <class ST>
-__global__ __launch_bounds__(256) void rescale_kernel(uint16_t* __restrict__ q, long long n,
-                                                      float factor) {
-  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-    q[i] = ST::enc(ST::dec(q[i], 1.0f) * factor, 1.0f);
+__device__ __forceinline__ uint16_t rescale1(uint16_t q, int shift) {
+  return ST::enc(__builtin_ldexpf(ST::dec(q, 1.0f), shift), 1.0f);
 }
 
-// Largest |decoded value| (unit scale) into *out: per-block maximum over a
-// grid-stride range, then one vector atomic max per block on the bit
-// pattern (non-negative binary32 values order as unsigned integers).
+template <class ST>
+__global__ __launch_bounds__(256) void rescale_kernel(uint16_t* __restrict__ q, long long n,
+                                                      long long n8, int shift) {
+  const long long stride = (long long)gridDim.x * 256;
+  uint4* q8 = reinterpret_cast<uint4*>(q);
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n8; i += stride) {
+    uint4 v = q8[i];
+    unsigned* w = &v.x;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const unsigned lo = rescale1<ST>((uint16_t)(w[c] & 0xffffu), shift);
+      const unsigned hi = rescale1<ST>((uint16_t)(w[c] >> 16), shift);
+      w[c] = lo | (hi << 16);
+    }
+    q8[i] = v;
+  }
+  for (long long i = n8 * 8 + blockIdx.x * 256ll + threadIdx.x; i < n; i += stride)
+    q[i] = rescale1<ST>(q[i], shift);
+}
+
+// Largest |decoded value| (unit scale) into *out.  Both formats order
+// non-negative values as unsigned integers (NaN patterns above infinity), so
+// the pass takes the largest 15-bit magnitude pattern, decodes that one value,
+// and issues one vector atomic max per block on its binary32 bit pattern.
 template <class ST>
 __global__ __launch_bounds__(256) void absmax_kernel(const uint16_t* __restrict__ q,
-                                                     long long n, unsigned* __restrict__ out) {
+                                                     long long n, long long n8,
+                                                     unsigned* __restrict__ out) {
   __shared__ unsigned red[4];
+  const long long stride = (long long)gridDim.x * 256;
+  const uint4* q8 = reinterpret_cast<const uint4*>(q);
   unsigned m = 0u;
-  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
-    const unsigned b = __float_as_uint(fabsf(ST::dec(q[i], 1.0f)));
+  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n8; i += stride) {
+    const uint4 v = q8[i];
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const unsigned lo = w[c] & 0x7fffu, hi = (w[c] >> 16) & 0x7fffu;
+      m = lo > m ? lo : m;
+      m = hi > m ? hi : m;
+    }
+  }
+  for (long long i = n8 * 8 + blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
+    const unsigned b = q[i] & 0x7fffu;
     m = b > m ? b : m;
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -78,8 +116,12 @@ __global__ __launch_bounds__(256) void absmax_kernel(const uint16_t* __restrict_
   if (threadIdx.x == 0) {
     unsigned r = red[0];
     for (int w = 1; w < 4; ++w) r = red[w] > r ? red[w] : r;
-    atomicMax(out, r);
+    atomicMax(out, __float_as_uint(ST::dec((uint16_t)r, 1.0f)));
   }
+}
+
+long long vec8_count(const void* q, long long n) {
+  return ((uintptr_t)q & 15u) == 0 ? n / 8 : 0;
 }
 
 int grid_for(long long n) {
@@ -138,11 +180,12 @@ void awp_s16_rescale(uint16_t* q, long long n, int fmt, int shift, hipStream_t S
     awp_set_error(AWP_ERR_ARG, "%s: shift %d outside [-60, 60]", W, shift);
     return;
   }
-  const float factor = ldexpf(1.0f, shift);
   if (fmt == AWP_FMT_BF16)
-    hipLaunchKernelGGL(rescale_kernel<StBF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n, factor);
+    hipLaunchKernelGGL(rescale_kernel<StBF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n,
+                       vec8_count(q, n), shift);
   else
-    hipLaunchKernelGGL(rescale_kernel<StF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n, factor);
+    hipLaunchKernelGGL(rescale_kernel<StF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n,
+                       vec8_count(q, n), shift);
   awp_check_launch(W);
 }
 
@@ -160,10 +203,10 @@ void awp_s16_absmax(const uint16_t* q, long long n, int fmt, float* out, hipStre
   if (n <= 0) return;
   if (fmt == AWP_FMT_BF16)
     hipLaunchKernelGGL(absmax_kernel<StBF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n,
-                       (unsigned*)out);
+                       vec8_count(q, n), (unsigned*)out);
   else
     hipLaunchKernelGGL(absmax_kernel<StF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n,
-                       (unsigned*)out);
+                       vec8_count(q, n), (unsigned*)out);
   awp_check_launch(W);
 }
 

@@ -28,21 +28,15 @@ __host__ __device__ __forceinline__ float awp_bf16_to_f32(uint16_t q) {
   return __builtin_bit_cast(float, (uint32_t)q << 16);
 }
 
-// binary32 <-> binary16 with the hardware conversions (round to nearest even;
-// binary16 subnormals kept, as the binary16 denormal mode is on), each issued
-// as an instruction of its own.  Left to the compiler, a conversion fuses
-// with the neighbouring scale multiply into v_fma_mix*, which follows the
-// binary32 denormal mode (flush) and so loses binary16 subnormals.
+// binary32 <-> binary16 with the hardware conversions (v_cvt_f16_f32 rounds
+// to nearest even and keeps binary16 subnormals, as the binary16 denormal
+// mode is on).
 __device__ __forceinline__ uint16_t awp_f32_to_f16(float x) {
-  uint32_t r;
-  asm("v_cvt_f16_f32 %0, %1" : "=v"(r) : "v"(x));
-  return (uint16_t)r;
+  return __builtin_bit_cast(uint16_t, (_Float16)x);
 }
 
 __device__ __forceinline__ float awp_f16_to_f32(uint16_t q) {
-  float x;
-  asm("v_cvt_f32_f16 %0, %1" : "=v"(x) : "v"((uint32_t)q));
-  return x;
+  return (float)__builtin_bit_cast(_Float16, q);
 }
 
 // Element access by index (ld/st) or by a uniform row pointer plus a 32-bit
@@ -75,8 +69,12 @@ template <class Codec>
 struct St16 {
   typedef uint16_t T;
   static constexpr int fmt = Codec::fmt;
-  __device__ static __forceinline__ float dec(T q, float s) { return Codec::to_f32(q) * s; }
-  __device__ static __forceinline__ T enc(float x, float inv) { return Codec::from_f32(x * inv); }
+  __device__ static __forceinline__ float dec(T q, float s) {
+    return Codec::scale(Codec::to_f32(q), s);
+  }
+  __device__ static __forceinline__ T enc(float x, float inv) {
+    return Codec::from_f32(Codec::scale(x, inv));
+  }
   __device__ static __forceinline__ float ld(const T* a, long long i, float s) {
     return dec(a[i], s);
   }
@@ -96,13 +94,21 @@ struct St16 {
 // computed with binary32 denormals flushed, so they are never subnormal.
 struct CodecBF16 {
   static constexpr int fmt = AWP_FMT_BF16;
+  __device__ static __forceinline__ float scale(float x, float s) { return x * s; }
   __device__ static __forceinline__ float to_f32(uint16_t q) { return awp_bf16_to_f32(q); }
   __device__ static __forceinline__ uint16_t from_f32(float x) {
     return __builtin_bit_cast(uint16_t, (__bf16)x);
   }
 };
+// binary16 scales by v_ldexp_f32 with the exponent of the power-of-two s
+// (the same value as x * s, flushes included).  A multiply beside the
+// conversion would be fused into v_fma_mix*, which follows the binary32
+// denormal mode and so loses binary16 subnormals; ldexp has no mix form.
 struct CodecF16 {
   static constexpr int fmt = AWP_FMT_F16;
+  __device__ static __forceinline__ float scale(float x, float s) {
+    return __builtin_ldexpf(x, __builtin_amdgcn_frexp_expf(s) - 1);
+  }
   __device__ static __forceinline__ float to_f32(uint16_t q) { return awp_f16_to_f32(q); }
   __device__ static __forceinline__ uint16_t from_f32(float x) { return awp_f32_to_f16(x); }
 };

@@ -45,9 +45,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=512)
-    ap.add_argument("--ny", type=int, default=512)
-    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--nx", type=int, default=None,
+                    help="cells along x (default per workload: anelastic 512, topo 384, "
+                    "adjoint 256, dm coarse grid 256)")
+    ap.add_argument("--ny", type=int, default=None)
+    ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -112,6 +114,11 @@ def _cpu_model() -> str:
     return "unknown"
 
 
+def _size(args, nx: int, ny: int, nz: int) -> tuple:
+    """The workload's grid: --nx/--ny/--nz where given, else its defaults."""
+    return (args.nx or nx, args.ny or ny, args.nz or nz)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -140,6 +147,7 @@ def main():
     if args.workload == "topo":
         bench_topo(args)
         return
+    args.nx, args.ny, args.nz = _size(args, 512, 512, 256)
 
     def workload(nx, ny):
         return configs.anelastic(nx=nx, ny=ny, nz=args.nz, nt=max(args.steps + args.warmup, 1))
@@ -291,9 +299,7 @@ def bench_adjoint(args) -> None:
     from awp_amd.adjoint import AdjointProblem
     from awp_amd.solver import Solver
 
-    nx = args.nx if args.nx != 512 else 256
-    ny = args.ny if args.ny != 512 else 256
-    nz = args.nz if args.nz != 256 else 128
+    nx, ny, nz = _size(args, 256, 256, 128)
     nt = args.nt
     t0 = time.perf_counter()
     bg, true, src, rec = configs.adjoint_pair(nx=nx, ny=ny, nz=nz, nt=nt)
@@ -369,9 +375,7 @@ def bench_topo(args) -> None:
     from awp_amd import _lib, configs
     from awp_amd.solver import Solver, Solver16
 
-    nx = args.nx if args.nx != 512 else 384
-    ny = args.ny if args.ny != 512 else 384
-    nz = args.nz if args.nz != 256 else 192
+    nx, ny, nz = _size(args, 384, 384, 192)
     nt = max(args.steps + args.warmup, 1)
     w = configs.topography_hills(nx=nx, ny=ny, nz=nz, nt=nt)
     topo = w.params["topography"]
@@ -467,9 +471,7 @@ def bench_dm(args) -> None:
     from awp_amd import _lib, configs
     from awp_amd.solver import Solver
 
-    nx = args.nx if args.nx != 512 else 256
-    ny = args.ny if args.ny != 512 else 256
-    nz = args.nz if args.nz != 256 else 192
+    nx, ny, nz = _size(args, 256, 256, 192)
     nt = max(args.steps + args.warmup, 1)
     w = configs.dm_plastic(nx=nx, ny=ny, nz=nz, nz_fine=args.nz_fine, nt=nt)
     t0 = time.perf_counter()

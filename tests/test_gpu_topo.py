@@ -137,3 +137,36 @@ def test_gpu_topography_16bit_bit_exact(gpu, oracle_lib, storage):
     assert np.array_equal(g.traces(), o.traces())
     for n in FIELDS16:
         assert np.array_equal(g.raw(n), o.levels[0].q[n]), n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage,tol", [("f32", 5e-5), ("bf16", 0.1), ("fp16", 2e-2)])
+def test_gpu_topography_workload_against_fp64(gpu, storage, tol):
+    """The full topography workload (384x384x192, six hills, 1500 steps) on
+    the GPU against the FP64 oracle's receiver traces
+    (golden/topography_384x384x192_fp64_traces.npz, written by
+    golden/fp64_topography.py).  Stated tolerances on the relative L2 error
+    over all receivers, components and samples: FP32 5e-5 (measured 1.4e-5),
+    BF16 0.1 (7.2e-2), FP16 2e-2 (8.8e-3)."""
+    from pathlib import Path
+    from awp_amd.solver import Solver, Solver16
+    torch, _lib = gpu
+    ref = np.load(Path(__file__).parent / "golden" / "topography_384x384x192_fp64_traces.npz")
+    ref = ref["traces"]
+    nt = ref.shape[0] - 1
+    w = configs.topography_hills(nx=384, ny=384, nz=192, nt=nt)
+    med = w.medium()
+    topo = w.params["topography"]
+    if storage == "f32":
+        s = Solver(med, w.sources, w.receivers, nt_record=nt + 1, topography=topo)
+    else:
+        s = Solver16(med, w.sources, w.receivers, nt_record=nt + 1, storage=storage,
+                     topography=topo)
+    del med
+    s.run(nt)
+    got = s.traces().astype(np.float64)
+    del s
+    torch.cuda.empty_cache()
+    err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"{storage}: receiver rel-L2 vs FP64 oracle {err:.3e}")
+    assert err < tol

@@ -182,3 +182,35 @@ def test_gpu_16bit_run_bit_exact(gpu, oracle_lib, storage):
         a, b = g.raw(n), o.levels[0].q[n]
         bad = np.argwhere(a != b)
         assert len(bad) == 0, (n, len(bad), tuple(bad[0]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fmt", [1, 2], ids=["bf16", "fp16"])
+def test_gpu_absmax_and_rescale(gpu, oracle_lib, fmt):
+    """awp_s16_absmax and awp_s16_rescale against the host codecs: ragged
+    lengths, 16-byte aligned and unaligned starts (vector and scalar paths),
+    a NaN anywhere."""
+    torch, lib, _lib = gpu
+    rng = np.random.default_rng(5)
+    st = torch.cuda.current_stream().cuda_stream
+    out = torch.zeros(1, dtype=torch.float32, device="cuda")
+    for n, off in ((1, 0), (7, 0), (8, 0), (1000003, 0), (1000003, 1), (4096, 3)):
+        m = n + off
+        x = (rng.standard_normal(m) * 10.0 ** rng.uniform(-3, 3, m)).astype(np.float32)
+        q = _enc(oracle_lib, x, fmt)
+        qd = torch.from_numpy(q.view(np.int16).copy()).cuda()
+        lib.awp_s16_absmax(qd.data_ptr() + 2 * off, n, fmt, out.data_ptr(), st)
+        _lib.check()
+        assert out.item() == float(np.abs(_dec(oracle_lib, q[off:], fmt)).max()), (n, off)
+        lib.awp_s16_rescale(qd.data_ptr() + 2 * off, n, fmt, -3, st)
+        _lib.check()
+        want = q.copy()
+        want[off:] = _enc(oracle_lib, _dec(oracle_lib, q[off:], fmt) * np.float32(2.0 ** -3), fmt)
+        assert np.array_equal(qd.cpu().numpy().view(np.uint16), want), (n, off)
+        if n > 5:
+            qn = q.copy()
+            qn[off + n - 2] = 0x7FC0 if fmt == 1 else 0x7E00
+            qd = torch.from_numpy(qn.view(np.int16).copy()).cuda()
+            lib.awp_s16_absmax(qd.data_ptr() + 2 * off, n, fmt, out.data_ptr(), st)
+            _lib.check()
+            assert math.isnan(out.item()), (n, off)
