@@ -92,8 +92,7 @@ __global__ __launch_bounds__(256) void absmax_kernel(const uint16_t* __restrict_
   const long long stride = (long long)gridDim.x * 256;
   const uint4* q8 = reinterpret_cast<const uint4*>(q);
   unsigned m = 0u;
-  for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n8; i += stride) {
-    const uint4 v = q8[i];
+  auto fold = [&m](const uint4 v) {
     const unsigned w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -101,7 +100,17 @@ __global__ __launch_bounds__(256) void absmax_kernel(const uint16_t* __restrict_
       m = lo > m ? lo : m;
       m = hi > m ? hi : m;
     }
+  };
+  // four independent 16-byte loads in flight per lane
+  long long i = blockIdx.x * 256ll + threadIdx.x;
+  for (; i + 3 * stride < n8; i += 4 * stride) {
+    const uint4 a = q8[i], b = q8[i + stride], c = q8[i + 2 * stride], d = q8[i + 3 * stride];
+    fold(a);
+    fold(b);
+    fold(c);
+    fold(d);
   }
+  for (; i < n8; i += stride) fold(q8[i]);
   for (long long i = n8 * 8 + blockIdx.x * 256ll + threadIdx.x; i < n; i += stride) {
     const unsigned b = q[i] & 0x7fffu;
     m = b > m ? b : m;

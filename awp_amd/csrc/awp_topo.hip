@@ -516,9 +516,15 @@ enum {
   C_WN, C_WO, C_WM1, C_WP1, C_WP2    // w1 at (i+1, j), (i, j), (i, j-1), (i, j+1), (i, j+2)
 };
 
-template <class ST>
+// RATES: store the nine strain-rate terms of every output cell into R
+// (nine arrays of the field's size, rstride floats apart) for
+// topo_str_q_kernel, instead of applying the material and memory-variable
+// update here.
+template <class ST, bool RATES>
 __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<ST> A,
-                                                              const TGeom G, const VelTiles V) {
+                                                              const TGeom G, const VelTiles V,
+                                                              float* __restrict__ R,
+                                                              long long rstride) {
   __shared__ float lds[2][ST_BJ][ST_NC][64];
   const int lane = (int)threadIdx.x;
   const int w = (int)threadIdx.y;
@@ -747,9 +753,42 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
       acc = acc + P4F(3) * M.f2_c[q + 2] * zw_p2;
       const float yzv2 = D4F(1) * wo + D4F(0) * wm1 + D4F(2) * wp1 + D4F(3) * wp2 -
                          J23i * g_k * acc;
-      topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2, yzv1,
-                          yzv2);
+      if (RATES) {
+        const float r9[9] = {vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2, yzv1, yzv2};
+#pragma unroll
+        for (int c = 0; c < 9; ++c) R[c * rstride + pos] = r9[c];
+      } else {
+        topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2,
+                            yzv1, yzv2);
+      }
     }
+  }
+}
+
+// The material averages, Q weights and memory-variable recursion of the
+// stress update (topo_str_update) over the cells topo_str_kernel<ST, true>
+// wrote rates for: levels 3 <= k < nz of the stress bounds.  One lane per
+// level, 4 rows per workgroup, a march over a chunk of i; few registers, so
+// the per-cell loads of this streaming pass stay in flight.
+template <class ST>
+__global__ __launch_bounds__(256) void topo_str_q_kernel(const TopoStrArgs<ST> A, const TGeom G,
+                                                         const float* __restrict__ R,
+                                                         long long rstride, int ichunk) {
+  const int k = 3 + (int)(blockIdx.x * 64 + threadIdx.x);  // topographic level
+  const int j = G.bj + (int)(blockIdx.y * 4 + threadIdx.y);
+  const int ib = G.bi + (int)blockIdx.z * ichunk;
+  const int ie = ib + ichunk < G.ei ? ib + ichunk : G.ei;
+  if (k >= G.nz || j >= G.ej) return;
+  const int sl = G.sl, yl = G.yl;
+  const int K = k + AWP_ALIGN;
+  const float dcz = A.dcrjz[K];
+  const float dcy = A.dcrjy[j];
+  for (int i = ib; i < ie; ++i) {
+    const int pos = yl * j + K + sl * i;
+    const float f_dcrj = A.dcrjx[i] * dcy * dcz;
+    topo_str_update<ST>(A, pos, sl, yl, f_dcrj, R[pos], R[rstride + pos], R[2 * rstride + pos],
+                        R[3 * rstride + pos], R[4 * rstride + pos], R[5 * rstride + pos],
+                        R[6 * rstride + pos], R[7 * rstride + pos], R[8 * rstride + pos]);
   }
 }
 
@@ -1047,7 +1086,20 @@ void topo_stress_launch(awp_topo_t* T, const WaveScales& W, const char* where) {
   V.ichunk = awp_tuning().topo_str_ichunk;
   dim3 block(64, ST_BJ, 1);
   dim3 grid(V.nint + 1, awp_cdiv(G.ej - G.bj, ST_BJ), awp_cdiv(G.ei - G.bi, V.ichunk));
-  hipLaunchKernelGGL(topo_str_kernel<ST>, grid, block, 0, st, A, G, V);
+  if (!T->rates) {
+    hipLaunchKernelGGL((topo_str_kernel<ST, false>), grid, block, 0, st, A, G, V, nullptr, 0ll);
+    awp_check_launch(where);
+    return;
+  }
+  // two passes: strain rates into T->rates, then the material and memory-
+  // variable update streaming them back
+  const long long rstride = (long long)G.sl * (T->nx + 4 + AWP_NGSL2);
+  hipLaunchKernelGGL((topo_str_kernel<ST, true>), grid, block, 0, st, A, G, V, T->rates,
+                     rstride);
+  const int qchunk = awp_tuning().topo_q_ichunk;
+  dim3 gq(awp_cdiv(T->nz - 3, 64), awp_cdiv(G.ej - G.bj, 4), awp_cdiv(G.ei - G.bi, qchunk));
+  hipLaunchKernelGGL(topo_str_q_kernel<ST>, gq, dim3(64, 4), 0, st, A, G, T->rates, rstride,
+                     qchunk);
   awp_check_launch(where);
 }
 

@@ -12,6 +12,7 @@ struct AwpTuning {
   int kalign;       // 1: start k tiles on a 32-float (128 B) line boundary
   int topo_vel_ichunk;  // planes per march of the curvilinear velocity kernel
   int topo_str_ichunk;  // same, curvilinear stress kernel
+  int topo_q_ichunk;    // planes per march of the two-pass stress update
 };
 
 AwpTuning& awp_tuning();

@@ -135,7 +135,7 @@ class AwpTopo(C.Structure):
                     "v1", "w1", "dcrjx", "dcrjy", "dcrjz")]
                 + [("metrics_f", AwpFGrid), ("metrics_g", AwpGGrid), ("stream_i", C.c_void_p),
                    ("c_DT", C.c_float), ("c_dth", C.c_float), ("c_dh1", C.c_float),
-                   ("constants_set", C.c_int)])
+                   ("constants_set", C.c_int), ("rates", C.c_void_p)])
 
 
 # topo_t member <- (level dict, key)
@@ -151,12 +151,16 @@ class TopoDevice:
     """Metric arrays in HBM, built by the library's setup kernels, and the
     awp_topo_t of one grid-0 level (level.f / level.m tensors)."""
 
-    def __init__(self, lib, level, topo: Topography, device, stream: int):
+    def __init__(self, lib, level, topo: Topography, device, stream: int,
+                 two_pass: bool = True):
         import torch
         d = level.dims
         topo.check(d.nxt, d.nyt)
         self.lib = lib
         self.level = level
+        # scratch of the two-pass stress update: nine strain-rate arrays
+        self.rates = (torch.empty((9,) + tuple(d.shape), dtype=torch.float32, device=device)
+                      if two_pass else None)
         f32 = dict(dtype=torch.float32, device=device)
         self.mf = {n: torch.zeros(f_shape(d.nxt, d.nyt), **f32) for n in F_NAMES}
         self.mg = {n: torch.zeros(g_size(d.nzt), **f32) for n in G_NAMES}
@@ -179,6 +183,7 @@ class TopoDevice:
         T.gridspacing = DH
         T.metrics_f = self.fg
         T.metrics_g = self.gg
+        T.rates = self.rates.data_ptr() if self.rates is not None else None
         self.bind()
         lib.topo_set_constants(C.byref(T))
 

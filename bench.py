@@ -54,12 +54,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps with no event timing (for rocprofv3)")
-    ap.add_argument("--workload", choices=("anelastic", "dm", "adjoint", "topo"),
+    ap.add_argument("--workload", choices=("anelastic", "dm", "adjoint", "topo", "cpu"),
                     default="anelastic",
                     help="anelastic: the headline single-grid workload; dm: discontinuous mesh "
                          "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz); "
                          "adjoint: gradient cost and finite-difference check; topo: curvilinear "
-                         "topography (384 x 384 x 192 unless --nx/--ny/--nz say otherwise)")
+                         "topography (384 x 384 x 192 unless --nx/--ny/--nz say otherwise); "
+                         "cpu: the CPU baseline table only (no GPU)")
     ap.add_argument("--save-traces", default="",
                     help="write the receiver traces (numpy .npy) of the timed run here")
     ap.add_argument("--nz-fine", type=int, default=96)
@@ -104,6 +105,48 @@ def cpu_baseline(workload_fn, target_s: float) -> dict:
                       f"cpu={_cpu_model()}"}
 
 
+def bench_cpu(args) -> None:
+    """CPU baseline table: the oracle (the reference's kernel loops restated in
+    C++ with OpenMP) on this host's cores, at 1 thread and at all threads
+    (AWP_CPU_THREADS, else at most 16), in FP32 and FP64.  The elastic 64^3
+    workload runs its 200 steps; the 512 x 512 x 256 anelastic workload runs
+    until --cpu-seconds or 10 steps, whichever comes first.  One untimed step
+    first touches the arrays."""
+    import numpy as np
+
+    from awp_amd import configs
+    from oracle.oracle import OracleSolver, set_threads
+    allt = int(os.environ.get("AWP_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    rows = []
+    cases = (("elastic_64x64x64", configs.elastic_halfspace(n=64, nt=201), 200, None),
+             ("anelastic_512x512x256", configs.anelastic(nt=11), 10, args.cpu_seconds))
+    for name, w, nmax, tmax in cases:
+        med = w.medium()
+        for dtype in (np.float32, np.float64):
+            for th in sorted({1, allt}):
+                set_threads(th)
+                s = OracleSolver(med, w.sources, None, dtype=dtype)
+                s.start()
+                s.step()
+                t0 = time.perf_counter()
+                n = 0
+                while n < nmax:
+                    s.step()
+                    n += 1
+                    el = time.perf_counter() - t0
+                    if tmax and el >= tmax:
+                        break
+                del s
+                row = {"workload": name, "dtype": np.dtype(dtype).name, "threads": th,
+                       "steps": n, "seconds": round(el, 3),
+                       "cell_updates_per_s": w.dims.ncells * n / el}
+                rows.append(row)
+                print(json.dumps(row), file=sys.stderr, flush=True)
+        del med
+    print(json.dumps({"metric": "CPU baseline (oracle on host cores)", "unit": "cell-updates/s",
+                      "cpu": _cpu_model(), "host_threads": os.cpu_count(), "rows": rows}))
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -126,6 +169,9 @@ def main():
                           "it (the path would shard as 2-D x/y slabs with an 8-cell velocity "
                           "halo exchange per step)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "cpu":
+        bench_cpu(args)
+        return
     import numpy as np
     import torch
 

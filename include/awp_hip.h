@@ -37,6 +37,14 @@ int awp_last_error(void);
 const char* awp_last_error_string(void);
 void awp_clear_error(void);
 
+/* Launch-shape knobs (no reference counterpart), process-global; results do
+ * not depend on them.  Keys: vel_bj, str_bj (rows per workgroup: 1, 2, 4, 8,
+ * 16), vel_target_wgs, str_target_wgs, vel_ichunk, str_ichunk (i-march
+ * chunking, 0 = automatic), xcd_remap, kalign (0/1), topo_vel_ichunk,
+ * topo_str_ichunk, topo_q_ichunk (planes per march, 1..4096).  Returns 0, or
+ * -1 for an unknown key or a value out of range. */
+int awp_set_tuning(const char* key, int value);
+
 /* Replaces SetDeviceConstValue, include/awp/kernel_launch.h:24-26
  * (src/awp/kernel.cu:68-117).  Stores per-grid DT/DH-derived constants and
  * strides, plus the plasticity rotation (fmajor, fminor, Rz, RzT). */
@@ -308,6 +316,11 @@ typedef struct {
    * memory, src/topography/kernels/stress_attenuation.cu:24-72) */
   float c_DT, c_dth, c_dh1;
   int constants_set;
+  /* no reference counterpart: optional caller-owned scratch of 9 arrays of
+   * the 3-D field size, back to back.  Non-null, the stress launchers run in
+   * two passes, the strain rates into rates and then the material and memory-
+   * variable update; results are the same bit for bit.  Null: one pass. */
+  float* rates;
 } awp_topo_t;
 
 /* topo_set_constants, src/topography/stress_attenuation.cu:19-23 */

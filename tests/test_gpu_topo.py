@@ -100,9 +100,12 @@ def test_gpu_launcher_one_call_bit_exact(gpu, oracle_lib, which):
 
 
 @pytest.mark.gpu
-def test_gpu_topography_run_bit_exact(gpu, oracle_lib):
+@pytest.mark.parametrize("two_pass", [True, False], ids=["two_pass", "one_pass"])
+def test_gpu_topography_run_bit_exact(gpu, oracle_lib, two_pass):
     """40 steps of the pmcl3d order with the curvilinear updates on grid 0
-    (source, free-surface pass, receivers): wavefield and traces."""
+    (source, free-surface pass, receivers): wavefield and traces, with the
+    stress update in one kernel and in two passes (strain rates, then the
+    material and memory-variable update)."""
     torch, _lib = gpu
     from awp_amd.solver import FIELDS, Solver
     from oracle.oracle import OracleSolver
@@ -110,6 +113,8 @@ def test_gpu_topography_run_bit_exact(gpu, oracle_lib):
     med = w.medium()
     topo = w.params["topography"]
     g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    if not two_pass:
+        g.topo.T.rates = None
     g.run(w.nt, check_every=10)
     o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
     o.run(w.nt)
