@@ -129,6 +129,13 @@ __global__ __launch_bounds__(256) void absmax_kernel(const uint16_t* __restrict_
   }
 }
 
+// Blocks of the absmax pass: each block ends in one atomic on the same
+// word, so the grid stays at four blocks per CU (1024) and every lane loops.
+int absmax_grid(long long n) {
+  const long long b = (n / 8 + 255) / 256;
+  return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
+}
+
 long long vec8_count(const void* q, long long n) {
   return ((uintptr_t)q & 15u) == 0 ? n / 8 : 0;
 }
@@ -211,10 +218,10 @@ void awp_s16_absmax(const uint16_t* q, long long n, int fmt, float* out, hipStre
   }
   if (n <= 0) return;
   if (fmt == AWP_FMT_BF16)
-    hipLaunchKernelGGL(absmax_kernel<StBF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n,
+    hipLaunchKernelGGL(absmax_kernel<StBF16>, dim3(absmax_grid(n)), dim3(256), 0, St, q, n,
                        vec8_count(q, n), (unsigned*)out);
   else
-    hipLaunchKernelGGL(absmax_kernel<StF16>, dim3(grid_for(n)), dim3(256), 0, St, q, n,
+    hipLaunchKernelGGL(absmax_kernel<StF16>, dim3(absmax_grid(n)), dim3(256), 0, St, q, n,
                        vec8_count(q, n), (unsigned*)out);
   awp_check_launch(W);
 }
