@@ -19,7 +19,7 @@ from .grid import ALIGN, NGSL, Dims
 from .media import Medium
 from .plasticity import PlasticMedium
 from .stepping import PmclStepOrder
-from .storage import FIELDS16, FORMATS, ScaledStorage, initial_scales
+from .storage import FIELDS16, FORMATS, ScaledStorage, default_check_every, initial_scales
 
 FIELDS = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz",
           "r1", "r2", "r3", "r4", "r5", "r6")
@@ -339,7 +339,8 @@ class Solver16(ScaledStorage, Solver):
 
     def __init__(self, medium: Medium, sources: PointSources | None = None,
                  receivers: Receivers | None = None, nt_record: int = 0, device: str = "cuda",
-                 storage: str = "bf16", check_every: int = 8, topography=None):
+                 storage: str = "bf16", check_every: int | None = None, topography=None,
+                 scaling: str = "auto"):
         if isinstance(medium, (list, tuple)):
             raise ValueError("16-bit storage runs one grid")
         if storage not in FORMATS:
@@ -347,11 +348,13 @@ class Solver16(ScaledStorage, Solver):
         super().__init__(medium, sources, receivers, nt_record, device, topography=topography)
         self.storage = storage
         self.fmt = FORMATS[storage]
-        self.check_every = int(check_every)
+        self.scaling = scaling
+        self.check_every = (default_check_every(storage, scaling) if check_every is None
+                            else int(check_every))
         L = self.levels[0]
         for name in FIELDS16:
             L.f[name] = torch.zeros(L.dims.shape, dtype=torch.int16, device=self.device)
-        self.scales = initial_scales(storage, medium, sources)
+        self.scales = initial_scales(storage, medium, sources, scaling)
         self._absbuf = torch.zeros(len(FIELDS16), dtype=torch.float32, device=self.device)
         self.rescales = 0
         self._sync_scales()

@@ -14,6 +14,10 @@ rounded to bfloat16 or binary16, s_f a power of two.  The policy:
     exponent drifts more than two octaves from 2^target (binary16 target
     2^10), so the values keep the format's full precision and range;
   * a non-finite largest magnitude (binary16 overflow) is an error.
+bfloat16 keeps binary32's exponent range, so a power-of-two scale changes none
+of its stored significands (only where binary32's flush of subnormals sets
+in).  Its default policy ("auto") is therefore unit scales and no drift check;
+scaling="scaled" applies the binary16 policy to bfloat16 too.
 Both the GPU driver and the oracle's 16-bit driver apply this same policy to
 bit-identical states, so they take the same decisions.
 """
@@ -43,10 +47,24 @@ def pow2_scale(bound: float, target: int) -> float:
     return math.ldexp(1.0, max(min(e, 119), -120))
 
 
-def initial_scales(fmt: str, medium, sources) -> dict:
+SCALINGS = ("auto", "scaled")
+
+
+def uses_scaling(fmt: str, scaling: str = "auto") -> bool:
+    if scaling not in SCALINGS:
+        raise ValueError(f"scaling must be one of {SCALINGS}")
+    return not (fmt == "bf16" and scaling == "auto")
+
+
+def default_check_every(fmt: str, scaling: str = "auto") -> int:
+    """Steps between scale checks: 8, or 0 (never) for unit-scale bfloat16."""
+    return 8 if uses_scaling(fmt, scaling) else 0
+
+
+def initial_scales(fmt: str, medium, sources, scaling: str = "auto") -> dict:
     """Per-field scales before the first step (see the module docstring)."""
     scales = {f: 1.0 for f in FIELDS16}
-    if sources is None or sources.npsrc == 0:
+    if not uses_scaling(fmt, scaling) or sources is None or sources.npsrc == 0:
         return scales
     DH, DT = float(medium.DH), float(medium.DT)
     acc = np.abs(sources.rates).sum(axis=1).max() * DT  # largest accumulated moment [N m]

@@ -366,17 +366,20 @@ class OracleSolver16(OracleSolver):
     on the GPU, so the sweep reads the stored ghosts."""
 
     def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
-                 storage: str = "bf16", check_every: int = 8, topography=None):
-        from awp_amd.storage import FIELDS16, FORMATS, initial_scales
+                 storage: str = "bf16", check_every: int | None = None, topography=None,
+                 scaling: str = "auto"):
+        from awp_amd.storage import FIELDS16, FORMATS, default_check_every, initial_scales
         super().__init__(medium, sources, receivers, nt_record, dtype=np.float32,
                          topography=topography)
         self.storage = storage
         self.fmt = FORMATS[storage]
-        self.check_every = int(check_every)
+        self.scaling = scaling
+        self.check_every = (default_check_every(storage, scaling) if check_every is None
+                            else int(check_every))
         self.fields16 = FIELDS16
         L = self.levels[0]
         L.q = {n: np.zeros(L.dims.shape, dtype=np.uint16) for n in FIELDS16}
-        self.scales = initial_scales(storage, medium, sources)
+        self.scales = initial_scales(storage, medium, sources, scaling)
         self.rescales = 0
 
     # codecs

@@ -20,7 +20,8 @@ import numpy as np
 import pytest
 
 from awp_amd import configs
-from awp_amd.storage import FIELDS16, initial_scales, pow2_scale, rescale_shift
+from awp_amd.storage import (FIELDS16, default_check_every, initial_scales, pow2_scale,
+                             rescale_shift)
 
 
 def _enc(o, x, fmt, inv=1.0):
@@ -104,6 +105,12 @@ def test_scale_policy():
     sc = initial_scales("fp16", w.medium(), w.sources)
     assert set(sc) == set(FIELDS16)
     assert all(math.frexp(v)[0] == 0.5 for v in sc.values())
+    # bfloat16: unit scales and no checks by default, the binary16 policy on request
+    assert set(initial_scales("bf16", w.medium(), w.sources).values()) == {1.0}
+    scaled = initial_scales("bf16", w.medium(), w.sources, scaling="scaled")
+    assert all(math.frexp(v)[0] == 0.5 for v in scaled.values()) and set(scaled.values()) != {1.0}
+    assert default_check_every("bf16") == 0 and default_check_every("fp16") == 8
+    assert default_check_every("bf16", "scaled") == 8
 
 
 def _small():
@@ -118,14 +125,19 @@ def test_oracle_16bit_accuracy(oracle_lib):
     o64.run(w.nt)
     ref = o64.traces()
     err = {}
-    for st in ("bf16", "fp16"):
-        o = OracleSolver16(med, w.sources, w.receivers, nt_record=w.nt, storage=st)
+    for st, scaling in (("bf16", "auto"), ("bf16", "scaled"), ("fp16", "auto")):
+        o = OracleSolver16(med, w.sources, w.receivers, nt_record=w.nt, storage=st,
+                           scaling=scaling)
         o.run(w.nt)
-        err[st] = float(np.linalg.norm(o.traces() - ref) / np.linalg.norm(ref))
-        assert o.rescales > 0
+        err[st, scaling] = float(np.linalg.norm(o.traces() - ref) / np.linalg.norm(ref))
+        if st == "bf16" and scaling == "auto":
+            assert o.rescales == 0 and set(o.scales.values()) == {1.0}
+        else:
+            assert o.rescales > 0
     print(f"16-bit storage, receiver rel-L2 vs FP64 oracle: {err}")
-    assert err["bf16"] < 0.03
-    assert err["fp16"] < 0.5
+    assert err["bf16", "auto"] < 0.03
+    assert err["bf16", "scaled"] < 0.03
+    assert err["fp16", "auto"] < 0.5
 
 
 @pytest.fixture(scope="module")
@@ -163,20 +175,28 @@ def test_gpu_codecs(gpu, oracle_lib, fmt):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("storage", ["bf16", "fp16"])
-def test_gpu_16bit_run_bit_exact(gpu, oracle_lib, storage):
+@pytest.mark.parametrize("storage,scaling", [("bf16", "auto"), ("bf16", "scaled"),
+                                             ("fp16", "auto")])
+def test_gpu_16bit_run_bit_exact(gpu, oracle_lib, storage, scaling):
     """A 50-step anelastic run with 16-bit storage (velocity, stress with FS
-    ghosts, source, fstr, receivers, rescaling every 8 steps): stored values
-    and traces bit-exact against the oracle's 16-bit run."""
+    ghosts, source, fstr, receivers; with scaling, rescaling every 8 steps):
+    stored values and traces bit-exact against the oracle's 16-bit run.
+    bfloat16 "auto" keeps unit scales and no checks, "scaled" follows the
+    binary16 policy."""
     from awp_amd.solver import Solver16
     from oracle.oracle import OracleSolver16
     w = _small()
     med = w.medium()
-    g = Solver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage)
+    g = Solver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage, scaling=scaling)
     g.run(w.nt, check_every=10)
-    o = OracleSolver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage)
+    o = OracleSolver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage,
+                       scaling=scaling)
     o.run(w.nt)
-    assert g.scales == o.scales and g.rescales == o.rescales > 0
+    assert g.scales == o.scales and g.rescales == o.rescales
+    if storage == "bf16" and scaling == "auto":
+        assert g.rescales == 0 and set(g.scales.values()) == {1.0}
+    else:
+        assert g.rescales > 0
     assert np.array_equal(g.traces(), o.traces())
     for n in FIELDS16:
         a, b = g.raw(n), o.levels[0].q[n]
