@@ -239,7 +239,6 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
       Ai3 = dbl_quot(A.nu, Ai3);
       const float f_dcrj = A.dcrjx[i + AWP_NGSL + 2] * dcy * dcz;
       float z11, z12, z12x, z22, z13, z23;
-      float zs13, zs23, zs33;  // plain z derivatives of s13 (u1), s23 (u2), s33 (u3)
       if (!bnd) {
         // order 3, 0, 1, 2, 4, 5, 6
 #pragma unroll
@@ -256,9 +255,6 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
             z13 = z13 + t13; z23 = z23 + t23;
           }
         }
-        zs13 = 0.f;
-        zs23 = 0.f;
-        zs33 = 0.f;
       } else {
         // levels nz-9 .. nz-1 live in lanes 55 .. 63 of the top segment
         const int lb = nz - 1 - k0;  // lane of level nz-1
@@ -277,7 +273,6 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
             z13 = z13 + t13; z23 = z23 + t23;
           }
         }
-        zs13 = zs23 = zs33 = 0.f;
       }
       float s;
       // u1
@@ -347,9 +342,6 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
       s = s - M.f1_c[q] * z13;
       s = s - M.f2_c[q] * z23;
       const float v3 = (A.a * ST::ld(A.u3, p, A.su3) + Ai3 * s) * f_dcrj;
-      (void)zs13;
-      (void)zs23;
-      (void)zs33;
       ST::st(A.u1, p, v1, A.iu1);
       ST::st(A.u2, p, v2, A.iu2);
       ST::st(A.u3, p, v3, A.iu3);
@@ -586,8 +578,6 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
   auto zw = [&](const float* c) { return !out ? 0.f : (bnd ? zrow(c_str_pdhz4r, c) : zmid(pdhz4, c)); };
 
   const int colj = yl * j + K;  // level K of column (i = 0, j)
-  const typename ST::T* u1 = A.u1.p;
-  (void)u1;
   // raw x windows at this level: u (i-1 .. i+2), v and w (i-2 .. i+1), and
   // the x-group z sums Zu (i-1 .. i+2), Zv, Zw (i-2 .. i+1)
   float ux[4], vx[4], wx[4], Zu[4], Zv[4], Zw[4];
