@@ -26,6 +26,20 @@ struct AdjGrid {
   float c1, c2, dth, dh1, DT;
 };
 
+// Block (x, y, z) of a (k tile, j tile, i plane) grid after the XCD-aware
+// renumbering (awp_xcd_remap): consecutive logical blocks, the k and j tiles
+// of one plane and then of the next, share one XCD's L2, so the j and i
+// neighbours the gathers read come from L2.
+struct Blk3 {
+  int x, y, z;
+};
+__device__ __forceinline__ Blk3 adj_block() {
+  const int nx = (int)gridDim.x, ny = (int)gridDim.y;
+  const int nb = nx * ny * (int)gridDim.z;
+  const int b = awp_xcd_remap((int)(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z)), nb);
+  return Blk3{b % nx, (b / nx) % ny, b / (nx * ny)};
+}
+
 __device__ __forceinline__ long long aidx(const AdjGrid& G, int i, int j, int k) {
   return (long long)i * G.slice + (long long)j * G.yline + k;
 }
@@ -93,9 +107,10 @@ struct Medium {
 __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es, CPtr3 V,
                                                                Medium M, Grad8 Gr,
                                                                const AdjGrid G) {
-  const int k = AWP_ALIGN + 3 + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = 2 + AWP_NGSL + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = 2 + AWP_NGSL + (int)blockIdx.z;
+  const Blk3 B = adj_block();
+  const int k = AWP_ALIGN + 3 + (int)(B.x * 64 + threadIdx.x);
+  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
+  const int i = 2 + AWP_NGSL + B.z;
   if (k > G.top || j >= G.ny + 2 + AWP_NGSL) return;
   const long long S = G.slice, Y = G.yline;
   const long long p = aidx(G, i, j, k);
@@ -198,9 +213,10 @@ __global__ __launch_bounds__(256) void adj_strain_gather_kernel(float* __restric
                                                                 float* __restrict__ vb,
                                                                 float* __restrict__ wb,
                                                                 CPtr6 E, const AdjGrid G) {
-  const int k = AWP_ALIGN + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = 2 + AWP_NGSL + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = 2 + AWP_NGSL + (int)blockIdx.z;
+  const Blk3 B = adj_block();
+  const int k = AWP_ALIGN + (int)(B.x * 64 + threadIdx.x);
+  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
+  const int i = 2 + AWP_NGSL + B.z;
   if (k > G.top + 1 || j >= G.ny + 2 + AWP_NGSL) return;
   const long long S = G.slice, Y = G.yline;
   const long long c = aidx(G, i, j, k);
@@ -303,9 +319,10 @@ __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ 
                                                             float* __restrict__ wb, Ptr6 Aw,
                                                             CPtr6 Sg, Medium M, Grad8 Gr,
                                                             const AdjGrid G) {
-  const int k = AWP_ALIGN + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = 2 + AWP_NGSL + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = 2 + AWP_NGSL + (int)blockIdx.z;
+  const Blk3 B = adj_block();
+  const int k = AWP_ALIGN + (int)(B.x * 64 + threadIdx.x);
+  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
+  const int i = 2 + AWP_NGSL + B.z;
   if (k > G.top || j >= G.ny + 2 + AWP_NGSL) return;
   const long long S = G.slice, Y = G.yline;
   const long long p = aidx(G, i, j, k);
@@ -343,9 +360,10 @@ __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ 
 // k from align+3 to top+2; A is zero outside the velocity interior).
 __global__ __launch_bounds__(256) void adj_vel_gather_kernel(Adj15 Ab, CPtr6 Aw,
                                                              const AdjGrid G) {
-  const int k = AWP_ALIGN + 3 + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int j = 2 + AWP_NGSL + (int)(blockIdx.y * 4 + threadIdx.y);
-  const int i = 2 + AWP_NGSL + (int)blockIdx.z;
+  const Blk3 B = adj_block();
+  const int k = AWP_ALIGN + 3 + (int)(B.x * 64 + threadIdx.x);
+  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
+  const int i = 2 + AWP_NGSL + B.z;
   if (k > G.top + 2 || j >= G.ny + 2 + AWP_NGSL) return;
   const long long S = G.slice, Y = G.yline;
   const long long q = aidx(G, i, j, k);
