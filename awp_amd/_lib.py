@@ -16,6 +16,15 @@ _i = C.c_int
 _f = C.c_float
 _s = C.c_void_p  # hipStream_t
 
+
+class AwpGrid3(C.Structure):
+    """awp_grid3_t (passed by value to cuinterp_init / cuinterp_lagrange_h)."""
+    _fields_ = [("off_x", C.c_int), ("off_y", C.c_int), ("off_z", C.c_int),
+                ("line", C.c_int), ("slice", C.c_int),
+                ("size_x", C.c_int), ("size_y", C.c_int), ("size_z", C.c_int),
+                ("numel", C.c_longlong)]
+
+
 # name -> argtypes, in header order
 SIGNATURES: dict[str, list] = {
     "awp_last_error": [],
@@ -68,11 +77,27 @@ SIGNATURES: dict[str, list] = {
     "awp_topo_metrics_f": [_fp, _i, _i, _f, _s],
     "awp_topo_geom_no_stretching": [_fp, _i, _f, _s],
     "awp_topo_metrics_g": [_fp, _i, _f, _s],
+    # awp_interp_t* travels as a plain address (awp_amd.interpolation.AwpInterp)
+    "cuinterp_init": [_fp] * 4 + [AwpGrid3] + [_fp] * 4 + [_i, _i],
+    "cuinterp_lagrange_h": [_fp] * 4 + [AwpGrid3] + [_fp] * 3,
+    "cuinterp_htod": [_fp],
+    "cuinterp_dtoh": [_fp],
+    "cuinterp_malloc": [_fp],
+    "cuinterp_finalize": [_fp],
+    "cuinterp_interp_H": [_fp] * 3,
+    "cusource_add_cartesian_H": [_fp] * 3 + [_f, _f],
+    "cusource_add_curvilinear_H": [_fp] * 3 + [_f, _f, _fp, _i, _fp],
+    "cusource_add_force_H": [_fp] * 4 + [_f, _f, _f, _fp, _i, _i, _i, _fp],
 }
 RESTYPES = {
     "awp_last_error": _i,
     "awp_last_error_string": C.c_char_p,
     "awp_set_tuning": _i,
+    "cuinterp_init": _i,
+    "cuinterp_lagrange_h": _i,
+    "cuinterp_htod": _i,
+    "cuinterp_dtoh": _i,
+    "cuinterp_malloc": _i,
 }
 
 _lib = None

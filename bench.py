@@ -54,13 +54,20 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps with no event timing (for rocprofv3)")
-    ap.add_argument("--workload", choices=("anelastic", "dm", "adjoint", "topo", "cpu"),
+    ap.add_argument("--workload",
+                    choices=("anelastic", "dm", "adjoint", "topo", "lagrange", "cpu"),
                     default="anelastic",
                     help="anelastic: the headline single-grid workload; dm: discontinuous mesh "
                          "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz); "
                          "adjoint: gradient cost and finite-difference check; topo: curvilinear "
                          "topography (384 x 384 x 192 unless --nx/--ny/--nz say otherwise); "
-                         "cpu: the CPU baseline table only (no GPU)")
+                         "lagrange: Lagrange-interpolated receivers and sources on the "
+                         "512 x 512 x 256 layout; cpu: the CPU baseline table only (no GPU)")
+    ap.add_argument("--receivers", type=int, default=20000,
+                    help="lagrange workload: receivers (three components each)")
+    ap.add_argument("--subfaults", type=int, default=4000,
+                    help="lagrange workload: point sources (six components each)")
+    ap.add_argument("--degree", type=int, default=3, help="lagrange workload: degree")
     ap.add_argument("--save-traces", default="",
                     help="write the receiver traces (numpy .npy) of the timed run here")
     ap.add_argument("--nz-fine", type=int, default=96)
@@ -192,6 +199,9 @@ def main():
         return
     if args.workload == "topo":
         bench_topo(args)
+        return
+    if args.workload == "lagrange":
+        bench_lagrange(args)
         return
     args.nx, args.ny, args.nz = _size(args, 512, 512, 256)
 
@@ -569,6 +579,179 @@ def bench_dm(args) -> None:
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out))
+
+
+def _lagrange_setup(nx, ny, nz, nrec, nsrc, degree, seed=11):
+    """Coordinates of the metric grid of an nx x ny x nz mesh (h = 50 m), the
+    receiver positions (uniform in the volume) and a finite-fault cluster:
+    nsrc points on a vertical plane, 40 % of the mesh long and 30 % deep,
+    spaced about one cell apart, so neighbouring stencils overlap."""
+    import numpy as np
+
+    from awp_amd.grid import Dims, NGSL
+    from awp_amd.interpolation import grid_coordinates, metric_grid
+    h = 50.0
+    dims = Dims(nx, ny, nz)
+    grid = metric_grid(dims)
+    x = grid_coordinates(grid.size_x, h, padding=NGSL)
+    y = grid_coordinates(grid.size_y, h, padding=NGSL)
+    z = grid_coordinates(grid.size_z, h)
+    rng = np.random.default_rng(seed)
+    rec = [rng.uniform(0.0, n * h, nrec) for n in (nx, ny, nz - 4)]
+    side = int(np.ceil(np.sqrt(nsrc)))
+    u, v = np.meshgrid(np.linspace(0.3, 0.7, side), np.linspace(0.4, 0.7, side), indexing="ij")
+    u, v = u.ravel()[:nsrc], v.ravel()[:nsrc]
+    src = [np.full(nsrc, 0.5 * nx * h) + rng.uniform(-0.5 * h, 0.5 * h, nsrc), u * ny * h,
+           v * (nz - 4) * h]
+    return dims, grid, (x, y, z), rec, src, h
+
+
+def bench_lagrange(args) -> None:
+    """Lagrange-interpolated receivers and sources (cuinterp_interp_H and
+    cusource_add_cartesian_H) on the 512 x 512 x 256 field layout.  A step
+    samples the three velocity fields at every receiver and adds the six
+    moment-rate components of every point source into the six stress fields
+    (degree 3 by default).  value and ms_per_step come from the wall clock
+    of the timed steps; each kernel's duration from HIP events on the launch
+    stream."""
+    import numpy as np
+    import torch
+
+    from awp_amd import _lib
+    from awp_amd.interpolation import Interpolation
+
+    nx, ny, nz = _size(args, 512, 512, 256)
+    nrec, nsrc, deg = args.receivers, args.subfaults, args.degree
+    dims, grid, (x, y, z), rec, src, h = _lagrange_setup(nx, ny, nz, nrec, nsrc, deg)
+    dt = 0.0025
+    t0 = time.perf_counter()
+    R = Interpolation(x, y, z, grid, *rec, degree=deg)
+    S = Interpolation(x, y, z, grid, *src, degree=deg)
+    setup_s = time.perf_counter() - t0
+    print(f"lagrange setup {setup_s:.2f} s; source map {S.num_cells} points for "
+          f"{nsrc * (deg + 1) ** 3} terms", file=sys.stderr, flush=True)
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    vel = [torch.randn(dims.size, device="cuda", generator=gen) for _ in range(3)]
+    stress = [torch.zeros(dims.size, device="cuda") for _ in range(6)]
+    nsteps_rec = args.steps + args.warmup
+    traces = torch.zeros((3, nsteps_rec, nrec), device="cuda")
+    mrate = torch.randn((6, nsteps_rec, nsrc), device="cuda", generator=gen)
+
+    def step(n):
+        for c in range(3):
+            R.interp(traces[c, n], vel[c])
+        for c in range(6):
+            S.add_cartesian(stress[c], mrate[c, n], h, dt)
+
+    if args.profile_steps:
+        for n in range(args.profile_steps):
+            step(n)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_steps": args.profile_steps, "workload": "lagrange"}))
+        return
+    for n in range(args.warmup):
+        step(n)
+    torch.cuda.synchronize()
+    _lib.check()
+    K = args.steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for n in range(K):
+        step(args.warmup + n)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _lib.check()
+    # Kernel durations: HIP events on the launch stream around each kernel
+    # group, with the GPU held busy first (torch.cuda._sleep) so the host has
+    # queued every launch of the group before the first event fires; the
+    # events then time the kernels, not the host's launch gaps.
+    stream = torch.cuda.current_stream()
+    rec_t, src_t = [], []
+    for n in range(min(K, 100)):
+        m = args.warmup + n
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        torch.cuda._sleep(2_000_000)
+        e[0].record(stream)
+        for c in range(3):
+            R.interp(traces[c, m], vel[c])
+        e[1].record(stream)
+        for c in range(6):
+            S.add_cartesian(stress[c], mrate[c, m], h, dt)
+        e[2].record(stream)
+        torch.cuda.synchronize()
+        rec_t.append(e[0].elapsed_time(e[1]) / 3)
+        src_t.append(e[1].elapsed_time(e[2]) / 6)
+    _lib.check()
+    rec_ms = float(np.mean(rec_t))
+    src_ms = float(np.mean(src_t))
+    nb = deg + 1
+    # algorithmic bytes of one launch: receivers read nb^3 field values, 3 nb
+    # coefficients and 4 indices per query and write one sample; sources read
+    # each touched point's position and list bounds, one code per term, each
+    # query's coefficients, indices and value, and read + write the point
+    rec_bytes = nrec * (4 * nb ** 3 + 12 * nb + 16 + 4)
+    terms = nsrc * nb ** 3
+    src_bytes = S.num_cells * (8 + 4 + 8) + terms * 4 + nsrc * (12 * nb + 16 + 4)
+    rec_gbs = rec_bytes / (rec_ms * 1e-3) / 1e9
+    src_gbs = src_bytes / (src_ms * 1e-3) / 1e9
+    out = {
+        "metric": "Lagrange receiver samples + source components per second",
+        "value": (3 * nrec + 6 * nsrc) * K / wall, "unit": "query-evaluations/s",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup, "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"lagrange_{nx}x{ny}x{nz}", "receivers": nrec,
+                   "point_sources": nsrc, "degree": deg, "h_m": h,
+                   "source_grid_points": S.num_cells},
+        "kernels": {
+            "cuinterp_interp_H": {"avg_ms": rec_ms, "bytes": rec_bytes, "GBps": rec_gbs},
+            "cusource_add_cartesian_H": {"avg_ms": src_ms, "bytes": src_bytes, "GBps": src_gbs}},
+        "roofline": {"bound": "hbm", "achieved": rec_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rec_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "cuinterp_interp_H"},
+        "cpu_baseline": None if args.no_cpu else lagrange_cpu_baseline(args, nx, ny, nz),
+        "setup_s": setup_s,
+    }
+    R.close()
+    S.close()
+    print(json.dumps(out))
+
+
+def lagrange_cpu_baseline(args, nx, ny, nz) -> dict:
+    """The oracle's receiver and source loops (one thread, the reference's
+    per-query arithmetic) on the same workload, steps repeated for about
+    --cpu-seconds / 2."""
+    import numpy as np
+
+    from oracle import interp as oi
+    deg = args.degree
+    dims, grid, xyz, rec, src, h = _lagrange_setup(nx, ny, nz, args.receivers, args.subfaults,
+                                                   deg)
+    rc = oi.lagrange_h(*xyz, *rec, deg)[1:]
+    sc = oi.lagrange_h(*xyz, *src, deg)[1:]
+    rng = np.random.default_rng(2)
+    vel = [rng.standard_normal(dims.size, dtype=np.float32) for _ in range(3)]
+    stress = [np.zeros(dims.size, np.float32) for _ in range(6)]
+    out = np.zeros(args.receivers, np.float32)
+    vals = rng.standard_normal(args.subfaults).astype(np.float32)
+    ridx = np.arange(args.receivers)
+    lidx = np.arange(args.subfaults)
+    target = max(2.0, args.cpu_seconds / 2)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        for c in range(3):
+            oi.interp(out, vel[c], rc, ridx, grid)
+        for c in range(6):
+            oi.source_add("cartesian", stress[c], vals, sc, lidx, grid, h, 0.0025)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= target or n >= 200:
+            break
+    return {"value": (3 * args.receivers + 6 * args.subfaults) * n / el,
+            "unit": "query-evaluations/s", "cores": 1, "kind": "port",
+            "sample": f"oracle receiver/source loops, same workload, {n} steps in {el:.1f} s, "
+                      f"1 thread, cpu={_cpu_model()}"}
 
 
 if __name__ == "__main__":

@@ -353,6 +353,82 @@ void awp_topo_geom_no_stretching(float* g, int nz, float h, hipStream_t St);
 /* metrics_build_g (metrics.c:223-229, 293-318): g_c, g3, g3_c from g */
 void awp_topo_metrics_g(const awp_g_grid_t* mg, int nz, float h, hipStream_t St);
 
+/* ---- Lagrange-interpolated receivers and sources ---------------------------
+ * The reference keeps one cu_interp_t per grid and receiver/source set
+ * (include/interpolation/interpolation.cuh:21-33).  awp_interp_t keeps its
+ * members under the same names; its grid3_t becomes awp_grid3_t, the three
+ * members of grid3_t that the path reads plus the field size for bounds
+ * checks.  Point (i, j, k) of the grid's coordinate arrays is field element
+ *   off_z + k + (off_y + j)*line + (off_x + i)*slice
+ * (grid_index, include/grid/grid_3d.h:113-115). */
+typedef struct {
+  int off_x, off_y, off_z;    /* grid3_t.offset1 */
+  int line, slice;            /* grid3_t.line, grid3_t.slice */
+  int size_x, size_y, size_z; /* grid_boundary_size(grid): coordinate array lengths */
+  long long numel;            /* floats in one field on this grid */
+} awp_grid3_t;
+
+typedef struct {
+  int num_basis; /* degree + 1 */
+  int num_query;
+  size_t size_l, size_i;
+  float *lx, *ly, *lz; /* host, num_query x num_basis */
+  int *ix, *iy, *iz, *ridx; /* host, num_query */
+  float *d_lx, *d_ly, *d_lz;
+  int *d_ix, *d_iy, *d_iz, *d_ridx;
+  awp_grid3_t grid;
+  /* no reference counterpart.  Stream the launchers use (0 by default). */
+  hipStream_t stream;
+  /* Written by cuinterp_htod: the grid points the queries touch, each with
+   * its (query, basis point) terms in ascending query order.  The source
+   * launchers run one thread per grid point over this map, so points shared
+   * by several queries receive their terms in query order, deterministically
+   * and without atomics. */
+  int num_cells;
+  long long* d_cell_pos;
+  int* d_cell_ptr;  /* num_cells + 1 */
+  int* d_cell_term; /* ((q*nb + i)*nb + j)*nb + k for basis point (i, j, k) */
+  int ix_max, iy_max, iz_max; /* largest index + num_basis - 1 uploaded */
+} awp_interp_t;
+
+/* cuinterp_init, src/interpolation/interpolation.cu:13-40: allocate, compute
+ * the Lagrange coefficients of num_query query points (qx, qy, qz) on the
+ * grid coordinates x, y, z (grid.size_x, ... entries) and upload them.
+ * indices become ridx (receivers: output slot; sources: input column).
+ * Returns 0 or the reference's error code (800: num_query <= 0,
+ * 900: allocation; 400 here also when an index leaves the field). */
+int cuinterp_init(awp_interp_t* out, const float* x, const float* y, const float* z,
+                  awp_grid3_t grid, const float* qx, const float* qy, const float* qz,
+                  const int* indices, int num_query, int degree);
+/* cuinterp_lagrange_h, interpolation.cu:42-70 (host only) */
+int cuinterp_lagrange_h(awp_interp_t* host, const float* x, const float* y,
+                        const float* z, awp_grid3_t grid, const float* qx,
+                        const float* qy, const float* qz);
+/* cuinterp_htod / cuinterp_dtoh / cuinterp_malloc / cuinterp_finalize,
+ * interpolation.cu:72-162.  htod also rebuilds the grid-point map and checks
+ * every index against the field (400 if one leaves it). */
+int cuinterp_htod(awp_interp_t* interp);
+int cuinterp_dtoh(awp_interp_t* interp);
+int cuinterp_malloc(awp_interp_t* interp);
+void cuinterp_finalize(awp_interp_t* interp);
+/* cuinterp_interp_H, interpolation.cu:164-175: out[ridx[q]] = sum over the
+ * num_basis^3 stencil of lx ly lz in, one thread per query */
+void cuinterp_interp_H(const awp_interp_t* I, float* out, const float* in);
+/* cusource_add_cartesian_H, src/topography/sources/source.cu:11-22:
+ * out += -dt/h^3 lx ly lz in[ridx[q]] */
+void cusource_add_cartesian_H(const awp_interp_t* I, float* out, const float* in,
+                              float h, float dt);
+/* cusource_add_curvilinear_H, source.cu:51-63: the same divided by the
+ * Jacobian f(i, j) dg(k) of the topography metrics (f: the (nx+20) x (ny+84)
+ * metric layout, dg: a g-grid array) */
+void cusource_add_curvilinear_H(const awp_interp_t* I, float* out, const float* in,
+                                float h, float dt, const float* f, int ny, const float* dg);
+/* cusource_add_force_H, source.cu:97-113: a body force on a velocity field,
+ * scaled by -quad_weight / (f dg rho) with rho from d1 (a 3-D field) */
+void cusource_add_force_H(const awp_interp_t* I, float* out, const float* in,
+                          const float* d1, float h, float dt, float quad_weight,
+                          const float* f, int nx, int ny, int nz, const float* dg);
+
 #ifdef __cplusplus
 }
 #endif
