@@ -195,3 +195,41 @@ def test_gpu_htod_rejects_indices_outside_the_field(env):
     I.I.ix[0] = grid.size_x + 40
     assert _lib.load().cuinterp_htod(C.byref(I.I)) == 400
     I.close()
+
+
+def test_gpu_receiver_file_drives_sampling(env, tmp_path, monkeypatch):
+    """The reference's receiver fixture (tests/fixtures/receiver.txt: four
+    surface-level receivers, degree 3, buffers of 2 x 3 records) read by the
+    v2 reader, mapped to query coordinates, sampled every step into the
+    device buffer and flushed to the host buffer: every record equals the
+    oracle's interpolation of that step's field."""
+    import shutil
+    from pathlib import Path
+    from awp_amd import inputs
+    torch, _lib, oi = env
+    fix = Path(__file__).resolve().parent / "golden" / "fixtures" / "receiver.txt"
+    shutil.copy(fix, tmp_path / "receiver.txt")
+    monkeypatch.chdir(tmp_path)
+    inp = inputs.read_input("receiver.txt")
+    dims = Dims(16, 16, 24)
+    grid, x, y, z = _grid(dims)
+    qx, qy, qz = inputs.cartesian_queries(inp, H, dims.nzt)
+    I = Interpolation(x, y, z, grid, qx, qy, qz, degree=inp.degree)
+    buf = inputs.StepBuffer(inp.length, inp.gpu_buffer_size, inp.cpu_buffer_size, inp.stride)
+    rng = np.random.default_rng(4)
+    coef = I.coefficients()
+    want = np.zeros_like(buf.h_buffer)
+    nsteps = inp.gpu_buffer_size * inp.cpu_buffer_size
+    for step in range(nsteps):
+        field = rng.standard_normal(dims.size).astype(np.float32)
+        if buf.is_device_ready(step):
+            I.interp(buf.device_slot(step), torch.from_numpy(field).cuda())
+            rec = np.zeros(inp.length, np.float32)
+            oi.interp(rec, field, coef, np.arange(inp.length), grid)
+            o = buf.host_offset(step) + buf.device_offset(step)
+            want[o:o + inp.length] = rec
+        torch.cuda.synchronize()
+        buf.copy_to_host(step)
+    assert buf.is_host_full(nsteps - 1)
+    assert np.array_equal(bits(buf.h_buffer), bits(want))
+    I.close()
