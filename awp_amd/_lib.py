@@ -38,6 +38,9 @@ SIGNATURES: dict[str, list] = {
     "dstrqc_H_new": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,
     "fstr_H": [_fp] * 3 + [_s, _i, _i, _i, _i],
     "addsrc_H": [_i, _i, _i, _fp, _i, _s] + [_fp] * 12 + [_i],
+    "SetDeviceFilterParameters": [_i, _fp, _fp],
+    "addkinsrc_H": [_i, _i, _fp, _i, _s] + [_fp] * 15 + [_i],
+    "frcvel_H": [_i, _i, _i, _fp, _i, _i, _s] + [_fp] * 9 + [_i, _i, _i],
     "velbuffer_H": [_fp] * 8 + [_i] * 13 + [_s, _i, _fp, _i],
     "awp_receivers_gather": [_fp] * 4 + [_i, _fp, _i, _s, _i],
     "drprecpc_calc_H_opt": [_fp] * 13 + [_i] * 5 + [_s, _i],

@@ -96,6 +96,33 @@ void addsrc_H(int i, int READ_STEP, int dim, int* psrc, int npsrc, hipStream_t S
               float* axx, float* ayy, float* azz, float* axz, float* ayz, float* axy,
               float* xx, float* yy, float* zz, float* xy, float* yz, float* xz, int d_i);
 
+/* Replaces SetDeviceFilterParameters, include/awp/kernel_launch.h:25-26
+ * (src/awp/kernel.cu:119-123): the IIR filter (order <= 19, a[0] != 0)
+ * addkinsrc_H applies to every source time function; order 0 = none.  Kept
+ * host-side and passed to the kernel by value. */
+void SetDeviceFilterParameters(int filtorder, double* srcfilt_b, double* srcfilt_a);
+/* Replaces addkinsrc_H, include/awp/kernel_launch.h:121-126 (src/awp/kernel.cu:
+ * 3677-3858): kinematic subfaults.  Two parameter rows per subfault j: row 2j
+ * of axx..ayz = STF type (1 Brune, 2 Liu), slip, rupture time, rise time,
+ * area; row 2j+1 of axx, ayy, azz = strike, dip, rake in degrees.  Once step
+ * i*DT passes the rupture time, sigma -= DT/DH^3 * stf * slip * area / mu *
+ * M(strike, dip, rake) at the subfault's cell and mom[j] += stf * slip * area
+ * / mu * DT; srcfilt_d holds npsrc x (order+1) doubles of filter state. */
+void addkinsrc_H(int i, int dim, int* psrc, int npsrc, hipStream_t St, float* mu, float* axx,
+                 float* ayy, float* azz, float* axz, float* ayz, float* axy, float* xx, float* yy,
+                 float* zz, float* xy, float* yz, float* xz, float* mom, double* srcfilt_d,
+                 int d_i);
+/* Replaces frcvel_H, include/awp/kernel_launch.h:74-76 (src/awp/kernel.cu:
+ * 1808-1929): prescribed velocities at the source cells, sampled every tskp
+ * steps in axx, ayy, azz (rows 0..READ_STEP, npsrc wide: the arrays hold
+ * (READ_STEP+1)*npsrc floats) and linearly interpolated in time; at the last
+ * prescribed step the force that holds the velocity is stored in row
+ * READ_STEP and applied from then on.  ymin/ymax (-1: none) bound the cells'
+ * x index, as the reference's kernel reads them. */
+void frcvel_H(int i, int READ_STEP, int dim, int* psrc, int npsrc, int tskp, hipStream_t St,
+              float* axx, float* ayy, float* azz, float* axz, float* ayz, float* axy, float* u1,
+              float* v1, float* w1, int ymin, int ymax, int d_i);
+
 /* Replaces velbuffer_H, include/awp/kernel_launch.h:181-186
  * (src/awp/kernel.cu:3927-4011).  Strided sub-volume sampling: u1, v1 at
  * ko = nzt+align-1-k and w1 at ko-1 on grid 0 (FOLLOWBATHY = 0), ko on finer
