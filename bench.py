@@ -55,14 +55,17 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps with no event timing (for rocprofv3)")
     ap.add_argument("--workload",
-                    choices=("anelastic", "dm", "adjoint", "topo", "lagrange", "cpu"),
+                    choices=("anelastic", "dm", "adjoint", "topo", "lagrange", "kinematic",
+                             "cpu"),
                     default="anelastic",
                     help="anelastic: the headline single-grid workload; dm: discontinuous mesh "
                          "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz); "
                          "adjoint: gradient cost and finite-difference check; topo: curvilinear "
                          "topography (384 x 384 x 192 unless --nx/--ny/--nz say otherwise); "
                          "lagrange: Lagrange-interpolated receivers and sources on the "
-                         "512 x 512 x 256 layout; cpu: the CPU baseline table only (no GPU)")
+                         "512 x 512 x 256 layout; kinematic: finite-fault subfaults "
+                         "(addkinsrc_H, Brune/Liu, 4th-order filter) on the same layout; cpu: "
+                         "the CPU baseline table only (no GPU)")
     ap.add_argument("--receivers", type=int, default=20000,
                     help="lagrange workload: receivers (three components each)")
     ap.add_argument("--subfaults", type=int, default=4000,
@@ -202,6 +205,9 @@ def main():
         return
     if args.workload == "lagrange":
         bench_lagrange(args)
+        return
+    if args.workload == "kinematic":
+        bench_kinematic(args)
         return
     args.nx, args.ny, args.nz = _size(args, 512, 512, 256)
 
@@ -715,6 +721,120 @@ def bench_lagrange(args) -> None:
     R.close()
     S.close()
     print(json.dumps(out))
+
+
+def _kinematic_setup(nx, ny, nz, n, seed=3):
+    """n subfaults on a vertical plane x = nx/2 (distinct cells, one per
+    (y, z) cell of a rectangle 90 % of ny wide and up to 80 % of nz deep),
+    rupture spreading at 2.8 km/s from the plane's centre, Brune and Liu
+    source time functions alternating, rise times 0.5-1.5 s, h = 50 m."""
+    import numpy as np
+    h = 50.0
+    rng = np.random.default_rng(seed)
+    wy, wz = int(0.9 * ny), max(1, min(int(0.8 * nz), -(-n // int(0.9 * ny))))
+    if n > wy * wz:
+        raise ValueError(f"{n} subfaults do not fit one per cell on a {wy} x {wz} plane")
+    cells = rng.choice(wy * wz, n, replace=False)
+    y = cells // wz + (ny - wy) // 2 + 1
+    z = cells % wz + 1
+    psrc = np.stack([np.full(n, nx // 2), y, z], 1).astype(np.int32)
+    r = np.hypot((y - ny / 2) * h, (z - wz / 2) * h)
+    cols = [np.zeros(2 * n, np.float32) for _ in range(6)]
+    cols[0][0::2] = 1 + (np.arange(n) % 2)
+    cols[1][0::2] = rng.uniform(0.5, 3.0, n)
+    cols[2][0::2] = r / 2800.0
+    cols[3][0::2] = rng.uniform(0.5, 1.5, n)
+    cols[4][0::2] = h * h
+    cols[0][1::2] = 30.0
+    cols[1][1::2] = 90.0
+    cols[2][1::2] = rng.uniform(150, 210, n)
+    return psrc, cols, h
+
+
+def bench_kinematic(args) -> None:
+    """Kinematic finite-fault sources (addkinsrc_H) on the 512 x 512 x 256
+    layout: --subfaults subfaults (default 80,000 when left at the lagrange
+    default), 4th-order Butterworth-type IIR filter on every source time
+    function, one launch per step after the rupture front has passed.
+    Kernel durations from HIP events with the GPU held busy first."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from awp_amd import _lib
+    from awp_amd.grid import Dims
+
+    nx, ny, nz = _size(args, 512, 512, 256)
+    n = 80000 if args.subfaults == 4000 else args.subfaults
+    lib = _lib.load()
+    dims = Dims(nx, ny, nz)
+    DT = 0.0025
+    psrc, cols, h = _kinematic_setup(nx, ny, nz, n)
+    F, I = C.c_float, C.c_int
+    z9 = (F * 9)()
+    lib.SetDeviceConstValue((F * 1)(h), F(DT), (I * 1)(nx), (I * 1)(ny), (I * 1)(nz), 1, 0.0,
+                            0.0, z9, z9)
+    # a 4th-order low-pass (Butterworth, cut-off 0.2 of Nyquist), coefficients fixed here
+    b = np.array([0.004824343357716228, 0.019297373430864913, 0.02894606014629737,
+                  0.019297373430864913, 0.004824343357716228])
+    a = np.array([1.0, -2.369513007182038, 2.313988414415881, -1.054665405878568,
+                  0.18737949236818502])
+    lib.SetDeviceFilterParameters(4, b.ctypes.data, a.ctypes.data)
+    _lib.check()
+    dev = [torch.zeros(dims.size, device="cuda") for _ in range(6)]
+    mu = torch.full((dims.size,), 3.0e10, device="cuda")
+    dps = torch.from_numpy(psrc.reshape(-1).copy()).cuda()
+    dcols = [torch.from_numpy(c).cuda() for c in cols]
+    mom = torch.zeros(n, device="cuda")
+    filt = torch.zeros(n * 5, dtype=torch.float64, device="cuda")
+    st = torch.cuda.current_stream().cuda_stream
+    first = int(float(cols[2][0::2].max()) / DT) + 2   # every subfault has started
+
+    def launch(step):
+        lib.addkinsrc_H(step, 3, dps.data_ptr(), n, st, mu.data_ptr(),
+                        *(c.data_ptr() for c in dcols), *(f.data_ptr() for f in dev),
+                        mom.data_ptr(), filt.data_ptr(), 0)
+
+    for k in range(args.warmup):
+        launch(first + k)
+    torch.cuda.synchronize()
+    _lib.check()
+    K = args.steps
+    t0 = time.perf_counter()
+    for k in range(K):
+        launch(first + args.warmup + k)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    times = []
+    stream = torch.cuda.current_stream()
+    for k in range(min(K, 50)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(1_000_000)
+        e0.record(stream)
+        for r in range(4):
+            launch(first + args.warmup + K + 4 * k + r)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / 4)
+    _lib.check()
+    ms = float(np.mean(times))
+    # per subfault: 8 parameters and 3 cell indices read, mu read, six stresses
+    # and the moment read and written, 5 filter states read and written
+    per = 8 * 4 + 3 * 4 + 4 + 2 * 6 * 4 + 2 * 4 + 2 * 5 * 8
+    gbs = per * n / (ms * 1e-3) / 1e9
+    print(json.dumps({
+        "metric": "kinematic subfault updates per second", "value": n * K / wall,
+        "unit": "subfault-updates/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / K, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"kinematic_{nx}x{ny}x{nz}", "subfaults": n, "filter_order": 4},
+        "kernels": {"addkinsrc_H": {"avg_ms": ms, "bytes": per * n, "GBps": gbs}},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": "addkinsrc_H"},
+        "cpu_baseline": None,
+    }))
+    lib.SetDeviceFilterParameters(0, None, None)
 
 
 def lagrange_cpu_baseline(args, nx, ny, nz) -> dict:
