@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--subfaults", type=int, default=4000,
                     help="lagrange workload: point sources (six components each)")
     ap.add_argument("--degree", type=int, default=3, help="lagrange workload: degree")
+    ap.add_argument("--volume-output", type=int, default=0, metavar="NTISKP",
+                    help="anelastic workload: also write the free-surface velocities every "
+                         "NTISKP steps (velbuffer, asynchronous copies, ten samples per file)")
     ap.add_argument("--subfault-order", choices=("cell", "file"), default="cell",
                     help="kinematic workload: subfaults sorted by cell at load time (the "
                          "default) or in a shuffled file order")
@@ -242,8 +245,22 @@ def main():
         print(json.dumps({"profile_steps": args.profile_steps, "workload": w.name}))
         return
 
+    vout = None
+    if args.volume_output:
+        # pmcl3d's volume output of the free surface every --volume-output
+        # steps, ten samples per file, written to a scratch directory
+        import tempfile
+
+        from awp_amd.output import VolumeOutput
+        d = sol.dims
+        vout = VolumeOutput(d, (0, 0, 0), (d.nxt - 1, d.nyt - 1, 0), (1, 1, 1),
+                            ntiskp=args.volume_output, write_step=10,
+                            basenames=("SX", "SY", "SZ"), directory=tempfile.mkdtemp())
+
     for _ in range(args.warmup):
         sol.step()
+        if vout is not None:
+            vout.sample(sol.cur_step, sol.f["u1"], sol.f["v1"], sol.f["w1"])
     torch.cuda.synchronize()
     _lib.check()
 
@@ -267,8 +284,12 @@ def main():
         sol.record(sol.cur_step - 1)
         if check_every and sol.cur_step % check_every == 0:
             sol.check_scales()
+        if vout is not None:
+            vout.sample(sol.cur_step, sol.f["u1"], sol.f["v1"], sol.f["w1"])
         e[3].record(stream)
     torch.cuda.synchronize()
+    if vout is not None:
+        vout.close()
     wall = time.perf_counter() - t0
     _lib.check()
     vel_ms = [e[0].elapsed_time(e[1]) for e in ev]
@@ -308,7 +329,9 @@ def main():
         "config": {"workload": w.name, "storage": args.storage, "nx": w.dims.nxt, "ny": w.dims.nyt, "nz": w.dims.nzt,
                    "h_m": w.DH, "dt_s": w.DT, "attenuation": "Qs=50vs[km/s], Qp=2Qs, 8 mechanisms",
                    "medium": "layered vs 1500-4500 m/s + 5% von Karman (a=500 m, H=0.5), seed 7",
-                   "step": "dvelcx_H_opt, dstrqc_H (+FS ghosts), addsrc_H, fstr_H, receivers"},
+                   "step": "dvelcx_H_opt, dstrqc_H (+FS ghosts), addsrc_H, fstr_H, receivers"
+                           + (f", free-surface volume output every {args.volume_output} steps"
+                              if args.volume_output else "")},
         "roofline": {"bound": "hbm", "kernel": "dstrqc_H", "achieved": str_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
                      "traffic": traffic, "algorithmic_bytes_per_cell": b_str,
@@ -487,8 +510,12 @@ def bench_topo(args) -> None:
         sol.record(sol.cur_step - 1)
         if check_every and sol.cur_step % check_every == 0:
             sol.check_scales()
+        if vout is not None:
+            vout.sample(sol.cur_step, sol.f["u1"], sol.f["v1"], sol.f["w1"])
         e[3].record(stream)
     torch.cuda.synchronize()
+    if vout is not None:
+        vout.close()
     wall = time.perf_counter() - t0
     _lib.check()
     vel_avg = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
