@@ -329,3 +329,60 @@ class StepBuffer:
         if not self.is_device_full(step):
             return
         self.host_block(step)[:] = self.d_buffer.cpu().numpy()
+
+
+def receiver_step(gpu_buffer_size: int, cpu_buffer_size: int, num_writes: int, stride: int,
+                  step: int) -> int:
+    """receiver_step (src/topography/receivers/receiver.c:106-113): the last
+    step of the file that step's output goes to."""
+    unit = gpu_buffer_size * cpu_buffer_size * num_writes * stride
+    return (1 + step // unit) * unit
+
+
+def receiver_filename(base: str, inp: Input, step: int, maxsteps: int) -> str:
+    """receiver_filename (receiver.c:93-104): base_<file step>, zero-padded
+    to ceil(log10(maxsteps)) + 1 digits."""
+    import math
+    filestep = receiver_step(inp.gpu_buffer_size, inp.cpu_buffer_size, inp.num_writes,
+                             inp.stride, step)
+    width = int(math.ceil(math.log10(float(maxsteps)))) + 1
+    return f"{base}_{filestep:0{width}d}"
+
+
+class ReceiverWriter:
+    """receiver_write (receiver.c:50-76) for one rank: sample into the device
+    buffer when the step is due, flush device-fulls to the host, and when the
+    host buffer is full write it, receiver-major (each receiver's samples
+    contiguous), into the current file; num_writes host-fulls go into one
+    file one after the other (mpi_io_idx_write, src/mpi/io.c:79-100)."""
+
+    def __init__(self, interp, inp: Input, base: str, maxsteps: int, directory: str = "."):
+        self.interp, self.inp, self.base = interp, inp, base
+        self.maxsteps, self.directory = int(maxsteps), directory
+        self.buf = StepBuffer(interp.num_query, inp.gpu_buffer_size, inp.cpu_buffer_size,
+                              inp.stride)
+        self.current_write = 0
+        self.files: list[str] = []
+
+    def write(self, step: int, field) -> None:
+        import torch
+        b = self.buf
+        if b.is_device_ready(step):
+            self.interp.interp(b.device_slot(step), field)
+        if b.is_device_full(step):
+            torch.cuda.current_stream().synchronize()
+            b.copy_to_host(step)
+        if b.is_host_full(step):
+            rows = b.num_host * b.num_device
+            data = b.h_buffer.reshape(rows, b.num_elements).T.copy()
+            name = os.path.join(self.directory, receiver_filename(self.base, self.inp, step,
+                                                                  self.maxsteps))
+            mode = "r+b" if self.current_write and os.path.exists(name) else "wb"
+            with open(name, mode) as fp:
+                fp.seek(self.current_write * data.nbytes)
+                data.tofile(fp)
+            if name not in self.files:
+                self.files.append(name)
+            self.current_write += 1
+            if self.current_write == self.inp.num_writes:
+                self.current_write = 0

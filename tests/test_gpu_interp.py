@@ -233,3 +233,39 @@ def test_gpu_receiver_file_drives_sampling(env, tmp_path, monkeypatch):
     assert buf.is_host_full(nsteps - 1)
     assert np.array_equal(bits(buf.h_buffer), bits(want))
     I.close()
+
+
+def test_gpu_receiver_files(env, tmp_path, monkeypatch):
+    """receiver_write's file output for the reference's receiver fixture: six
+    samples per host buffer (2 x 3), files recv_0006 and recv_0012, each
+    receiver's six samples contiguous, equal to the oracle's interpolation of
+    each step's field."""
+    import shutil
+    from pathlib import Path
+    from awp_amd import inputs
+    torch, _lib, oi = env
+    fix = Path(__file__).resolve().parent / "golden" / "fixtures" / "receiver.txt"
+    shutil.copy(fix, tmp_path / "receiver.txt")
+    monkeypatch.chdir(tmp_path)
+    inp = inputs.read_input("receiver.txt")
+    dims = Dims(16, 16, 24)
+    grid, x, y, z = _grid(dims)
+    I = Interpolation(x, y, z, grid, *inputs.cartesian_queries(inp, H, dims.nzt),
+                      degree=inp.degree)
+    w = inputs.ReceiverWriter(I, inp, "recv", inp.steps, directory=str(tmp_path))
+    rng = np.random.default_rng(6)
+    coef = I.coefficients()
+    want = []
+    for step in range(12):
+        field = rng.standard_normal(dims.size).astype(np.float32)
+        w.write(step, torch.from_numpy(field).cuda())
+        rec = np.zeros(inp.length, np.float32)
+        oi.interp(rec, field, coef, np.arange(inp.length), grid)
+        want.append(rec)
+    torch.cuda.synchronize()
+    assert [Path(f).name for f in w.files] == ["recv_0006", "recv_0012"]
+    for n, f in enumerate(w.files):
+        got = np.fromfile(f, dtype=np.float32).reshape(inp.length, 6)
+        exp = np.stack(want[6 * n:6 * n + 6], 1)
+        assert np.array_equal(bits(got), bits(exp)), f
+    I.close()
