@@ -119,7 +119,8 @@ class Solver(PmclStepOrder):
                  sources: PointSources | Sequence | None = None,
                  receivers: Receivers | None = None, nt_record: int = 0,
                  device: str = "cuda", variant: str = "dstrqc", NPC: int = 0,
-                 plastic: PlasticMedium | Sequence | None = None, topography=None):
+                 plastic: PlasticMedium | Sequence | None = None, topography=None,
+                 kinematic=None):
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a GPU: the HIP path has no CPU fallback")
         self.lib = _lib.load()
@@ -171,6 +172,21 @@ class Solver(PmclStepOrder):
             f32 = dict(dtype=torch.float32, device=self.device)
             self.rec = dict(idx=torch.from_numpy(idx.reshape(-1).copy()).to(self.device),
                             n=len(idx), out=torch.zeros((max(nt_record, 1), len(idx), 3), **f32))
+        # kinematic finite fault on grid 0 (IFAULT = 5): replaces the point sources
+        self.kinematic = None
+        if kinematic is not None:
+            kinematic.check(self.levels[0].dims)
+            dev = dict(device=self.device)
+            self.kinematic = dict(
+                fault=kinematic,
+                psrc=torch.from_numpy(np.ascontiguousarray(kinematic.cells, np.int32)
+                                      .reshape(-1).copy()).to(self.device),
+                cols=[torch.from_numpy(c).to(self.device) for c in kinematic.columns()],
+                mom=torch.zeros(kinematic.n, dtype=torch.float32, **dev),
+                filt=torch.zeros(kinematic.n * (kinematic.order + 1), dtype=torch.float64,
+                                 **dev),
+                b=None if kinematic.b is None else np.ascontiguousarray(kinematic.b, np.float64),
+                a=None if kinematic.a is None else np.ascontiguousarray(kinematic.a, np.float64))
         self.cur_step = 0
 
     # grid-0 views kept for single-grid callers
@@ -298,6 +314,21 @@ class Solver(PmclStepOrder):
                           _ptr(axx), _ptr(ayy), _ptr(azz), _ptr(axz), _ptr(ayz), _ptr(axy),
                           _ptr(f["xx"]), _ptr(f["yy"]), _ptr(f["zz"]), _ptr(f["xy"]),
                           _ptr(f["yz"]), _ptr(f["xz"]), p)
+
+    def add_kinematic(self, p: int) -> None:
+        K, L = self.kinematic, self.levels[p]
+        f = L.f
+        # the filter is process-wide state in the library: set it for this fault
+        if K["b"] is None:
+            self.lib.SetDeviceFilterParameters(0, None, None)
+        else:
+            self.lib.SetDeviceFilterParameters(len(K["b"]) - 1, K["b"].ctypes.data,
+                                               K["a"].ctypes.data)
+        c = K["cols"]
+        self.lib.addkinsrc_H(self.cur_step, 3, _ptr(K["psrc"]), K["fault"].n, self.stream,
+                             _ptr(L.m["mu"]), *(_ptr(x) for x in c), _ptr(f["xx"]),
+                             _ptr(f["yy"]), _ptr(f["zz"]), _ptr(f["xy"]), _ptr(f["yz"]),
+                             _ptr(f["xz"]), _ptr(K["mom"]), _ptr(K["filt"]), p)
 
     def free_surface(self) -> None:
         d, f = self.dims, self.f

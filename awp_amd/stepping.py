@@ -37,12 +37,21 @@ class PmclStepOrder:
     ngrids: int = 1
     plastic = None  # None, or one plastic medium per grid
 
+    kinematic = None  # a KinematicFault on grid 0 (IFAULT = 5), or None
+
     def start(self) -> None:
-        """Initial source sample before the loop (pmcl3d.c:1175-1186)."""
+        """Initial source sample before the loop (pmcl3d.c:1175-1186; only
+        for point sources, IFAULT < 4)."""
+        if self.kinematic is not None:
+            return
         for p in range(self.ngrids):
             self.add_source(p, 1)
 
     def _sources(self) -> None:
+        if self.kinematic is not None:
+            # IFAULT = 5: the subfaults of grid 0 every step (pmcl3d.c:2025-2034)
+            self.add_kinematic(0)
+            return
         for p in range(self.ngrids):
             nst = self.source_steps(p)
             if nst and self.cur_step < nst:

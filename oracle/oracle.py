@@ -188,7 +188,7 @@ class OracleSolver(PmclStepOrder):
 
     def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
                  dtype=np.float32, variant: str = "dstrqc", NPC: int = 0, plastic=None,
-                 topography=None):
+                 topography=None, kinematic=None):
         self.lib = load()
         self.dtype = np.dtype(dtype)
         self.suf = "f32" if self.dtype == np.float32 else "f64"
@@ -215,6 +215,15 @@ class OracleSolver(PmclStepOrder):
                 raise ValueError("topography runs one grid without plasticity")
             L0 = self.levels[0]
             self.topo = TopoHost(self.lib, topography, L0.dims, L0.DH, L0.DT, self.dtype)
+        self.kinematic = None
+        if kinematic is not None:
+            if self.dtype != np.float32:
+                raise ValueError("the kinematic-source oracle is single precision")
+            kinematic.check(self.levels[0].dims)
+            self.kinematic = dict(
+                fault=kinematic, psrc=np.ascontiguousarray(kinematic.cells, np.int32),
+                cols=kinematic.columns(), mom=np.zeros(kinematic.n, np.float32),
+                filt=np.zeros(kinematic.n * (kinematic.order + 1), np.float64))
 
     @property
     def dims(self):
@@ -329,6 +338,15 @@ class OracleSolver(PmclStepOrder):
             sample_step, s["nst"], 3, _p(s["psrc"]), s["npsrc"], *(_p(c) for c in s["cols"]),
             _p(f["xx"]), _p(f["yy"]), _p(f["zz"]), _p(f["xy"]), _p(f["yz"]), _p(f["xz"]),
             d.nxt, d.nyt, d.nzt, L.DH, L.DT)
+
+    def add_kinematic(self, p: int) -> None:
+        from oracle import kin
+        K, L = self.kinematic, self.levels[p]
+        f, F = L.f, K["fault"]
+        kin.addkinsrc(self.cur_step, K["psrc"], L.m["mu"], K["cols"][:5],
+                      [f[n] for n in ("xx", "yy", "zz", "xy", "yz", "xz")], K["mom"],
+                      K["filt"] if F.order else None, F.b, F.a, np.float32(L.DT),
+                      np.float32(L.DH), L.dims)
 
     def free_surface(self) -> None:
         d, f = self.dims, self.f
