@@ -269,3 +269,41 @@ def test_gpu_receiver_files(env, tmp_path, monkeypatch):
         exp = np.stack(want[6 * n:6 * n + 6], 1)
         assert np.array_equal(bits(got), bits(exp)), f
     I.close()
+
+
+def test_gpu_receivers_at_the_benchmark_size(env):
+    """200,000 degree-3 receivers on the 512 x 512 x 256 layout of the
+    benchmark, on a field holding a cubic: bit for bit against the oracle,
+    and the cubic reproduced to float32 rounding."""
+    torch, _lib, oi = env
+    rng = np.random.default_rng(21)
+    dims = Dims(512, 512, 256)
+    grid, x, y, z = _grid(dims)
+    nq = 200000
+    qx = rng.uniform(x[0], x[-1], nq)
+    qy = rng.uniform(y[0], y[-1], nq)
+    qz = rng.uniform(z[0], z[-1], nq)
+    print(f"receivers nq={nq} dims={dims}", flush=True)
+    # the field on the device: c(x, y, z) at every coordinate point, zero elsewhere
+    X = torch.from_numpy(x.astype(np.float64)).cuda()
+    Y = torch.from_numpy(y.astype(np.float64)).cuda()
+    Z = torch.from_numpy(z.astype(np.float64)).cuda()
+    cub = (0.01 * X[:, None, None] ** 3 - 0.2 * Y[None, :, None] ** 2 * Z[None, None, :]
+           + 0.05 * Z[None, None, :] ** 3 + 1.0).float()
+    a = torch.zeros(dims.shape, device="cuda")
+    a[2:2 + len(x), 2:2 + len(y), ALIGN:ALIGN + len(z)] = cub
+    del cub
+    a = a.reshape(-1)
+    I = Interpolation(x, y, z, grid, qx, qy, qz, degree=3)
+    out = torch.zeros(nq, device="cuda")
+    I.interp(out, a)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    host = a.cpu().numpy()
+    want = np.zeros(nq, np.float32)
+    oi.interp(want, host, I.coefficients(), np.arange(nq), grid)
+    assert np.array_equal(bits(got), bits(want))
+    q = [np.float32(v).astype(np.float64) for v in (qx, qy, qz)]
+    exact = 0.01 * q[0] ** 3 - 0.2 * q[1] ** 2 * q[2] + 0.05 * q[2] ** 3 + 1.0
+    assert np.max(np.abs(got - exact)) < 2e-6 * float(np.abs(host).max()) * 64
+    I.close()
