@@ -179,3 +179,56 @@ def test_gpu_kinematic_run_matches_oracle(env):
         got, want = g.field(name), o.f[name]
         assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max(), name
     lib.SetDeviceFilterParameters(0, None, None)
+
+
+def test_gpu_addkinsrc_at_the_benchmark_size(env):
+    """80,000 subfaults on one vertical plane of the 512 x 512 x 256 layout
+    with a 4th-order filter, five steps after every rupture time: stresses,
+    moments and filter states within 1e-5 of the oracle."""
+    from scipy import signal
+    torch, _lib, lib, ok = env
+    dims = Dims(512, 512, 256)
+    DH, DT = 50.0, 0.0025
+    _set_grid(lib, dims, DH, DT)
+    rng = np.random.default_rng(31)
+    flat = rng.choice(460 * 174, 80000, replace=False)
+    psrc = np.ascontiguousarray(np.stack([np.full(80000, 256), 27 + flat // 174, 1 + flat % 174],
+                                         1).astype(np.int32))
+    n = len(psrc)
+    cols = [np.zeros(2 * n, np.float32) for _ in range(6)]
+    cols[0][0::2] = 1 + np.arange(n) % 2
+    cols[1][0::2] = rng.uniform(0.5, 3.0, n)
+    cols[2][0::2] = rng.uniform(0.0, 0.01, n)
+    cols[3][0::2] = rng.uniform(0.05, 0.2, n)
+    cols[4][0::2] = DH * DH
+    cols[0][1::2], cols[1][1::2], cols[2][1::2] = 30.0, 90.0, rng.uniform(150, 210, n)
+    b, a = signal.butter(4, 0.2)
+    lib.SetDeviceFilterParameters(4, np.ascontiguousarray(b).ctypes.data,
+                                  np.ascontiguousarray(a).ctypes.data)
+    mu = np.full(dims.size, 3.0e10, np.float32)
+    host = [np.zeros(dims.size, np.float32) for _ in range(6)]
+    mom = np.zeros(n, np.float32)
+    filt = np.zeros(n * 5, np.float64)
+    dev = [torch.zeros(dims.size, device="cuda") for _ in range(6)]
+    dmom = torch.zeros(n, device="cuda")
+    dfilt = torch.zeros(n * 5, dtype=torch.float64, device="cuda")
+    dpsrc = torch.from_numpy(psrc.reshape(-1).copy()).cuda()
+    dcols = [torch.from_numpy(c).cuda() for c in cols]
+    dmu = torch.from_numpy(mu).cuda()
+    st = torch.cuda.current_stream().cuda_stream
+    print(f"addkinsrc n={n} dims={dims}", flush=True)
+    for step in range(1, 10):
+        lib.addkinsrc_H(step, 3, dpsrc.data_ptr(), n, st, dmu.data_ptr(),
+                        *(c.data_ptr() for c in dcols), *(f.data_ptr() for f in dev),
+                        dmom.data_ptr(), dfilt.data_ptr(), 0)
+        ok.addkinsrc(step, psrc, mu, cols[:5], host, mom, filt, b, a, np.float32(DT),
+                     np.float32(DH), dims)
+    torch.cuda.synchronize()
+    _lib.check()
+    for name, g, w in [("mom", dmom, mom), ("filter", dfilt, filt)] + \
+            [(f"field{c}", dev[c], host[c]) for c in range(6)]:
+        g = g.cpu().numpy()
+        err = float(np.abs(g.astype(np.float64) - w).max()) / max(float(np.abs(w).max()), 1e-30)
+        assert err < 1e-5, (name, err)
+    assert np.count_nonzero(mom) == n
+    lib.SetDeviceFilterParameters(0, None, None)
