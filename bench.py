@@ -743,7 +743,7 @@ def bench_lagrange(args) -> None:
             "cuinterp_interp_H": {"avg_ms": rec_ms, "bytes": rec_bytes, "GBps": rec_gbs},
             "cusource_add_cartesian_H": {"avg_ms": src_ms, "bytes": src_bytes, "GBps": src_gbs}},
         "roofline": {"bound": "hbm", "achieved": rec_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": rec_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": rec_gbs / HBM_PEAK_GBS, "traffic": _lagrange_traffic(args, nx, ny, nz),
                      "kernel": "cuinterp_interp_H"},
         "cpu_baseline": None if args.no_cpu else lagrange_cpu_baseline(args, nx, ny, nz),
         "setup_s": setup_s,
@@ -874,6 +874,19 @@ def bench_kinematic(args) -> None:
         "cpu_baseline": None,
     }))
     lib.SetDeviceFilterParameters(0, None, None)
+
+
+def _lagrange_traffic(args, nx, ny, nz):
+    """HBM bytes per receiver launch from the committed rocprofv3 counter
+    summary, for the workload it was measured on (else None)."""
+    f = ROOT / "profiles" / "lagrange_512x512x256_traffic.json"
+    if (nx, ny, nz, args.receivers, args.subfaults, args.degree) != (512, 512, 256, 20000, 4000,
+                                                                      3) or not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
 
 
 def lagrange_cpu_baseline(args, nx, ny, nz) -> dict:
