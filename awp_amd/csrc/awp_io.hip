@@ -76,6 +76,14 @@ __global__ __launch_bounds__(256) void velbuffer_kernel(
     ko = b - k;
     koz = (b == V.nzt + AWP_ALIGN - 1) ? ko - 1 : ko;
   }
+  if (ko < 0 || koz < 0 || ko >= V.yline || koz >= V.yline) {
+    // a bathymetry level k below the surface index that leaves the column:
+    // no read outside the arrays, the sample reads NaN
+    const long long t = (long long)tk * V.rec_nxt * V.rec_nyt + (long long)tj * V.rec_nxt + ti;
+    Bufx[t] = Bufy[t] = Bufz[t] = __builtin_nanf("");
+    if (V.NVE == 3) Bufeta[t] = __builtin_nanf("");
+    return;
+  }
   const long long base = (long long)i * V.slice + (long long)j * V.yline;
   const long long t = (long long)((k - V.nbgz) / V.nskpz) * V.rec_nxt * V.rec_nyt +
                       (long long)((j - 2 - AWP_NGSL - V.nbgy) / V.nskpy) * V.rec_nxt +
@@ -229,6 +237,10 @@ void velbuffer_H(const float* u1, const float* v1, const float* w1, const float*
   if (nbgx < 0 || nedx >= g.nxt + 2 + AWP_NGSL || nbgy < 0 || nedy >= g.nyt + 2 + AWP_NGSL ||
       nbgz < 0 || (FOLLOWBATHY == 0 && nedz > g.nzt + AWP_ALIGN - 2)) {
     awp_set_error(AWP_ERR_ARG, "%s: sampling window leaves the padded grid", W);
+    return;
+  }
+  if (d_i == 0 && FOLLOWBATHY != 0 && bathy == nullptr) {
+    awp_set_error(AWP_ERR_ARG, "%s: FOLLOWBATHY on grid 0 needs the bathymetry array", W);
     return;
   }
   VbLaunch V;

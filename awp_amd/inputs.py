@@ -349,12 +349,39 @@ def receiver_filename(base: str, inp: Input, step: int, maxsteps: int) -> str:
     return f"{base}_{filestep:0{width}d}"
 
 
+def write_receiver_block(name: str, data: np.ndarray, current_write: int,
+                         num_writes: int) -> None:
+    """One host-full of receiver samples into a receiver file, as
+    mpi_io_idx_write places it (src/mpi/io.c:55-100 with the offsets of
+    receiver_init_indexed, receiver.c:80-91): receiver r's block of write w
+    (data[r], num_elements samples) goes to element offset
+    (r * num_writes + w) * num_elements, so each receiver's num_writes
+    blocks lie together.  data: (receivers, num_elements) float32, receiver
+    r being the r-th receiver of the input file."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    nq, ne = data.shape
+    if not 0 <= current_write < num_writes:
+        raise ValueError(f"write {current_write} of {num_writes}")
+    if num_writes == 1:
+        with open(name, "wb") as fp:
+            data.tofile(fp)
+        return
+    size = nq * num_writes * ne * 4
+    if not os.path.exists(name) or os.path.getsize(name) < size:
+        with open(name, "ab") as fp:
+            fp.truncate(size)
+    mm = np.memmap(name, dtype=np.float32, mode="r+", shape=(nq, num_writes, ne))
+    mm[:, current_write, :] = data
+    mm.flush()
+    del mm
+
+
 class ReceiverWriter:
     """receiver_write (receiver.c:50-76) for one rank: sample into the device
     buffer when the step is due, flush device-fulls to the host, and when the
-    host buffer is full write it, receiver-major (each receiver's samples
-    contiguous), into the current file; num_writes host-fulls go into one
-    file one after the other (mpi_io_idx_write, src/mpi/io.c:79-100)."""
+    host buffer is full write it into the current file, transposed to
+    receiver-major order; num_writes host-fulls share one file, each
+    receiver's blocks together (write_receiver_block)."""
 
     def __init__(self, interp, inp: Input, base: str, maxsteps: int, directory: str = "."):
         self.interp, self.inp, self.base = interp, inp, base
@@ -377,10 +404,9 @@ class ReceiverWriter:
             data = b.h_buffer.reshape(rows, b.num_elements).T.copy()
             name = os.path.join(self.directory, receiver_filename(self.base, self.inp, step,
                                                                   self.maxsteps))
-            mode = "r+b" if self.current_write and os.path.exists(name) else "wb"
-            with open(name, mode) as fp:
-                fp.seek(self.current_write * data.nbytes)
-                data.tofile(fp)
+            if self.current_write == 0 and os.path.exists(name):
+                os.remove(name)     # a new file of num_writes host-fulls
+            write_receiver_block(name, data, self.current_write, self.inp.num_writes)
             if name not in self.files:
                 self.files.append(name)
             self.current_write += 1

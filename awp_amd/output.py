@@ -73,7 +73,9 @@ class VolumeOutput:
         if self.copied[s] is not None:
             compute.wait_event(self.copied[s])
         bx, by, bz, beta = self.dev[s]
-        eta = neta if neta is not None else u1
+        if self.nve == 3 and neta is None:
+            raise ValueError("NVE = 3 writes the plastic strain: pass neta")
+        eta = neta if neta is not None else u1   # read only when NVE = 3
         nb, ne, ns = self.nbg, self.ned, self.nskp
         lib.velbuffer_H(u1.data_ptr(), v1.data_ptr(), w1.data_ptr(), eta.data_ptr(),
                         bx.data_ptr(), by.data_ptr(), bz.data_ptr(), beta.data_ptr(), self.nve,

@@ -71,17 +71,32 @@ def lithostatic_sigma2(dims: Dims, DH: float, d1: np.ndarray) -> np.ndarray:
     return s
 
 
+def hoek_brown_constants(GSI, mi, D=0.0):
+    """mb, s, a of the generalised Hoek-Brown criterion
+    sigma1 = sigma3 + sigma_ci (mb sigma3 / sigma_ci + s)^a (Hoek,
+    Carranza-Torres and Corkum 2002; mesh.c:1148-1150)."""
+    GSI = np.asarray(GSI, dtype=np.float64)
+    mb = mi * np.exp((GSI - 100.0) / (28.0 - 14.0 * D))
+    s = np.exp((GSI - 100.0) / (9.0 - 3.0 * D))
+    a = 0.5 + 1.0 / 6.0 * (np.exp(-GSI / 15.0) - np.exp(-20.0 / 3.0))
+    return mb, s, a
+
+
+def hoek_brown_sigma_cm(sigma_ci, GSI, mi, D=0.0):
+    """Global rock-mass strength sigma'_cm (mesh.c:1152-1153)."""
+    mb, s, a = hoek_brown_constants(GSI, mi, D)
+    return np.asarray(sigma_ci, dtype=np.float64) * (mb + 4.0 * s - a * (mb - 8.0 * s)) * (
+        mb / 4.0 + s) ** (a - 1.0) / (2.0 * (1.0 + a) * (2.0 + a))
+
+
 def hoek_brown(sigma_0, sigma_ci, GSI, mi, D=0.0, tunnel=1):
     """Generalised Hoek-Brown -> equivalent Mohr-Coulomb phi [deg], cohesion
     (Hoek et al. 2002; mesh.c:1139-1176).  Vectorised, float64."""
     sigma_0 = np.asarray(sigma_0, dtype=np.float64)
     GSI = np.asarray(GSI, dtype=np.float64)
     sigma_ci = np.asarray(sigma_ci, dtype=np.float64)
-    mb = mi * np.exp((GSI - 100.0) / (28.0 - 14.0 * D))
-    s = np.exp((GSI - 100.0) / (9.0 - 3.0 * D))
-    a = 0.5 + 1.0 / 6.0 * (np.exp(-GSI / 15.0) - np.exp(-20.0 / 3.0))
-    sigma_cm = sigma_ci * (mb + 4.0 * s - a * (mb - 8.0 * s)) * (mb / 4.0 + s) ** (a - 1.0) / (
-        2.0 * (1.0 + a) * (2.0 + a))
+    mb, s, a = hoek_brown_constants(GSI, mi, D)
+    sigma_cm = hoek_brown_sigma_cm(sigma_ci, GSI, mi, D)
     with np.errstate(divide="ignore", invalid="ignore"):
         if tunnel == 0:
             s3max = sigma_cm * 0.72 * (sigma_cm / sigma_0) ** (-0.91)

@@ -15,7 +15,10 @@ algorithmic traffic of 140 B per cell-update (23 per-cell arrays read,
 12 written, 4 B each); its duration is measured with HIP events around
 every stress launch on the launch stream.  cpu_baseline: the FP32 CPU
 oracle (a port of the reference loops, OpenMP) on a bounded sub-grid of the
-same workload.
+same workload.  adjoint: the metric's third quantity, the cost of one
+material gradient of the 256 x 256 x 128, 2000-step adjoint workload over
+one plain forward of it (checkpoint every 50 steps), with one
+finite-difference direction as a sanity check.
 """
 from __future__ import annotations
 
@@ -51,6 +54,9 @@ def parse():
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", dest="adjoint_check", action="store_false",
+                    help="anelastic workload: skip the adjoint-cost measurement that the "
+                         "default line carries (256x256x128, 2000 steps, one FD direction)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps with no event timing (for rocprofv3)")
@@ -99,7 +105,7 @@ def cpu_baseline(workload_fn, target_s: float) -> dict:
     import numpy as np
 
     from oracle.oracle import OracleSolver, set_threads
-    threads = int(os.environ.get("AWP_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    threads = host_threads()
     set_threads(threads)
     w = workload_fn(128, 128)
     med = w.medium()
@@ -117,14 +123,15 @@ def cpu_baseline(workload_fn, target_s: float) -> dict:
     cells = w.dims.ncells
     return {"value": cells * n / el, "unit": "cell-updates/s", "cores": threads, "kind": "port",
             "sample": f"FP32 oracle, {w.dims.nxt}x{w.dims.nyt}x{w.dims.nzt} sub-grid of the same "
-                      f"layered+random+Q medium, {n} steps in {el:.1f} s, {threads} OpenMP threads, "
+                      f"layered+random+Q medium, {n} steps in {el:.1f} s, {threads} OpenMP threads "
+                      f"(every CPU this job may use; the host has {os.cpu_count()}), "
                       f"cpu={_cpu_model()}"}
 
 
 def bench_cpu(args) -> None:
     """CPU baseline table: the oracle (the reference's kernel loops restated in
-    C++ with OpenMP) on this host's cores, at 1 thread and at all threads
-    (AWP_CPU_THREADS, else at most 16), in FP32 and FP64.  The elastic 64^3
+    C++ with OpenMP) on this host's cores, at 1 thread and at every CPU the
+    job may use (host_threads), in FP32 and FP64.  The elastic 64^3
     workload runs its 200 steps; the 512 x 512 x 256 anelastic workload runs
     until --cpu-seconds or 10 steps, whichever comes first.  One untimed step
     first touches the arrays."""
@@ -132,7 +139,7 @@ def bench_cpu(args) -> None:
 
     from awp_amd import configs
     from oracle.oracle import OracleSolver, set_threads
-    allt = int(os.environ.get("AWP_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    allt = host_threads()
     rows = []
     cases = (("elastic_64x64x64", configs.elastic_halfspace(n=64, nt=201), 200, None),
              ("anelastic_512x512x256", configs.anelastic(nt=11), 10, args.cpu_seconds))
@@ -160,7 +167,24 @@ def bench_cpu(args) -> None:
                 print(json.dumps(row), file=sys.stderr, flush=True)
         del med
     print(json.dumps({"metric": "CPU baseline (oracle on host cores)", "unit": "cell-updates/s",
-                      "cpu": _cpu_model(), "host_threads": os.cpu_count(), "rows": rows}))
+                      "cpu": _cpu_model(), "host_cpus": os.cpu_count(), "job_threads": allt,
+                      "rows": rows}))
+
+
+def host_threads() -> int:
+    """Threads for the CPU baseline: AWP_CPU_THREADS if set, else every CPU
+    this job may run on (OMP_NUM_THREADS where the host sets the job's CPU
+    share, else the process's CPU affinity)."""
+    if os.environ.get("AWP_CPU_THREADS"):
+        return int(os.environ["AWP_CPU_THREADS"])
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and int(omp) > 0:
+        return min(int(omp), aff)
+    return aff
 
 
 def _cpu_model() -> str:
@@ -334,7 +358,13 @@ def main():
                               if args.volume_output else "")},
         "roofline": {"bound": "hbm", "kernel": "dstrqc_H", "achieved": str_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes_per_cell": b_str,
+                     "traffic": traffic,
+                     "traffic_source": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and "
+                                        "WRITE_SIZE of this kernel at this size, scaled by a copy "
+                                        "kernel's counts (profiles/dstrqc_anelastic_512x512x256_"
+                                        "traffic.json); read from that file, not counted in this "
+                                        "run") if traffic is not None else None,
+                     "algorithmic_bytes_per_cell": b_str,
                      "cells_per_launch": cells, "avg_launch_ms": str_avg},
         "kernels": {"dvelcx_H_opt": {"avg_ms": vel_avg, "GBps": vel_gbs,
                                      "frac": vel_gbs / HBM_PEAK_GBS},
@@ -348,6 +378,10 @@ def main():
     }
     del sol
     torch.cuda.empty_cache()
+    if args.adjoint_check and args.storage == "f32":
+        # the metric's third quantity: gradient cost of the 256x256x128,
+        # 2000-step adjoint workload over one plain forward of it
+        out["adjoint"] = adjoint_cost(256, 256, 128, 2000, 50, 1)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(workload, args.cpu_seconds)
     else:
@@ -372,23 +406,21 @@ def copy_probe(nbytes: int) -> None:
     print(json.dumps({"copy_probe_bytes": n * 4, "GBps_read_plus_write": 2 * n * 4 / dt / 1e9}))
 
 
-def bench_adjoint(args) -> None:
+def adjoint_cost(nx: int, ny: int, nz: int, nt: int, checkpoint_every: int,
+                 fd_directions: int) -> dict:
     """Adjoint gradient: smooth background vs a 10 % Gaussian vs anomaly,
-    4 explosive sources, 8 x 8 surface receivers, 256 x 256 x 128 cells at
-    h = 100 m unless --nx/--ny/--nz say otherwise.  Times one plain forward
-    (FP32, the pmcl3d step) and one gradient (checkpointed forward with the
-    binary64 misfit, segment recomputation, adjoint sweep, mapping to
-    lam, mu, rho), and checks the gradient against central finite
+    4 explosive sources, 8 x 8 surface receivers, h = 100 m.  Times one
+    plain forward (FP32, the pmcl3d step) and one gradient (checkpointed
+    forward with the binary64 misfit, segment recomputation, adjoint sweep,
+    mapping to lam, mu, rho), and checks the gradient against central finite
     differences of the misfit along seeded random directions."""
     import numpy as np
     import torch
 
-    from awp_amd import _lib, configs
+    from awp_amd import configs
     from awp_amd.adjoint import AdjointProblem
     from awp_amd.solver import Solver
 
-    nx, ny, nz = _size(args, 256, 256, 128)
-    nt = args.nt
     t0 = time.perf_counter()
     bg, true, src, rec = configs.adjoint_pair(nx=nx, ny=ny, nz=nz, nt=nt)
     setup_s = time.perf_counter() - t0
@@ -398,7 +430,7 @@ def bench_adjoint(args) -> None:
     obs = obs_run.traces()
     del obs_run
     torch.cuda.empty_cache()
-    P = AdjointProblem(bg, src, rec, nt, obs, checkpoint_every=args.checkpoint_every)
+    P = AdjointProblem(bg, src, rec, nt, obs, checkpoint_every=checkpoint_every)
     lam0 = torch.from_numpy(bg.lam.copy())
     mu0 = torch.from_numpy(bg.mu.copy())
     rho0 = torch.from_numpy(bg.d1.copy())
@@ -422,7 +454,7 @@ def bench_adjoint(args) -> None:
     glam, gmu, grho = glam.cpu(), gmu.cpu(), grho.cpu()
     rng = np.random.default_rng(2024)
     fd = []
-    for n in range(args.fd_directions):
+    for n in range(fd_directions):
         dl, dm, dr = (torch.from_numpy(rng.standard_normal(a.shape)) * a
                       for a in (lam0, mu0, rho0))
         eps = 2e-3
@@ -434,20 +466,33 @@ def bench_adjoint(args) -> None:
         a = float((glam * dl).sum() + (gmu * dm).sum() + (grho * dr).sum())
         fd.append({"adjoint": a, "finite_difference": f, "rel_err": abs(f - a) / abs(a)})
         print(f"direction {n}: {fd[-1]}", file=sys.stderr, flush=True)
-    cells = nx * ny * nz
+    del P
+    torch.cuda.empty_cache()
+    return {"workload": f"adjoint_{nx}x{ny}x{nz}", "nt": nt, "checkpoint_every": checkpoint_every,
+            "value": t_grad / t_fwd, "unit": "gradient time / plain forward time",
+            "plain_forward_s": t_fwd, "gradient_s": t_grad,
+            "forward_cell_updates_per_s": nx * ny * nz * nt / t_fwd, "misfit": J,
+            "fd_check": fd, "fd_max_rel_err": max(x["rel_err"] for x in fd) if fd else None,
+            "setup_s": setup_s}
+
+
+def bench_adjoint(args) -> None:
+    """The adjoint workload alone (256 x 256 x 128 unless --nx/--ny/--nz
+    say otherwise), as one JSON line."""
+    nx, ny, nz = _size(args, 256, 256, 128)
+    r = adjoint_cost(nx, ny, nz, args.nt, args.checkpoint_every, args.fd_directions)
     out = {
-        "metric": METRIC, "value": t_grad / t_fwd, "unit": "gradient time / plain forward time",
-        "n_gpus": 1, "steps": nt, "warmup": 0, "ms_per_step": t_fwd * 1e3 / nt,
+        "metric": METRIC, "value": r["value"], "unit": r["unit"],
+        "n_gpus": 1, "steps": args.nt, "warmup": 0,
+        "ms_per_step": r["plain_forward_s"] * 1e3 / args.nt,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 wavefields, f64 misfit and gradient", "data": "synthetic",
-        "config": {"workload": f"adjoint_{nx}x{ny}x{nz}", "h_m": 100.0, "dt_s": 0.005, "nt": nt,
+        "config": {"workload": r["workload"], "h_m": 100.0, "dt_s": 0.005, "nt": args.nt,
                    "checkpoint_every": args.checkpoint_every, "sources": "4 explosive",
                    "receivers": "8 x 8 surface", "model": "smooth vs(z) + 10 % Gaussian anomaly"},
-        "plain_forward_s": t_fwd, "gradient_s": t_grad,
-        "forward_cell_updates_per_s": cells * nt / t_fwd, "misfit": J, "fd_check": fd,
-        "fd_max_rel_err": max(x["rel_err"] for x in fd) if fd else None,
-        "setup_s": setup_s, "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": None,
     }
+    out.update({k: v for k, v in r.items() if k not in ("value", "unit", "workload")})
     print(json.dumps(out))
 
 
@@ -510,12 +555,8 @@ def bench_topo(args) -> None:
         sol.record(sol.cur_step - 1)
         if check_every and sol.cur_step % check_every == 0:
             sol.check_scales()
-        if vout is not None:
-            vout.sample(sol.cur_step, sol.f["u1"], sol.f["v1"], sol.f["w1"])
         e[3].record(stream)
     torch.cuda.synchronize()
-    if vout is not None:
-        vout.close()
     wall = time.perf_counter() - t0
     _lib.check()
     vel_avg = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))

@@ -113,3 +113,19 @@ def test_topo_struct_layout_matches_c():
     want = [C.sizeof(AwpTopo), AwpTopo.rho.offset, AwpTopo.metrics_f.offset,
             AwpTopo.stream_i.offset, AwpTopo.constants_set.offset]
     assert got == want
+
+
+def test_velbuffer_needs_bathymetry_when_following_it():
+    """velbuffer_H with FOLLOWBATHY = 1 on grid 0 reads the bathymetry array:
+    a null one is an argument error, raised before any HIP call."""
+    from awp_amd import _lib
+    lib = _lib.load()
+    F, I = ctypes.c_float, ctypes.c_int
+    z9 = (F * 9)()
+    lib.SetDeviceConstValue((F * 1)(100.0), F(0.005), (I * 1)(8), (I * 1)(8), (I * 1)(8), 1,
+                            0.0, 0.0, z9, z9)
+    lib.awp_clear_error()
+    lib.velbuffer_H(*([None] * 8), 1, 0, 7, 1, 0, 7, 1, 0, 0, 1, 8, 8, 1, None, 1, None, 0)
+    assert lib.awp_last_error() == 1
+    assert b"bathymetry" in lib.awp_last_error_string()
+    lib.awp_clear_error()

@@ -159,3 +159,38 @@ def test_buffer_copy_to_host_known_answer():
         if b.is_device_full(step):
             b.copy_to_host(step)
     assert np.array_equal(b.h_buffer, [24, 24, 28, 28, 32, 32, 36, 36, 40, 40, 44, 44])
+
+
+def test_receiver_file_blocks_follow_the_indexed_view(tmp_path):
+    """mpi_io_idx_write's file view (src/mpi/io.c:55-100): receiver r's block
+    of write w lands at element (r * num_writes + w) * num_elements.  First
+    the known answer of the reference's test_mpi_indexed_write
+    (tests/mpi/test_io.c:186-240: 4 blocks of 2 elements, 2 writes of the
+    same data, block i holding global_data[i]); then distinct values per
+    receiver, write and element, which fix the order of all three."""
+    from awp_amd.inputs import write_receiver_block
+    name = str(tmp_path / "temp.bin")
+    global_data = np.arange(1, 5, dtype=np.float32)
+    data = np.repeat(global_data[:, None], 2, axis=1)
+    for w in range(2):
+        write_receiver_block(name, data, w, 2)
+    buf = np.fromfile(name, dtype=np.float32)
+    ans = np.zeros(16, dtype=np.float32)
+    for i in range(4):
+        for j in range(2):
+            for k in range(2):
+                ans[k + j * 2 + i * 2 * 2] = global_data[i]
+    assert np.array_equal(buf, ans)
+    name = str(tmp_path / "r.bin")
+    nq, ne, nw = 3, 5, 3
+    blocks = [np.arange(nq * ne, dtype=np.float32).reshape(nq, ne) + 100 * w for w in range(nw)]
+    for w in range(nw):
+        write_receiver_block(name, blocks[w], w, nw)
+    got = np.fromfile(name, dtype=np.float32).reshape(nq, nw, ne)
+    for r in range(nq):
+        for w in range(nw):
+            assert np.array_equal(got[r, w], blocks[w][r])
+    # one write per file: the transposed host buffer as it is
+    name = str(tmp_path / "one.bin")
+    write_receiver_block(name, blocks[0], 0, 1)
+    assert np.array_equal(np.fromfile(name, dtype=np.float32), blocks[0].ravel())

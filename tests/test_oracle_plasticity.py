@@ -174,3 +174,107 @@ def test_nve3_regions_cover_the_stress_region_once():
     assert np.all(cover[xl:xr + 1, yl:yr + 1] == 1)
     assert cover.sum() == dims.nxt * dims.nyt
     assert R["yls2"] % 2 == 0 and R["yre2"] % 2 == 1
+
+
+# ---------------------------------------------------------------------------
+# Plasticity setup (mesh.c:1139-1402): lithostatic sigma2, Hoek-Brown
+# equivalent Mohr-Coulomb parameters, principal-stress rotation.
+
+def test_lithostatic_sigma2_closed_forms():
+    """sigma2 at the cell centres is -g times the overburden: for constant
+    density -rho g depth, depth = (ktop - k + 1/2) DH (the depth at which
+    the kernels take the fluid pressure, kernel.cu:1294); for two layers the
+    density of each cell holds from its centre up to the next centre above,
+    so the interface sits at the centre of the deepest upper-layer cell.
+    The ghost level above the surface repeats the surface value; columns
+    i, j in [2, n + 18) and levels down to align - 2 are filled."""
+    from awp_amd.grid import NGSL2
+    from awp_amd.plasticity import lithostatic_sigma2
+    dims = Dims(6, 5, 30)
+    DH, g = 75.0, 9.81
+    top = dims.ktop
+    k = np.arange(ALIGN - 2, top + 1)
+    depth = (top - k + 0.5) * DH
+    i0, i1, j0, j1 = 2, dims.nxt + 2 + NGSL2, 2, dims.nyt + 2 + NGSL2
+    rho = np.full(dims.shape, 2700.0)
+    s = lithostatic_sigma2(dims, DH, rho)
+    np.testing.assert_allclose(s[i0:i1, j0:j1, ALIGN - 2:top + 1], -2700.0 * g * depth
+                               * np.ones((i1 - i0, j1 - j0, 1)), rtol=1e-13)
+    assert np.array_equal(s[:, :, top + 1], s[:, :, top])
+    assert not s[:i0].any() and not s[i1:].any() and not s[:, :j0].any()
+    assert not s[:, j1:].any() and not s[:, :, :ALIGN - 2].any()
+    # two layers: cells k > kI light, k <= kI dense
+    kI = top - 11
+    rho = np.where(np.arange(dims.shape[2]) > kI, 2200.0, 2900.0)[None, None, :] * np.ones(
+        dims.shape)
+    s = lithostatic_sigma2(dims, DH, rho)
+    D = (top - (kI + 1) + 0.5) * DH          # centre of the deepest light cell
+    want = -g * (2200.0 * np.minimum(depth, D) + 2900.0 * np.maximum(depth - D, 0.0))
+    np.testing.assert_allclose(s[4, 7, ALIGN - 2:top + 1], want, rtol=1e-13)
+
+
+def _mc_fit(sigma_ci, mb, s, a, lo, hi, n=400001):
+    """Least-squares line sigma1 = A + B sigma3 through the Hoek-Brown
+    envelope on [lo, hi]: the equivalent Mohr-Coulomb criterion of Hoek,
+    Carranza-Torres and Corkum (2002); returns phi [deg], cohesion, A."""
+    x = np.linspace(lo, hi, n)
+    y = x + sigma_ci * (mb * x / sigma_ci + s) ** a
+    B, A = np.polyfit(x, y, 1)
+    sp = (B - 1.0) / (B + 1.0)
+    return np.degrees(np.arcsin(sp)), A * (1.0 - sp) / (2.0 * np.sqrt(1.0 - sp * sp)), A
+
+
+@pytest.mark.parametrize("GSI,mi,sigma_ci,sigma_0", [(50.0, 10.0, 50e6, 20e6),
+                                                      (80.0, 25.0, 100e6, 5e6),
+                                                      (30.0, 7.0, 30e6, 50e6),
+                                                      (100.0, 12.0, 80e6, 10e6)])
+def test_hoek_brown_is_the_least_squares_mohr_coulomb_fit(GSI, mi, sigma_ci, sigma_0):
+    """The closed forms of phi' and c' (mesh.c:1139-1176) are the
+    least-squares Mohr-Coulomb line through the Hoek-Brown envelope
+    sigma1 = sigma3 + sigma_ci (mb sigma3/sigma_ci + s)^a over
+    sigma_t = -s sigma_ci/mb < sigma3 < sigma3max, with the published tunnel
+    relation sigma3max = 0.47 sigma_cm (sigma_cm/sigma_0)^-0.94; and the
+    rock-mass strength sigma'_cm is the intercept of the same fit over
+    sigma_t < sigma3 < sigma_ci/4.  GSI = 100 gives mb = mi, s = 1, a = 1/2
+    (intact rock).  The fits are evaluated numerically, independently of the
+    closed forms."""
+    from awp_amd.plasticity import hoek_brown, hoek_brown_constants, hoek_brown_sigma_cm
+    mb, s, a = hoek_brown_constants(GSI, mi)
+    if GSI == 100.0:
+        assert (mb, s, a) == (mi, 1.0, 0.5)
+    st = -s * sigma_ci / mb
+    scm = float(hoek_brown_sigma_cm(sigma_ci, GSI, mi))
+    _, _, A = _mc_fit(sigma_ci, mb, s, a, st, sigma_ci / 4.0)
+    assert abs(scm / A - 1.0) < 1e-5, (scm, A)
+    s3max = scm * 0.47 * (scm / sigma_0) ** (-0.94)
+    phi_fit, c_fit, _ = _mc_fit(sigma_ci, mb, s, a, st, s3max)
+    phi, c = hoek_brown(sigma_0, sigma_ci, GSI, mi, 0.0, 1)
+    assert abs(float(phi) / phi_fit - 1.0) < 1e-5, (phi, phi_fit)
+    assert abs(float(c) / c_fit - 1.0) < 1e-4, (c, c_fit)
+    # the slope relation (tunnel = 0) takes 0.72 and -0.91
+    s3max = scm * 0.72 * (scm / sigma_0) ** (-0.91)
+    phi_fit, c_fit, _ = _mc_fit(sigma_ci, mb, s, a, st, s3max)
+    phi, c = hoek_brown(sigma_0, sigma_ci, GSI, mi, 0.0, 0)
+    assert abs(float(phi) / phi_fit - 1.0) < 1e-5 and abs(float(c) / c_fit - 1.0) < 1e-4
+
+
+@pytest.mark.parametrize("s0", [0.0, 35.0, -120.0, 200.0])
+def test_rotation_frame(s0):
+    """rotation_matrix (mesh.c:1370-1402) with the reference's dips
+    (0, 90, 0) and strike (s0, 0, s0 + 90): row 0 is the horizontal unit
+    vector at azimuth s0 from x (the major principal direction), row 1 the
+    vertical (the intermediate principal stress is vertical, mesh.c:1204),
+    row 2 the horizontal normal; the frame is orthonormal to single
+    precision and RzT is its transpose."""
+    pp = PlasticParams(strike=(s0, 0.0, s0 + 90.0))
+    Rz, RzT = pp.rotation()
+    R = Rz.reshape(3, 3).astype(np.float64)
+    a = np.radians(s0)
+    np.testing.assert_allclose(R[0], [np.cos(a), np.sin(a), 0.0], atol=2e-7)
+    np.testing.assert_allclose(R[1], [0.0, 0.0, 1.0], atol=2e-7)
+    np.testing.assert_allclose(R[2], [-np.sin(a), np.cos(a), 0.0], atol=2e-7)
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=5e-7)
+    assert np.array_equal(RzT.reshape(3, 3), Rz.reshape(3, 3).T)
+    # a strike pair that is not 90 degrees apart is not a rotation
+    Rb = PlasticParams(strike=(s0, 0.0, s0 + 60.0)).rotation()[0].reshape(3, 3)
+    assert np.abs(Rb @ Rb.T - np.eye(3)).max() > 0.4

@@ -117,6 +117,80 @@ def test_boundary_tables_polynomial_exactness(tables):
     assert dv[0, 1] == -ds[0, 1]
 
 
+def _right_block(blk: dict, nrows: int, ncols: int) -> np.ndarray:
+    """Dense top-rows operator from an SBP right-boundary block: the listed
+    rows, then the last (interior) row shifted by one column per row."""
+    op, idx = np.array(blk["op_right"]), np.array(blk["idx_right"])
+    M = np.zeros((nrows, ncols))
+    for r in range(op.shape[0]):
+        M[r, idx[r]] += op[r]
+    last = np.nonzero(op[-1])[0]
+    for r in range(op.shape[0], nrows):
+        c = last + (r - op.shape[0] + 1)
+        if c.max() < ncols:
+            M[r, c] = op[-1, last]
+    return M
+
+
+def test_velocity_composite_top_rows_are_products_of_pinned_factors(tables):
+    """dhpz4r and dphz4r (optimized_velocity.cu:1056, 1110) are z-derivatives
+    of z-interpolants, D(P(.), interp="first") in the reference's generator
+    (tools/kernel_generation/scheme.py:150-163).  Each row is the product of
+    the one-sided derivative (with the traction penalty) and the
+    interpolation between the node and cell-centre grids.  The factors are
+    pinned on their own: TV_DHZ4R / TV_DZ4R by exactness on polynomials
+    vanishing at the surface, TM_PHZR and the node interpolation Pp (a
+    fixture from the generator's coefficient files) by exactness to
+    degree 1, and the product reproduces every coefficient of both
+    composite tables to rounding."""
+    import json
+    T = tables
+    blocks = json.loads((ROOT / "tests" / "golden" / "sbp_right_blocks.json").read_text())
+    # the kernel tables are the generator's operators
+    assert np.allclose(T["TV_DHZ4R"], _right_block(blocks["Dm_bc42"], 6, 8), rtol=0, atol=1e-15)
+    assert np.allclose(T["TV_DZ4R"], _right_block(blocks["Dp_bc42"], 6, 7), rtol=0, atol=1e-15)
+    assert np.allclose(T["TM_PHZR"], _right_block(blocks["Pm42"], 6, 8), rtol=0, atol=1e-15)
+    # Pp: cell-centre values -> node values, exact on degree 0 and 1
+    zn, zc = _grids(NZ)
+    Pp = _right_block(blocks["Pp42"], 12, 12)
+    degs = _exact_degrees(Pp[:6, :7], zc, zn, False, True, NZ)
+    assert degs[:2] == [0, 1], degs
+    # composites = products of the pinned factors
+    DhP = _right_block({"op_right": T["TV_DHZ4R"].tolist(),
+                        "idx_right": [list(range(8))] * 6}, 12, 12) @ Pp
+    assert np.abs(DhP[:6, :9] - T["TV_DHPZ4R"]).max() < 1e-15
+    assert not np.any(DhP[:6, 9:])
+    Pm = _right_block({"op_right": T["TM_PHZR"].tolist(),
+                       "idx_right": [list(range(8))] * 6}, 12, 12)
+    DP = _right_block({"op_right": T["TV_DZ4R"].tolist(),
+                       "idx_right": [list(range(7))] * 6}, 12, 12) @ Pm
+    assert np.abs(DP[:6, :9] - T["TV_DPHZ4R"]).max() < 1e-15
+    assert not np.any(DP[:6, 9:])
+    # and the generator's own composite files hold the same numbers
+    assert np.abs(np.array(blocks["DmPp_bc42"]["op_right"]) - T["TV_DHPZ4R"]).max() < 1e-15
+    assert np.abs(np.array(blocks["DpPm_bc42"]["op_right"]) - T["TV_DPHZ4R"]).max() < 1e-15
+
+
+def test_composite_product_check_catches_a_planted_coefficient(tables):
+    """A change of three composite coefficients that keeps the row exact on
+    (z - z_s) (which the exactness test cannot see) breaks the product
+    identity."""
+    import json
+    T = tables
+    blocks = json.loads((ROOT / "tests" / "golden" / "sbp_right_blocks.json").read_text())
+    bad = T["TV_DHPZ4R"].copy()
+    bad[2, 1] += 1e-3   # second difference over equally spaced cell centres:
+    bad[2, 2] -= 2e-3   # orthogonal to constants and to linear functions
+    bad[2, 3] += 1e-3
+    Pp = _right_block(blocks["Pp42"], 12, 12)
+    DhP = _right_block({"op_right": T["TV_DHZ4R"].tolist(),
+                        "idx_right": [list(range(8))] * 6}, 12, 12) @ Pp
+    zn, zc = _grids(NZ)
+    zb = NZ - 2
+    assert _exact_degrees(bad, zc, zc, True, True, NZ, zero_at=zb)[:1] == [0]
+    assert np.abs(DhP[:6, :9] - bad).max() > 1e-4
+
+
 # ---------------------------------------------------------------------------
 def _dims():
     return Dims(24, 20, NZ)
