@@ -9,20 +9,14 @@ from __future__ import annotations
 import ctypes as C
 from pathlib import Path
 
+from .refstructs import Grid3
+
 LIB_PATH = Path(__file__).resolve().parent / "libawp_hip.so"
 
 _fp = C.c_void_p  # device pointers travel as plain addresses
 _i = C.c_int
 _f = C.c_float
 _s = C.c_void_p  # hipStream_t
-
-
-class AwpGrid3(C.Structure):
-    """awp_grid3_t (passed by value to cuinterp_init / cuinterp_lagrange_h)."""
-    _fields_ = [("off_x", C.c_int), ("off_y", C.c_int), ("off_z", C.c_int),
-                ("line", C.c_int), ("slice", C.c_int),
-                ("size_x", C.c_int), ("size_y", C.c_int), ("size_z", C.c_int),
-                ("numel", C.c_longlong)]
 
 
 # name -> argtypes, in header order
@@ -66,7 +60,7 @@ SIGNATURES: dict[str, list] = {
     "awp_adjoint_map": [_fp] * 11 + [_s],
     "awp_dvelcx_oop": [_fp] * 16 + [_i, _i, _s, _i, _i, _i, _i],
     "awp_dstrqc_oop": [_fp] * 24 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,
-    # awp_topo_t* travels as a plain address (ctypes.byref of awp_amd.topography.AwpTopo)
+    # topo_t* travels as a plain address (ctypes.byref of awp_amd.refstructs.Topo)
     "topo_set_constants": [_fp],
     "topo_velocity_interior_H": [_fp],
     "topo_velocity_front_H": [_fp],
@@ -74,15 +68,18 @@ SIGNATURES: dict[str, list] = {
     "topo_stress_interior_H": [_fp],
     "topo_stress_left_H": [_fp],
     "topo_stress_right_H": [_fp],
+    "awp_topo_stress_interior_rates_H": [_fp, _fp],
     "awp_topo_velocity_s16": [_fp, _i, _fp],
-    "awp_topo_stress_s16": [_fp, _i, _fp],
+    "awp_topo_stress_s16": [_fp, _i, _fp, _fp],
     "awp_topo_geom_custom": [_fp, _i, _i, _i, _f, _s],
     "awp_topo_metrics_f": [_fp, _i, _i, _f, _s],
     "awp_topo_geom_no_stretching": [_fp, _i, _f, _s],
     "awp_topo_metrics_g": [_fp, _i, _f, _s],
-    # awp_interp_t* travels as a plain address (awp_amd.interpolation.AwpInterp)
-    "cuinterp_init": [_fp] * 4 + [AwpGrid3] + [_fp] * 4 + [_i, _i],
-    "cuinterp_lagrange_h": [_fp] * 4 + [AwpGrid3] + [_fp] * 3,
+    # cu_interp_t* travels as a plain address (awp_amd.refstructs.CuInterp)
+    "awp_interp_set_stream": [_fp, _s],
+    "awp_interp_num_cells": [_fp],
+    "cuinterp_init": [_fp] * 4 + [Grid3] + [_fp] * 4 + [_i, _i],
+    "cuinterp_lagrange_h": [_fp] * 4 + [Grid3] + [_fp] * 3,
     "cuinterp_htod": [_fp],
     "cuinterp_dtoh": [_fp],
     "cuinterp_malloc": [_fp],
@@ -96,6 +93,8 @@ RESTYPES = {
     "awp_last_error": _i,
     "awp_last_error_string": C.c_char_p,
     "awp_set_tuning": _i,
+    "awp_interp_set_stream": _i,
+    "awp_interp_num_cells": _i,
     "cuinterp_init": _i,
     "cuinterp_lagrange_h": _i,
     "cuinterp_htod": _i,

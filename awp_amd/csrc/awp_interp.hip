@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <climits>
+#include <map>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -142,10 +144,69 @@ int lagrange1_coef(float* xloc, float* l, int* first, const float* x, int n, flo
   return err;
 }
 
+// What the path reads of a grid3_t, in a compact form for kernel arguments.
+struct IGrid {
+  int off_x, off_y, off_z;    // offset1
+  int line, slice;
+  int size_x, size_y, size_z; // grid_boundary_size: coordinate array lengths
+  long long numel;            // mem.x * mem.y * mem.z: values in one field
+};
+
+// grid_boundary_size (src/grid/grid_3d.c:143-160): a node grid (shift 0)
+// with a closed upper boundary drops its last point.
+IGrid igrid(const grid3_t& g) {
+  IGrid o;
+  o.off_x = g.offset1.x;
+  o.off_y = g.offset1.y;
+  o.off_z = g.offset1.z;
+  o.line = g.line;
+  o.slice = g.slice;
+  o.size_x = g.size.x - (g.shift.x == 0 && g.boundary2.x == 1 ? 1 : 0);
+  o.size_y = g.size.y - (g.shift.y == 0 && g.boundary2.y == 1 ? 1 : 0);
+  o.size_z = g.size.z - (g.shift.z == 0 && g.boundary2.z == 1 ? 1 : 0);
+  o.numel = (long long)g.mem.x * g.mem.y * g.mem.z;
+  return o;
+}
+
 // grid_index (include/grid/grid_3d.h:113-115) in 64-bit arithmetic
-__host__ __device__ inline long long grid_pos(const awp_grid3_t& g, int i, int j, int k) {
+__host__ __device__ inline long long grid_pos(const IGrid& g, int i, int j, int k) {
   return (long long)g.off_z + k + (long long)(g.off_y + j) * g.line +
          (long long)(g.off_x + i) * g.slice;
+}
+
+// The source launchers' grid-point map of one cu_interp_t, built by
+// cuinterp_htod and kept library-side under the struct's d_ix (a device
+// allocation unique to that struct for as long as it lives): the grid points
+// the queries touch, each with its (query, basis point) terms in ascending
+// query order.
+struct SourceMap {
+  long long* d_cell_pos = nullptr;
+  int* d_cell_ptr = nullptr;   // num_cells + 1
+  int* d_cell_term = nullptr;  // ((q*nb + i)*nb + j)*nb + k for basis point (i, j, k)
+  int num_cells = 0;
+  int ix_max = -1, iy_max = -1, iz_max = -1;  // largest index + num_basis - 1
+  hipStream_t stream = nullptr;
+};
+std::mutex g_map_mutex;
+std::map<const void*, SourceMap> g_maps;
+
+void free_map(SourceMap& m) {
+  if (m.d_cell_pos) (void)hipFree(m.d_cell_pos);
+  if (m.d_cell_ptr) (void)hipFree(m.d_cell_ptr);
+  if (m.d_cell_term) (void)hipFree(m.d_cell_term);
+  m = SourceMap{};
+}
+
+// A copy of the map of I (an empty one with num_cells = -1 if there is none).
+SourceMap find_map(const cu_interp_t* I) {
+  std::lock_guard<std::mutex> lock(g_map_mutex);
+  auto it = g_maps.find(I->d_ix);
+  if (it == g_maps.end()) {
+    SourceMap none;
+    none.num_cells = -1;
+    return none;
+  }
+  return it->second;
 }
 
 // Field layouts the curvilinear and force sources read (source.cu:79-80, 131-140):
@@ -173,7 +234,7 @@ __global__ __launch_bounds__(kInterpThreads) void interp_kernel(
     float* __restrict__ out, const float* __restrict__ in, const float* __restrict__ lx,
     const float* __restrict__ ly, const float* __restrict__ lz, const int nb_arg,
     const int* __restrict__ ix, const int* __restrict__ iy, const int* __restrict__ iz,
-    const int* __restrict__ ridx, const int nq, const awp_grid3_t g) {
+    const int* __restrict__ ridx, const int nq, const IGrid g) {
   const int nb = NB > 0 ? NB : nb_arg;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
@@ -259,17 +320,7 @@ __global__ __launch_bounds__(kThreads) void source_kernel(
   out[p] = acc;
 }
 
-void free_map(awp_interp_t* I) {
-  if (I->d_cell_pos) (void)hipFree(I->d_cell_pos);
-  if (I->d_cell_ptr) (void)hipFree(I->d_cell_ptr);
-  if (I->d_cell_term) (void)hipFree(I->d_cell_term);
-  I->d_cell_pos = nullptr;
-  I->d_cell_ptr = nullptr;
-  I->d_cell_term = nullptr;
-  I->num_cells = 0;
-}
-
-bool launch_ok(const awp_interp_t* I, const void* out, const void* in, const char* where) {
+bool launch_ok(const cu_interp_t* I, const void* out, const void* in, const char* where) {
   if (!I || !out || !in) {
     awp_set_error(AWP_ERR_ARG, "%s: null argument", where);
     return false;
@@ -283,33 +334,37 @@ bool launch_ok(const awp_interp_t* I, const void* out, const void* in, const cha
 }
 
 // Indices of the source map must address the metric, g and density arrays.
-bool jac_ok(const awp_interp_t* I, long long f_rows, int fline, int g_len, const Jac* J,
-            int nx, int ny, int nz, const char* where) {
-  const bool f_in = I->ix_max + 2 < f_rows && I->iy_max + AWP_ALIGN + 2 < fline;
-  const bool g_in = I->iz_max + AWP_ALIGN < g_len;
+bool jac_ok(const SourceMap& M, long long f_rows, int fline, int g_len, const Jac* J, int nx,
+            int ny, int nz, const char* where) {
+  const bool f_in = M.ix_max + 2 < f_rows && M.iy_max + AWP_ALIGN + 2 < fline;
+  const bool g_in = M.iz_max + AWP_ALIGN < g_len;
   bool r_in = true;
   if (J && J->rho)
-    r_in = I->ix_max + AWP_NGSL + 2 < nx + 4 + AWP_NGSL2 &&
-           I->iy_max + AWP_NGSL + 2 < ny + 4 + AWP_NGSL2 && I->iz_max + AWP_ALIGN < nz + 2 * AWP_ALIGN;
+    r_in = M.ix_max + AWP_NGSL + 2 < nx + 4 + AWP_NGSL2 &&
+           M.iy_max + AWP_NGSL + 2 < ny + 4 + AWP_NGSL2 && M.iz_max + AWP_ALIGN < nz + 2 * AWP_ALIGN;
   if (!f_in || !g_in || !r_in) {
     awp_set_error(AWP_ERR_ARG,
                   "%s: stencil indices (up to %d, %d, %d) leave the metric/density arrays", where,
-                  I->ix_max, I->iy_max, I->iz_max);
+                  M.ix_max, M.iy_max, M.iz_max);
+    return false;
+  }
+  return true;
+}
+
+bool map_ok(const SourceMap& M, const char* where) {
+  if (M.num_cells < 0 || (M.num_cells > 0 && (!M.d_cell_pos || !M.d_cell_ptr || !M.d_cell_term))) {
+    awp_set_error(AWP_ERR_STATE, "%s: no grid-point map (cuinterp_htod)", where);
     return false;
   }
   return true;
 }
 
 template <int KIND>
-void source_launch(const awp_interp_t* I, float* out, const float* in, float h, float dt,
-                   const Jac& J, const char* where) {
-  if (I->num_cells == 0) return;
-  if (!I->d_cell_pos || !I->d_cell_ptr || !I->d_cell_term) {
-    awp_set_error(AWP_ERR_STATE, "%s: no grid-point map (cuinterp_htod)", where);
-    return;
-  }
+void source_launch(const cu_interp_t* I, const SourceMap& M, float* out, const float* in,
+                   float h, float dt, const Jac& J, const char* where) {
+  if (M.num_cells == 0) return;
   const float dth = dt / (h * h * h);  // source.cu:38
-  const int blocks = (I->num_cells + kThreads - 1) / kThreads;
+  const int blocks = (M.num_cells + kThreads - 1) / kThreads;
   auto kern = source_kernel<KIND, 0>;
   switch (I->num_basis) {
     case 1: kern = source_kernel<KIND, 1>; break;
@@ -318,9 +373,9 @@ void source_launch(const awp_interp_t* I, float* out, const float* in, float h, 
     case 4: kern = source_kernel<KIND, 4>; break;
     default: break;
   }
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), 0, I->stream, out, in, I->d_lx, I->d_ly,
-                     I->d_lz, I->num_basis, I->d_ix, I->d_iy, I->d_iz, I->d_ridx, I->d_cell_pos,
-                     I->d_cell_ptr, I->d_cell_term, I->num_cells, -dth, J);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), 0, M.stream, out, in, I->d_lx, I->d_ly,
+                     I->d_lz, I->num_basis, I->d_ix, I->d_iy, I->d_iz, I->d_ridx, M.d_cell_pos,
+                     M.d_cell_ptr, M.d_cell_term, M.num_cells, -dth, J);
   awp_check_launch(where);
 }
 
@@ -328,7 +383,7 @@ void source_launch(const awp_interp_t* I, float* out, const float* in, float h, 
 
 extern "C" {
 
-int cuinterp_malloc(awp_interp_t* I) {
+int cuinterp_malloc(cu_interp_t* I) {
   I->lx = (float*)calloc(I->size_l ? I->size_l : 1, sizeof(float));
   I->ly = (float*)calloc(I->size_l ? I->size_l : 1, sizeof(float));
   I->lz = (float*)calloc(I->size_l ? I->size_l : 1, sizeof(float));
@@ -338,11 +393,6 @@ int cuinterp_malloc(awp_interp_t* I) {
   I->ridx = (int*)calloc(I->size_i ? I->size_i : 1, sizeof(int));
   I->d_lx = I->d_ly = I->d_lz = nullptr;
   I->d_ix = I->d_iy = I->d_iz = I->d_ridx = nullptr;
-  I->d_cell_pos = nullptr;
-  I->d_cell_ptr = nullptr;
-  I->d_cell_term = nullptr;
-  I->num_cells = 0;
-  I->ix_max = I->iy_max = I->iz_max = -1;
   if (!I->lx || !I->ly || !I->lz || !I->ix || !I->iy || !I->iz || !I->ridx)
     return kErrInterpMalloc;
   const size_t bl = sizeof(float) * I->size_l, bi = sizeof(int) * I->size_i;
@@ -360,9 +410,10 @@ int cuinterp_malloc(awp_interp_t* I) {
   return kSuccess;
 }
 
-int cuinterp_lagrange_h(awp_interp_t* H, const float* x, const float* y, const float* z,
-                        awp_grid3_t grid, const float* qx, const float* qy, const float* qz) {
+int cuinterp_lagrange_h(cu_interp_t* H, const float* x, const float* y, const float* z,
+                        grid3_t grid3, const float* qx, const float* qy, const float* qz) {
   const int nb = H->num_basis, deg = nb - 1;
+  const IGrid grid = igrid(grid3);
   if (nb < 1 || nb > kMaxBasis) return kErrInconsistentSize;
   if (grid.size_x < nb || grid.size_y < nb || grid.size_z < nb) return kErrInconsistentSize;
   float loc[kMaxBasis];
@@ -376,8 +427,10 @@ int cuinterp_lagrange_h(awp_interp_t* H, const float* x, const float* y, const f
   return err;
 }
 
-int cuinterp_htod(awp_interp_t* I) {
+int cuinterp_htod(cu_interp_t* I) {
   const int nb = I->num_basis, nq = I->num_query;
+  const IGrid grid = igrid(I->grid);
+  if (!I->d_ix) return kErrInterpMalloc;
   if (nb < 1 || nb > kMaxBasis || nq < 1) return kErrInconsistentSize;
   const long long nb3 = (long long)nb * nb * nb, nterm = nb3 * nq;
   if (nterm > INT_MAX) return kErrInconsistentSize;
@@ -392,8 +445,8 @@ int cuinterp_htod(awp_interp_t* I) {
     for (int i = 0; i < nb; ++i)
       for (int j = 0; j < nb; ++j)
         for (int k = 0; k < nb; ++k) {
-          const long long p = grid_pos(I->grid, I->ix[q] + i, I->iy[q] + j, I->iz[q] + k);
-          if (p < 0 || p >= I->grid.numel) return kErrGridOutOfBounds;
+          const long long p = grid_pos(grid, I->ix[q] + i, I->iy[q] + j, I->iz[q] + k);
+          if (p < 0 || p >= grid.numel) return kErrGridOutOfBounds;
           const int t = ((q * nb + i) * nb + j) * nb + k;
           pt[(size_t)t] = {p, t};
         }
@@ -411,7 +464,17 @@ int cuinterp_htod(awp_interp_t* I) {
   ptr.push_back((int)nterm);
 
   const size_t bl = sizeof(float) * (size_t)nq * nb, bi = sizeof(int) * (size_t)nq;
-  free_map(I);
+  SourceMap M;
+  {
+    std::lock_guard<std::mutex> lock(g_map_mutex);
+    auto it = g_maps.find(I->d_ix);
+    if (it != g_maps.end()) {
+      M.stream = it->second.stream;
+      free_map(it->second);
+      g_maps.erase(it);
+    }
+  }
+  const hipStream_t keep_stream = M.stream;
   bool ok = hipMemcpy(I->d_lx, I->lx, bl, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(I->d_ly, I->ly, bl, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(I->d_lz, I->lz, bl, hipMemcpyHostToDevice) == hipSuccess &&
@@ -419,29 +482,32 @@ int cuinterp_htod(awp_interp_t* I) {
             hipMemcpy(I->d_iy, I->iy, bi, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(I->d_iz, I->iz, bi, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemcpy(I->d_ridx, I->ridx, bi, hipMemcpyHostToDevice) == hipSuccess;
-  ok = ok && hipMalloc((void**)&I->d_cell_pos, sizeof(long long) * pos.size()) == hipSuccess &&
-       hipMalloc((void**)&I->d_cell_ptr, sizeof(int) * ptr.size()) == hipSuccess &&
-       hipMalloc((void**)&I->d_cell_term, sizeof(int) * term.size()) == hipSuccess;
+  ok = ok && hipMalloc((void**)&M.d_cell_pos, sizeof(long long) * pos.size()) == hipSuccess &&
+       hipMalloc((void**)&M.d_cell_ptr, sizeof(int) * ptr.size()) == hipSuccess &&
+       hipMalloc((void**)&M.d_cell_term, sizeof(int) * term.size()) == hipSuccess;
   ok = ok &&
-       hipMemcpy(I->d_cell_pos, pos.data(), sizeof(long long) * pos.size(),
+       hipMemcpy(M.d_cell_pos, pos.data(), sizeof(long long) * pos.size(),
                  hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(I->d_cell_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice) ==
+       hipMemcpy(M.d_cell_ptr, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice) ==
            hipSuccess &&
-       hipMemcpy(I->d_cell_term, term.data(), sizeof(int) * term.size(),
+       hipMemcpy(M.d_cell_term, term.data(), sizeof(int) * term.size(),
                  hipMemcpyHostToDevice) == hipSuccess;
   if (!ok) {
-    free_map(I);
+    free_map(M);
     awp_set_error(AWP_ERR_HIP, "cuinterp_htod: device copy failed");
     return kErrInterpMalloc;
   }
-  I->num_cells = (int)pos.size();
-  I->ix_max = xm;
-  I->iy_max = ym;
-  I->iz_max = zm;
+  M.num_cells = (int)pos.size();
+  M.ix_max = xm;
+  M.iy_max = ym;
+  M.iz_max = zm;
+  M.stream = keep_stream;
+  std::lock_guard<std::mutex> lock(g_map_mutex);
+  g_maps[I->d_ix] = M;
   return kSuccess;
 }
 
-int cuinterp_dtoh(awp_interp_t* I) {
+int cuinterp_dtoh(cu_interp_t* I) {
   const size_t bl = sizeof(float) * (size_t)I->num_query * I->num_basis;
   const size_t bi = sizeof(int) * (size_t)I->num_query;
   const bool ok = hipMemcpy(I->lx, I->d_lx, bl, hipMemcpyDeviceToHost) == hipSuccess &&
@@ -458,24 +524,18 @@ int cuinterp_dtoh(awp_interp_t* I) {
   return kSuccess;
 }
 
-int cuinterp_init(awp_interp_t* out, const float* x, const float* y, const float* z,
-                  awp_grid3_t grid, const float* qx, const float* qy, const float* qz,
+int cuinterp_init(cu_interp_t* out, const float* x, const float* y, const float* z,
+                  grid3_t grid, const float* qx, const float* qy, const float* qz,
                   const int* indices, int num_query, int degree) {
   out->num_basis = degree + 1;
   out->num_query = num_query;
   out->size_l = (size_t)(num_query > 0 ? num_query : 0) * (size_t)(degree + 1);
   out->size_i = (size_t)(num_query > 0 ? num_query : 0);
   out->grid = grid;
-  out->stream = nullptr;
   out->lx = out->ly = out->lz = nullptr;
   out->ix = out->iy = out->iz = out->ridx = nullptr;
   out->d_lx = out->d_ly = out->d_lz = nullptr;
   out->d_ix = out->d_iy = out->d_iz = out->d_ridx = nullptr;
-  out->d_cell_pos = nullptr;
-  out->d_cell_ptr = nullptr;
-  out->d_cell_term = nullptr;
-  out->num_cells = 0;
-  out->ix_max = out->iy_max = out->iz_max = -1;
   if (num_query <= 0) return kErrNonPositive;
   if (degree < 0 || degree + 1 > kMaxBasis) return kErrInconsistentSize;
   int err = cuinterp_malloc(out);
@@ -486,7 +546,15 @@ int cuinterp_init(awp_interp_t* out, const float* x, const float* y, const float
   return cuinterp_htod(out);
 }
 
-void cuinterp_finalize(awp_interp_t* I) {
+void cuinterp_finalize(cu_interp_t* I) {
+  if (I->d_ix) {
+    std::lock_guard<std::mutex> lock(g_map_mutex);
+    auto it = g_maps.find(I->d_ix);
+    if (it != g_maps.end()) {
+      free_map(it->second);
+      g_maps.erase(it);
+    }
+  }
   free(I->lx);
   free(I->ly);
   free(I->lz);
@@ -504,11 +572,25 @@ void cuinterp_finalize(awp_interp_t* I) {
     if (p) (void)hipFree(p);
   I->d_lx = I->d_ly = I->d_lz = nullptr;
   I->d_ix = I->d_iy = I->d_iz = I->d_ridx = nullptr;
-  free_map(I);
 }
 
-void cuinterp_interp_H(const awp_interp_t* I, float* out, const float* in) {
+int awp_interp_num_cells(const cu_interp_t* I) {
+  if (!I) return -1;
+  return find_map(I).num_cells;
+}
+
+int awp_interp_set_stream(const cu_interp_t* I, hipStream_t stream) {
+  if (!I) return kErrGridOutOfBounds;
+  std::lock_guard<std::mutex> lock(g_map_mutex);
+  auto it = g_maps.find(I->d_ix);
+  if (it == g_maps.end()) return kErrGridOutOfBounds;
+  it->second.stream = stream;
+  return kSuccess;
+}
+
+void cuinterp_interp_H(const cu_interp_t* I, float* out, const float* in) {
   if (!launch_ok(I, out, in, "cuinterp_interp_H")) return;
+  const SourceMap M = find_map(I);
   const int blocks = (I->num_query + kInterpThreads - 1) / kInterpThreads;
   auto kern = interp_kernel<0>;
   switch (I->num_basis) {
@@ -518,23 +600,28 @@ void cuinterp_interp_H(const awp_interp_t* I, float* out, const float* in) {
     case 4: kern = interp_kernel<4>; break;
     default: break;
   }
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kInterpThreads), 0, I->stream, out, in, I->d_lx,
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(kInterpThreads), 0, M.stream, out, in, I->d_lx,
                      I->d_ly, I->d_lz, I->num_basis, I->d_ix, I->d_iy, I->d_iz, I->d_ridx,
-                     I->num_query, I->grid);
+                     I->num_query, igrid(I->grid));
   awp_check_launch("cuinterp_interp_H");
 }
 
-void cusource_add_cartesian_H(const awp_interp_t* I, float* out, const float* in, float h,
+void cusource_add_cartesian_H(const cu_interp_t* I, float* out, const float* in, float h,
                               float dt) {
-  if (!launch_ok(I, out, in, "cusource_add_cartesian_H")) return;
+  const char* where = "cusource_add_cartesian_H";
+  if (!launch_ok(I, out, in, where)) return;
+  const SourceMap M = find_map(I);
+  if (!map_ok(M, where)) return;
   const Jac J = {nullptr, nullptr, nullptr, 0, 0, 0, 0.0f};
-  source_launch<kCartesian>(I, out, in, h, dt, J, "cusource_add_cartesian_H");
+  source_launch<kCartesian>(I, M, out, in, h, dt, J, where);
 }
 
-void cusource_add_curvilinear_H(const awp_interp_t* I, float* out, const float* in, float h,
+void cusource_add_curvilinear_H(const cu_interp_t* I, float* out, const float* in, float h,
                                 float dt, const float* f, int ny, const float* dg) {
   const char* where = "cusource_add_curvilinear_H";
   if (!launch_ok(I, out, in, where)) return;
+  const SourceMap M = find_map(I);
+  if (!map_ok(M, where)) return;
   if (!f || !dg) {
     awp_set_error(AWP_ERR_ARG, "%s: null metric array", where);
     return;
@@ -542,17 +629,20 @@ void cusource_add_curvilinear_H(const awp_interp_t* I, float* out, const float* 
   const int fline = 2 * AWP_ALIGN + AWP_NGSL2 + ny + 4;
   // The metric array has as many rows as the field has slices (nx + 20) and
   // dg as many levels as a field line (nz + 64).
-  const long long f_rows = I->grid.slice > 0 ? I->grid.numel / I->grid.slice : 0;
-  if (!jac_ok(I, f_rows, fline, I->grid.line, nullptr, 0, 0, 0, where)) return;
+  const IGrid G = igrid(I->grid);
+  const long long f_rows = G.slice > 0 ? G.numel / G.slice : 0;
+  if (!jac_ok(M, f_rows, fline, G.line, nullptr, 0, 0, 0, where)) return;
   const Jac J = {f, dg, nullptr, fline, 0, 0, 0.0f};
-  source_launch<kCurvilinear>(I, out, in, h, dt, J, where);
+  source_launch<kCurvilinear>(I, M, out, in, h, dt, J, where);
 }
 
-void cusource_add_force_H(const awp_interp_t* I, float* out, const float* in, const float* d1,
+void cusource_add_force_H(const cu_interp_t* I, float* out, const float* in, const float* d1,
                           float h, float dt, float quad_weight, const float* f, int nx, int ny,
                           int nz, const float* dg) {
   const char* where = "cusource_add_force_H";
   if (!launch_ok(I, out, in, where)) return;
+  const SourceMap M = find_map(I);
+  if (!map_ok(M, where)) return;
   if (!f || !dg || !d1) {
     awp_set_error(AWP_ERR_ARG, "%s: null metric or density array", where);
     return;
@@ -560,9 +650,9 @@ void cusource_add_force_H(const awp_interp_t* I, float* out, const float* in, co
   const int fline = 2 * AWP_ALIGN + AWP_NGSL2 + ny + 4;
   const int rline = 2 * AWP_ALIGN + nz;
   const Jac J = {f, dg, d1, fline, rline, (long long)rline * (AWP_NGSL2 + ny + 4), quad_weight};
-  if (!jac_ok(I, (long long)nx + 4 + AWP_NGSL2, fline, nz + 2 * AWP_ALIGN, &J, nx, ny, nz, where))
+  if (!jac_ok(M, (long long)nx + 4 + AWP_NGSL2, fline, nz + 2 * AWP_ALIGN, &J, nx, ny, nz, where))
     return;
-  source_launch<kForce>(I, out, in, h, dt, J, where);
+  source_launch<kForce>(I, M, out, in, h, dt, J, where);
 }
 
 }  // extern "C"

@@ -888,7 +888,15 @@ __global__ __launch_bounds__(256) void metrics_g_kernel(const float* g0, float* 
 
 // ---------------------------------------------------------------------------
 // Host side.
-bool topo_geom(const awp_topo_t* T, const char* where, TGeom* G) {
+// The stress kernels' constants (set_constants, stress_attenuation.cu:39-72),
+// process-wide like the reference's __constant__ block.
+struct TopoConstants {
+  float DT, dth, dh1;
+  int set;
+};
+TopoConstants g_topo_constants = {0.0f, 0.0f, 0.0f, 0};
+
+bool topo_geom(const topo_t* T, const char* where, TGeom* G) {
   if (!T) {
     awp_set_error(AWP_ERR_ARG, "%s: topo_t is NULL", where);
     return false;
@@ -915,9 +923,9 @@ bool topo_geom(const awp_topo_t* T, const char* where, TGeom* G) {
   return true;
 }
 
-TMetrics topo_metrics(const awp_topo_t* T) {
-  const awp_f_grid_t& f = T->metrics_f;
-  const awp_g_grid_t& g = T->metrics_g;
+TMetrics topo_metrics(const topo_t* T) {
+  const f_grid_t& f = T->metrics_f;
+  const g_grid_t& g = T->metrics_g;
   TMetrics M;
   M.f = f.d_f;
   M.f_1 = f.d_f_1;
@@ -948,7 +956,7 @@ bool metrics_present(const TMetrics& M, const char* where) {
 }
 
 template <class ST>
-void topo_velocity_launch(awp_topo_t* T, const WaveScales& W, const char* where) {
+void topo_velocity_launch(topo_t* T, const WaveScales& W, const char* where) {
   TGeom G;
   if (!topo_geom(T, where, &G)) return;
   if (!T->use) return;
@@ -1004,11 +1012,11 @@ void topo_velocity_launch(awp_topo_t* T, const WaveScales& W, const char* where)
 }
 
 template <class ST>
-void topo_stress_launch(awp_topo_t* T, const WaveScales& W, const char* where) {
+void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float* rates) {
   TGeom G;
   if (!topo_geom(T, where, &G)) return;
   if (!T->use) return;
-  if (!T->constants_set) {
+  if (!g_topo_constants.set) {
     awp_set_error(AWP_ERR_STATE, "%s: topo_set_constants has not been called", where);
     return;
   }
@@ -1043,9 +1051,9 @@ void topo_stress_launch(awp_topo_t* T, const WaveScales& W, const char* where) {
   A.wwo = T->wwo;
   A.M = topo_metrics(T);
   A.W = W;
-  A.DT = T->c_DT;
-  A.dth = T->c_dth;
-  A.dh1 = T->c_dh1;
+  A.DT = g_topo_constants.DT;
+  A.dth = g_topo_constants.dth;
+  A.dh1 = g_topo_constants.dh1;
   if (!metrics_present(A.M, where)) return;
   const void* need[] = {T->xx, T->yy, T->zz, T->xy, T->xz, T->yz, T->r1, T->r2, T->r3,
                         T->r4, T->r5, T->r6, T->u1, T->v1, T->w1, T->lami, T->mui,
@@ -1076,19 +1084,19 @@ void topo_stress_launch(awp_topo_t* T, const WaveScales& W, const char* where) {
   V.ichunk = awp_tuning().topo_str_ichunk;
   dim3 block(64, ST_BJ, 1);
   dim3 grid(V.nint + 1, awp_cdiv(G.ej - G.bj, ST_BJ), awp_cdiv(G.ei - G.bi, V.ichunk));
-  if (!T->rates) {
+  if (!rates) {
     hipLaunchKernelGGL((topo_str_kernel<ST, false>), grid, block, 0, st, A, G, V, nullptr, 0ll);
     awp_check_launch(where);
     return;
   }
-  // two passes: strain rates into T->rates, then the material and memory-
+  // two passes: strain rates into rates, then the material and memory-
   // variable update streaming them back
   const long long rstride = (long long)G.sl * (T->nx + 4 + AWP_NGSL2);
-  hipLaunchKernelGGL((topo_str_kernel<ST, true>), grid, block, 0, st, A, G, V, T->rates,
+  hipLaunchKernelGGL((topo_str_kernel<ST, true>), grid, block, 0, st, A, G, V, rates,
                      rstride);
   const int qchunk = awp_tuning().topo_q_ichunk;
   dim3 gq(awp_cdiv(T->nz - 3, 64), awp_cdiv(G.ej - G.bj, 4), awp_cdiv(G.ei - G.bi, qchunk));
-  hipLaunchKernelGGL(topo_str_q_kernel<ST>, gq, dim3(64, 4), 0, st, A, G, T->rates, rstride,
+  hipLaunchKernelGGL(topo_str_q_kernel<ST>, gq, dim3(64, 4), 0, st, A, G, rates, rstride,
                      qchunk);
   awp_check_launch(where);
 }
@@ -1108,7 +1116,7 @@ bool neighbourless(int rank, const char* where) {
 
 extern "C" {
 
-void topo_set_constants(awp_topo_t* T) {
+void topo_set_constants(topo_t* T) {
   if (!T) {
     awp_set_error(AWP_ERR_ARG, "topo_set_constants: topo_t is NULL");
     return;
@@ -1116,37 +1124,45 @@ void topo_set_constants(awp_topo_t* T) {
   // set_constants(gridspacing, dth * gridspacing, ...) (stress_attenuation.cu:19-72)
   const float h = T->gridspacing;
   const float dt = T->dth * h;
-  T->c_DT = dt;
-  T->c_dth = dt / h;
-  T->c_dh1 = (float)(1.0 / (double)h);
-  T->constants_set = 1;
+  g_topo_constants.DT = dt;
+  g_topo_constants.dth = dt / h;
+  g_topo_constants.dh1 = (float)(1.0 / (double)h);
+  g_topo_constants.set = 1;
 }
 
-void topo_velocity_interior_H(awp_topo_t* T) {
+void topo_velocity_interior_H(topo_t* T) {
   topo_velocity_launch<StF32>(T, WaveScales{}, "topo_velocity_interior_H");
 }
 
-void topo_velocity_front_H(awp_topo_t* T) {
+void topo_velocity_front_H(topo_t* T) {
   if (T && T->use) neighbourless(T->y_rank_f, "topo_velocity_front_H");
 }
 
-void topo_velocity_back_H(awp_topo_t* T) {
+void topo_velocity_back_H(topo_t* T) {
   if (T && T->use) neighbourless(T->y_rank_b, "topo_velocity_back_H");
 }
 
-void topo_stress_interior_H(awp_topo_t* T) {
-  topo_stress_launch<StF32>(T, WaveScales{}, "topo_stress_interior_H");
+void topo_stress_interior_H(topo_t* T) {
+  topo_stress_launch<StF32>(T, WaveScales{}, "topo_stress_interior_H", nullptr);
 }
 
-void topo_stress_left_H(awp_topo_t* T) {
+void awp_topo_stress_interior_rates_H(topo_t* T, float* rates) {
+  if (!rates) {
+    awp_set_error(AWP_ERR_ARG, "awp_topo_stress_interior_rates_H: rates is NULL");
+    return;
+  }
+  topo_stress_launch<StF32>(T, WaveScales{}, "awp_topo_stress_interior_rates_H", rates);
+}
+
+void topo_stress_left_H(topo_t* T) {
   if (T && T->use) neighbourless(T->x_rank_l, "topo_stress_left_H");
 }
 
-void topo_stress_right_H(awp_topo_t* T) {
+void topo_stress_right_H(topo_t* T) {
   if (T && T->use) neighbourless(T->x_rank_r, "topo_stress_right_H");
 }
 
-void awp_topo_velocity_s16(awp_topo_t* T, int fmt, const float* scale) {
+void awp_topo_velocity_s16(topo_t* T, int fmt, const float* scale) {
   WaveScales W;
   if (!awp_scales_from_host("awp_topo_velocity_s16", fmt, scale, &W)) return;
   if (fmt == AWP_FMT_BF16)
@@ -1155,13 +1171,13 @@ void awp_topo_velocity_s16(awp_topo_t* T, int fmt, const float* scale) {
     topo_velocity_launch<StF16>(T, W, "awp_topo_velocity_s16");
 }
 
-void awp_topo_stress_s16(awp_topo_t* T, int fmt, const float* scale) {
+void awp_topo_stress_s16(topo_t* T, int fmt, const float* scale, float* rates) {
   WaveScales W;
   if (!awp_scales_from_host("awp_topo_stress_s16", fmt, scale, &W)) return;
   if (fmt == AWP_FMT_BF16)
-    topo_stress_launch<StBF16>(T, W, "awp_topo_stress_s16");
+    topo_stress_launch<StBF16>(T, W, "awp_topo_stress_s16", rates);
   else
-    topo_stress_launch<StF16>(T, W, "awp_topo_stress_s16");
+    topo_stress_launch<StF16>(T, W, "awp_topo_stress_s16", rates);
 }
 
 void awp_topo_geom_custom(float* f, int nx, int ny, int nz, float h, hipStream_t St) {
@@ -1176,7 +1192,7 @@ void awp_topo_geom_custom(float* f, int nx, int ny, int nz, float h, hipStream_t
   awp_check_launch("awp_topo_geom_custom");
 }
 
-void awp_topo_metrics_f(const awp_f_grid_t* mf, int nx, int ny, float h, hipStream_t St) {
+void awp_topo_metrics_f(const f_grid_t* mf, int nx, int ny, float h, hipStream_t St) {
   if (!mf || nx < 1 || ny < 1 || !(h > 0.0f)) {
     awp_set_error(AWP_ERR_ARG, "awp_topo_metrics_f: bad arguments");
     return;
@@ -1210,7 +1226,7 @@ void awp_topo_geom_no_stretching(float* g, int nz, float h, hipStream_t St) {
   awp_check_launch("awp_topo_geom_no_stretching");
 }
 
-void awp_topo_metrics_g(const awp_g_grid_t* mg, int nz, float h, hipStream_t St) {
+void awp_topo_metrics_g(const g_grid_t* mg, int nz, float h, hipStream_t St) {
   if (!mg || !mg->d_g || !mg->d_g_c || !mg->d_g3 || !mg->d_g3_c || nz < 18 || !(h > 0.0f)) {
     awp_set_error(AWP_ERR_ARG, "awp_topo_metrics_g: bad arguments (nz >= 18, four arrays)");
     return;

@@ -10,9 +10,10 @@ Lagrange coefficients of every query point on the host and uploads them;
 awp_amd.grid; receiver outputs and source inputs are 1-D device tensors
 indexed by the query's `indices` entry.
 
-A grid is described as the reference's grid3_t is read by grid_index
-(include/grid/grid_3d.h:113-115): point (i, j, k) of the coordinate arrays is
-field element off_z + k + (off_y + j)*line + (off_x + i)*slice.
+A grid is the reference's grid3_t (awp_amd.refstructs.Grid3), read as
+grid_index does (include/grid/grid_3d.h:113-115): point (i, j, k) of the
+coordinate arrays is field element
+offset1.z + k + (offset1.y + j)*line + (offset1.x + i)*slice.
 """
 from __future__ import annotations
 
@@ -21,8 +22,8 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import AwpGrid3
 from .grid import ALIGN, NGSL, Dims
+from .refstructs import CuInterp, Grid3, boundary_size, grid3, numel
 
 # include/awp/error.h:6-35
 ERRORS = {
@@ -42,31 +43,48 @@ _FP = C.POINTER(C.c_float)
 _IP = C.POINTER(C.c_int)
 
 
-class AwpInterp(C.Structure):
-    _fields_ = [("num_basis", C.c_int), ("num_query", C.c_int),
-                ("size_l", C.c_size_t), ("size_i", C.c_size_t),
-                ("lx", _FP), ("ly", _FP), ("lz", _FP),
-                ("ix", _IP), ("iy", _IP), ("iz", _IP), ("ridx", _IP),
-                ("d_lx", C.c_void_p), ("d_ly", C.c_void_p), ("d_lz", C.c_void_p),
-                ("d_ix", C.c_void_p), ("d_iy", C.c_void_p), ("d_iz", C.c_void_p),
-                ("d_ridx", C.c_void_p),
-                ("grid", AwpGrid3),
-                ("stream", C.c_void_p),
-                ("num_cells", C.c_int),
-                ("d_cell_pos", C.c_void_p), ("d_cell_ptr", C.c_void_p),
-                ("d_cell_term", C.c_void_p),
-                ("ix_max", C.c_int), ("iy_max", C.c_int), ("iz_max", C.c_int)]
+class GridDesc:
+    """A grid3_t (`c`) with the quantities the Lagrange path derives from it:
+    size_x/y/z, the coordinate-array lengths (grid_boundary_size), off_x/y/z
+    (offset1), line, slice, and numel, the values in one field (mem.x mem.y
+    mem.z)."""
+
+    def __init__(self, c: Grid3):
+        self.c = c
+        self.size_x, self.size_y, self.size_z = boundary_size(c)
+        self.off_x, self.off_y, self.off_z = c.offset1.x, c.offset1.y, c.offset1.z
+        self.line, self.slice = c.line, c.slice
+        self.numel = numel(c)
+
+    @classmethod
+    def plain(cls, off_x: int, off_y: int, off_z: int, line: int, slice_: int, size_x: int,
+              size_y: int, size_z: int, numel_: int) -> "GridDesc":
+        """A cell-centred grid (no point dropped) of the given coordinate sizes on
+        a field of at least numel_ values with strides line and slice."""
+        mem = (-(-int(numel_) // int(slice_)), int(slice_) // int(line), int(line))
+        return cls(grid3((size_x, size_y, size_z), (1, 1, 1), (off_x, off_y, off_z), mem, line,
+                         slice_))
 
 
-def metric_grid(dims: Dims, size_z: int | None = None) -> AwpGrid3:
-    """The reference's metric grid (grid_init_metric_grid, src/grid/grid_3d.c:
-    padding ngsl in x and y) on the padded field layout: coordinate index 0
-    in x and y is field index 2 (ngsl cells outside the interior), index 0 in
-    z is field level `align`.  size_z defaults to nzt (the grid's boundary
-    size drops the top point of a node grid with a closed top)."""
-    return AwpGrid3(2, 2, ALIGN, dims.yline, dims.slice,
-                    dims.nxt + 2 * NGSL, dims.nyt + 2 * NGSL,
-                    dims.nzt if size_z is None else size_z, dims.size)
+def metric_grid(dims: Dims, size_z: int | None = None, h: float = 0.0) -> GridDesc:
+    """The reference's metric grid of one rank (grid_init_metric_grid,
+    src/grid/grid_3d.c:26-102: padding ngsl in x and y) on the padded field
+    layout: coordinate index 0 in x and y is field index 2 (ngsl cells
+    outside the interior), index 0 in z is field level `align`.  A node grid
+    (shift 0) with an open top keeps nzt points in z (size_z = nzt, the
+    default); with a closed top (size_z = nzt - 1) grid_boundary_size drops
+    the top point."""
+    nz = dims.nzt
+    if size_z is None or size_z == nz:
+        b2 = (0, 0, 0)
+    elif size_z == nz - 1:
+        b2 = (0, 0, 1)
+    else:
+        raise ValueError(f"size_z must be nzt ({nz}) or nzt - 1")
+    mem = (dims.nxt + 4 + 2 * NGSL, dims.nyt + 4 + 2 * NGSL, nz + 2 * ALIGN)
+    return GridDesc(grid3((dims.nxt + 2 * NGSL, dims.nyt + 2 * NGSL, nz), (0, 0, 0),
+                          (2, 2, ALIGN), mem, mem[2], mem[1] * mem[2], boundary2=b2,
+                          gridspacing=h))
 
 
 def grid_coordinates(size: int, h: float, shift: int = 0, padding: int = 0) -> np.ndarray:
@@ -84,7 +102,7 @@ def _f32(a) -> np.ndarray:
 class Interpolation:
     """One cu_interp_t: num_query points, Lagrange degree `degree`."""
 
-    def __init__(self, x, y, z, grid: AwpGrid3, qx, qy, qz, indices=None, degree: int = 3):
+    def __init__(self, x, y, z, grid: GridDesc, qx, qy, qz, indices=None, degree: int = 3):
         lib = _lib.load()
         self._x, self._y, self._z = _f32(x), _f32(y), _f32(z)
         if (len(self._x), len(self._y), len(self._z)) != (grid.size_x, grid.size_y, grid.size_z):
@@ -101,10 +119,10 @@ class Interpolation:
             raise ValueError("indices must be non-negative")
         self._slots = int(idx.max()) + 1 if nq else 0
         self.grid = grid
-        self.I = AwpInterp()
+        self.I = CuInterp()
         self._live = False
         fp = lambda a: a.ctypes.data_as(_FP)  # noqa: E731
-        err = lib.cuinterp_init(C.byref(self.I), fp(self._x), fp(self._y), fp(self._z), grid,
+        err = lib.cuinterp_init(C.byref(self.I), fp(self._x), fp(self._y), fp(self._z), grid.c,
                                 fp(qx), fp(qy), fp(qz), idx.ctypes.data_as(_IP), nq, int(degree))
         self._live = self.I.lx is not None and bool(self.I.lx)
         if err != 0:
@@ -127,11 +145,16 @@ class Interpolation:
 
     @property
     def num_cells(self) -> int:
-        return self.I.num_cells
+        """Grid points of the source map."""
+        return int(_lib.load().awp_interp_num_cells(C.byref(self.I)))
 
     def _stream(self) -> None:
         import torch
-        self.I.stream = torch.cuda.current_stream().cuda_stream
+        st = torch.cuda.current_stream().cuda_stream
+        if st != getattr(self, "_st", None):
+            if _lib.load().awp_interp_set_stream(C.byref(self.I), st) != 0:
+                raise RuntimeError("awp_interp_set_stream: no source map (cuinterp_htod)")
+            self._st = st
 
     def _check_field(self, t, slots) -> None:
         if t.numel() < self.grid.numel:
@@ -195,7 +218,7 @@ class Interpolation:
             pass
 
 
-def lagrange_h(x, y, z, grid: AwpGrid3, qx, qy, qz, degree: int):
+def lagrange_h(x, y, z, grid: GridDesc, qx, qy, qz, degree: int):
     """The library's host coefficient computation alone (cuinterp_lagrange_h),
     no device memory: returns (err, lx, ly, lz, ix, iy, iz)."""
     lib = _lib.load()
@@ -203,10 +226,11 @@ def lagrange_h(x, y, z, grid: AwpGrid3, qx, qy, qz, degree: int):
     nq, nb = len(qx), degree + 1
     lx, ly, lz = (np.zeros(nq * nb, np.float32) for _ in range(3))
     ix, iy, iz = (np.zeros(nq, np.int32) for _ in range(3))
-    H = AwpInterp()
+    H = CuInterp()
     H.num_basis, H.num_query = nb, nq
     H.lx, H.ly, H.lz = (a.ctypes.data_as(_FP) for a in (lx, ly, lz))
     H.ix, H.iy, H.iz = (a.ctypes.data_as(_IP) for a in (ix, iy, iz))
     fp = lambda a: a.ctypes.data_as(_FP)  # noqa: E731
-    err = lib.cuinterp_lagrange_h(C.byref(H), fp(x), fp(y), fp(z), grid, fp(qx), fp(qy), fp(qz))
+    err = lib.cuinterp_lagrange_h(C.byref(H), fp(x), fp(y), fp(z), grid.c, fp(qx), fp(qy),
+                                  fp(qz))
     return (err, lx.reshape(nq, nb), ly.reshape(nq, nb), lz.reshape(nq, nb), ix, iy, iz)

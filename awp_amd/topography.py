@@ -1,6 +1,6 @@
 """Curvilinear topography of grid 0: the free-surface elevation, the
 reference's binary topography file, and the device state (metric arrays and
-the awp_topo_t the launchers read).
+the topo_t the launchers read).
 
 Geometry (reference src/topography/geometry/geometry.c:188-210,
 metrics/metrics.c): the grid follows z = g(r) f(x, y) with g(r) = r h on the
@@ -18,8 +18,9 @@ from pathlib import Path
 import numpy as np
 
 from .grid import ALIGN, NGSL
+from .refstructs import FGrid, GGrid, Topo
 
-# Metric array names in the order of awp_f_grid_t / awp_g_grid_t.
+# Metric array names (f_grid_t / g_grid_t members without their d_ prefix).
 F_NAMES = ("f", "f_1", "f_2", "f_c", "f1_1", "f1_2", "f1_c", "f2_1", "f2_2", "f2_c")
 G_NAMES = ("g", "g3", "g3_c", "g_c")
 
@@ -115,27 +116,29 @@ class Topography:
 
 
 # ---------------------------------------------------------------------------
-# awp_topo_t (include/awp_hip.h)
-class AwpFGrid(C.Structure):
-    _fields_ = [("d_" + n, C.c_void_p) for n in F_NAMES]
+def f_grid(arrays: dict, nx: int, ny: int, h: float) -> FGrid:
+    """An f_grid_t over device arrays {name: tensor}, with its sizes as
+    metrics_init_f sets them (src/topography/metrics/metrics.c:8-33)."""
+    fg = FGrid()
+    rows, cols = f_shape(nx, ny)
+    fg.size[:] = [nx, ny, 1]
+    fg.mem[:] = [rows, cols, 1]
+    fg.offset[:] = [NGSL + 2, NGSL + ALIGN + 2, 0]
+    fg.hi = float(np.float32(1.0 / h))
+    fg.line, fg.slice = 1, cols
+    for n in F_NAMES:
+        setattr(fg, "d_" + n, arrays[n].data_ptr())
+    return fg
 
 
-class AwpGGrid(C.Structure):
-    _fields_ = [("d_" + n, C.c_void_p) for n in G_NAMES]
-
-
-class AwpTopo(C.Structure):
-    _fields_ = ([("use", C.c_int), ("x_rank_l", C.c_int), ("x_rank_r", C.c_int),
-                 ("y_rank_b", C.c_int), ("y_rank_f", C.c_int), ("nx", C.c_int),
-                 ("ny", C.c_int), ("nz", C.c_int), ("dth", C.c_float),
-                 ("timestep", C.c_float), ("gridspacing", C.c_float)]
-                + [(n, C.c_void_p) for n in (
-                    "rho", "lami", "mui", "xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3",
-                    "r4", "r5", "r6", "qpi", "qsi", "ww", "wwo", "vx1", "vx2", "coeff", "u1",
-                    "v1", "w1", "dcrjx", "dcrjy", "dcrjz")]
-                + [("metrics_f", AwpFGrid), ("metrics_g", AwpGGrid), ("stream_i", C.c_void_p),
-                   ("c_DT", C.c_float), ("c_dth", C.c_float), ("c_dh1", C.c_float),
-                   ("constants_set", C.c_int), ("rates", C.c_void_p)])
+def g_grid(arrays: dict, nz: int, h: float) -> GGrid:
+    """A g_grid_t over device arrays {name: tensor} (metrics.c:204-221)."""
+    gg = GGrid()
+    gg.size, gg.mem, gg.line, gg.offset = nz, nz + 2 * ALIGN, 1, ALIGN
+    gg.hi = float(np.float32(1.0 / h))
+    for n in G_NAMES:
+        setattr(gg, "d_" + n, arrays[n].data_ptr())
+    return gg
 
 
 # topo_t member <- (level dict, key)
@@ -149,7 +152,9 @@ _FIELDS = ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4", "r5", "r6
 
 class TopoDevice:
     """Metric arrays in HBM, built by the library's setup kernels, and the
-    awp_topo_t of one grid-0 level (level.f / level.m tensors)."""
+    topo_t of one grid-0 level (level.f / level.m tensors).  two_pass: the
+    stress update runs in two passes over nine strain-rate arrays
+    (awp_topo_stress_interior_rates_H), else in one (topo_stress_interior_H)."""
 
     def __init__(self, lib, level, topo: Topography, device, stream: int,
                  two_pass: bool = True):
@@ -165,30 +170,33 @@ class TopoDevice:
         self.mf = {n: torch.zeros(f_shape(d.nxt, d.nyt), **f32) for n in F_NAMES}
         self.mg = {n: torch.zeros(g_size(d.nzt), **f32) for n in G_NAMES}
         self.mf["f"].copy_(torch.from_numpy(topo.f_input(np.float32)))
-        self.fg = AwpFGrid(*[self.mf[n].data_ptr() for n in F_NAMES])
-        self.gg = AwpGGrid(*[self.mg[n].data_ptr() for n in G_NAMES])
         DH, DT = float(level.medium.DH), float(level.medium.DT)
+        self.fg = f_grid(self.mf, d.nxt, d.nyt, DH)
+        self.gg = g_grid(self.mg, d.nzt, DH)
         lib.awp_topo_geom_custom(self.mf["f"].data_ptr(), d.nxt, d.nyt, d.nzt, DH, stream)
         lib.awp_topo_metrics_f(C.byref(self.fg), d.nxt, d.nyt, DH, stream)
         lib.awp_topo_geom_no_stretching(self.mg["g"].data_ptr(), d.nzt, DH, stream)
         lib.awp_topo_metrics_g(C.byref(self.gg), d.nzt, DH, stream)
-        self.T = AwpTopo()
+        self.T = Topo()
         T = self.T
         T.use = 1
         T.x_rank_l = T.x_rank_r = T.y_rank_b = T.y_rank_f = -1
         T.nx, T.ny, T.nz = d.nxt, d.nyt, d.nzt
+        T.px = T.py = 1
+        T.mx, T.my, T.mz = d.nxt + 4 + 2 * NGSL, d.nyt + 4 + 2 * NGSL, d.nzt + 2 * ALIGN
+        T.line, T.slice = d.yline, d.slice
+        T.gridsize = d.size
         # topo_t.dth = dt/h in _prec, timestep = 1 (topography.c:71-72)
         T.dth = float(np.float32(DT) / np.float32(DH))
         T.timestep = 1.0
         T.gridspacing = DH
         T.metrics_f = self.fg
         T.metrics_g = self.gg
-        T.rates = self.rates.data_ptr() if self.rates is not None else None
         self.bind()
         lib.topo_set_constants(C.byref(T))
 
     def bind(self) -> None:
-        """(Re)point the awp_topo_t at the level's current tensors."""
+        """(Re)point the topo_t at the level's current tensors."""
         L, T = self.level, self.T
         for member, where, key in _BIND:
             setattr(T, member, getattr(L, where)[key].data_ptr())
@@ -201,7 +209,10 @@ class TopoDevice:
 
     def stress(self, stream: int) -> None:
         self.T.stream_i = stream
-        self.lib.topo_stress_interior_H(C.byref(self.T))
+        if self.rates is not None:
+            self.lib.awp_topo_stress_interior_rates_H(C.byref(self.T), self.rates.data_ptr())
+        else:
+            self.lib.topo_stress_interior_H(C.byref(self.T))
 
     def velocity_s16(self, stream: int, fmt: int, scales) -> None:
         self.T.stream_i = stream
@@ -209,7 +220,8 @@ class TopoDevice:
 
     def stress_s16(self, stream: int, fmt: int, scales) -> None:
         self.T.stream_i = stream
-        self.lib.awp_topo_stress_s16(C.byref(self.T), fmt, scales)
+        self.lib.awp_topo_stress_s16(C.byref(self.T), fmt, scales,
+                                     self.rates.data_ptr() if self.rates is not None else None)
 
     def metrics(self) -> dict:
         """Host copies of the metric arrays."""

@@ -271,7 +271,8 @@ void awp_adjoint_inject(float* ub, float* vb, float* wb, const int* idx, int npt
 /* g64 += g32; g32 = 0 over n values */
 void awp_adjoint_flush(float* g32, double* g64, long long n, hipStream_t St);
 /* res = d - obs (binary64) and *J = 1/2 sum res^2 over n samples; work holds
- * 256 doubles */
+ * AWP_MISFIT_PARTS doubles (one partial sum per block) */
+#define AWP_MISFIT_PARTS 256
 void awp_misfit(const float* traces, const float* obs, long long n, double* res, double* J,
                 double* work, hipStream_t St);
 /* maps the binary64 coefficient gradients gcoef[8] and grad_lam_mu to the
@@ -302,157 +303,270 @@ void awp_dstrqc_oop(float* xx, float* yy, float* zz, float* xy, float* xz, float
                     float* vx2, int* ww, float* wwo, int NX, int NPC, int rankx, int ranky,
                     int s_i, int e_i, int s_j, int e_j, int d_i);
 
-/* ---- Curvilinear topography (grid 0) -------------------------------------
- * The reference passes a topo_t (include/topography/topography.h:136-273) to
- * its launchers; awp_topo_t keeps the members those launchers read, under
- * the same names, with device pointers only.
- *
- * Metric arrays (device): the ten f* arrays are (nx+20) x (ny+84) floats,
- * element (i, j) of topographic coordinates i in [-10, nx+10),
- * j in [-42, ny+42) at (j+42) + (i+10)*(ny+84); f holds the geometry on
- * [-8, nx+8) x [-8, ny+8) and the builders fill the others on
- * [-6, n+6)^2.  The four g* arrays hold nz+64 floats, level k at k+32.
- * Wavefield and medium arrays are the 3-D fields of the Cartesian kernels
- * (topographic (i, j, k) is array element (i+10, j+10, k+32)). */
+/* ---- The reference's structs at the topography and Lagrange boundary -----
+ * The topography and Lagrange launchers take the reference's own structs by
+ * pointer.  They are restated here member for member, in the reference's
+ * order and with its types: _prec and prec are float, CUDA's dim3 is three
+ * 32-bit unsigned integers (awp_dim3_t), cudaStream_t is hipStream_t (both
+ * are opaque pointers).  A caller built against the reference's headers
+ * therefore hands these launchers the layout they read; the CPU tests check
+ * every member offset against the member lists of the reference's headers.
+ * The launchers read only the members named in their comments; the others
+ * are carried for layout. */
 typedef struct {
+  int x, y, z;
+} int3_t; /* include/awp/definitions.h:55-58 */
+
+typedef struct {
+  uint32_t x, y, z;
+} awp_dim3_t; /* CUDA's dim3 */
+
+/* fcn_grid_t = grid3_t, include/grid/grid_3d.h:90-111.  The Lagrange path
+ * reads offset1, line and slice (grid_index: offset1.z + k + (offset1.y + j)
+ * * line + (offset1.x + i) * slice), size, shift and boundary2 (the
+ * coordinate array lengths of grid_boundary_size, src/grid/grid_3d.c:143-160)
+ * and mem (a field holds mem.x * mem.y * mem.z values). */
+typedef struct {
+  int3_t size;
+  int3_t inner_size;
+  int3_t outer_size;
+  int3_t mem;
+  int3_t coordinate;
+  int3_t shift;
+  int3_t offset1;
+  int3_t offset2;
+  int3_t boundary1;
+  int3_t boundary2;
+  int3_t alignment;
+  float gridspacing;
+  int3_t padding;
+  int num_bytes;
+  int line;
+  int slice;
+  int exclude_top_row;
+} fcn_grid_t;
+typedef fcn_grid_t grid3_t;
+
+/* f_grid_t, include/topography/metrics/metrics.h:58-97.  The launchers read
+ * the device arrays d_f .. d_f2_c: (nx+20) x (ny+84) floats, element (i, j)
+ * of topographic coordinates i in [-10, nx+10), j in [-42, ny+42) at
+ * (j+42) + (i+10)*(ny+84); d_f holds the geometry on [-8, nx+8) x [-8, ny+8)
+ * and awp_topo_metrics_f fills the others on [-6, n+6)^2. */
+typedef struct {
+  int size[3];
+  int mem[3];
+  int offset[3];
+  int bounds_x[2];
+  int bounds_y[2];
+  int bounds_stress_x[2];
+  int bounds_stress_y[2];
+  float hi;
+  int line;
+  int slice;
+  float *f, *f_1, *f_2, *f_c, *f1_1, *f1_2, *f1_c, *f2_1, *f2_2, *f2_c; /* host */
   float *d_f, *d_f_1, *d_f_2, *d_f_c, *d_f1_1, *d_f1_2, *d_f1_c, *d_f2_1, *d_f2_2, *d_f2_c;
-} awp_f_grid_t; /* f_grid_t, include/topography/metrics/metrics.h */
+} f_grid_t;
 
+/* g_grid_t, include/topography/metrics/metrics.h:106-127.  The launchers
+ * read d_g, d_g_c, d_g3, d_g3_c: nz+64 floats each, level k at k+32. */
 typedef struct {
-  float *d_g, *d_g3, *d_g3_c, *d_g_c;
-} awp_g_grid_t; /* g_grid_t, include/topography/metrics/metrics.h */
+  int size;
+  int mem;
+  int line;
+  int offset;
+  int bounds_z[2];
+  float hi;
+  float *g, *g_c, *g3, *g3_c;         /* host */
+  float *d_g, *d_g_c, *d_g3, *d_g3_c; /* device */
+} g_grid_t;
 
+/* topo_t, include/topography/topography.h:136-273.  The launchers read use,
+ * the neighbour ranks, nx ny nz, dth timestep gridspacing, the medium
+ * (rho lami mui qpi qsi ww wwo vx1 vx2 coeff), the wavefield (xx .. yz,
+ * r1 .. r6, u1 v1 w1), dcrjx dcrjy dcrjz, metrics_f, metrics_g and
+ * stream_i.  Wavefield and medium arrays are the 3-D fields of the
+ * Cartesian kernels: topographic (i, j, k) is array element
+ * (i+10, j+10, k+32). */
 typedef struct {
   int use;
-  /* neighbour ranks, -1 on the domain boundary; on one GPU all four are -1 */
-  int x_rank_l, x_rank_r, y_rank_b, y_rank_f;
-  int nx, ny, nz;
-  float dth, timestep, gridspacing; /* dt/h (single precision), 1, h */
-  float *rho, *lami, *mui;
+  int dbg;
+  int verbose;
+  int rank;
+  const char* topography_file;
+  int x_rank_l; /* -1 on the domain boundary; on one GPU all four are -1 */
+  int x_rank_r;
+  int y_rank_b;
+  int y_rank_f;
+  int coord[2];
+  int px;
+  int py;
+  int nx;
+  int ny;
+  int nz;
+  int mx;
+  int my;
+  int mz;
+  int off_x[4];
+  int off_y[4];
+  int off_z[4];
+  int velocity_bounds_left[2];
+  int velocity_bounds_right[2];
+  int velocity_bounds_front[2];
+  int velocity_bounds_back[2];
+  int velocity_bounds_xdir[2];
+  int velocity_offset_x[6];
+  int velocity_offset_y[6];
+  int boundary_top_size;
+  int boundary_bottom_size;
+  awp_dim3_t velocity_grid_interior;
+  awp_dim3_t velocity_grid_front;
+  awp_dim3_t velocity_grid_back;
+  int stress_bounds_left[2];
+  int stress_bounds_right[2];
+  int stress_bounds_ydir[2];
+  int stress_offset_x[6];
+  int stress_offset_y[4];
+  awp_dim3_t stress_grid_interior;
+  awp_dim3_t stress_grid_left;
+  awp_dim3_t stress_grid_right;
+  int sxx[3];
+  int syy[3];
+  int szz[3];
+  int sxy[3];
+  int sxz[3];
+  int syz[3];
+  int su1[3];
+  int sv1[3];
+  int sw1[3];
+  int gridsize;
+  int slice;
+  int slice_gl;
+  int line;
+  float dth;         /* dt/h */
+  float timestep;    /* 1, or 0 for the spatial part only */
+  float gridspacing; /* h */
+  float *rho, *lami, *mui, *lam_mu;
   float *xx, *yy, *zz, *xy, *xz, *yz;
   float *r1, *r2, *r3, *r4, *r5, *r6;
   float *qpi, *qsi;
   int* ww;
   float *wwo, *vx1, *vx2, *coeff;
   float *u1, *v1, *w1;
+  float *f_u1, *f_v1, *f_w1, *b_u1, *b_v1, *b_w1;
+  f_grid_t metrics_f;
+  g_grid_t metrics_g;
+  grid3_t topography_grid;
+  grid3_t stress_grid;
+  grid3_t velocity_grid;
+  float *x1, *y1, *z1;
   float *dcrjx, *dcrjy, *dcrjz;
-  awp_f_grid_t metrics_f;
-  awp_g_grid_t metrics_g;
+  hipStream_t stream_1;
+  hipStream_t stream_2;
   hipStream_t stream_i;
-  /* written by topo_set_constants (the reference keeps them in __constant__
-   * memory, src/topography/kernels/stress_attenuation.cu:24-72) */
-  float c_DT, c_dth, c_dh1;
-  int constants_set;
-  /* no reference counterpart: optional caller-owned scratch of 9 arrays of
-   * the 3-D field size, back to back.  Non-null, the stress launchers run in
-   * two passes, the strain rates into rates and then the material and memory-
-   * variable update; results are the same bit for bit.  Null: one pass. */
-  float* rates;
-} awp_topo_t;
+} topo_t;
 
-/* topo_set_constants, src/topography/stress_attenuation.cu:19-23 */
-void topo_set_constants(awp_topo_t* T);
+/* ---- Curvilinear topography (grid 0) ------------------------------------- */
+/* topo_set_constants, src/topography/stress_attenuation.cu:19-23: the
+ * stress kernels' DT, dt/h and 1/h from T's dth and gridspacing, kept
+ * process-wide as the reference keeps them in __constant__ memory; the
+ * stress launchers refuse to run before it. */
+void topo_set_constants(topo_t* T);
 /* include/topography/velocity.cuh:16-18: curvilinear velocity on levels
  * 3 <= k < nz (interior stencils below nz-6, one-sided rows above).  front and
  * back compute the halo-send strips of a multi-GPU run: with no neighbour
  * (-1) they return, as the reference's do; a neighbour is an error here. */
-void topo_velocity_interior_H(awp_topo_t* T);
-void topo_velocity_front_H(awp_topo_t* T);
-void topo_velocity_back_H(awp_topo_t* T);
+void topo_velocity_interior_H(topo_t* T);
+void topo_velocity_front_H(topo_t* T);
+void topo_velocity_back_H(topo_t* T);
 /* include/topography/stress.cuh:17-19: curvilinear stress + Q on levels
  * 3 <= k < nz of the stress bounds (topography.c:377-420) */
-void topo_stress_interior_H(awp_topo_t* T);
-void topo_stress_left_H(awp_topo_t* T);
-void topo_stress_right_H(awp_topo_t* T);
+void topo_stress_interior_H(topo_t* T);
+void topo_stress_left_H(topo_t* T);
+void topo_stress_right_H(topo_t* T);
+/* no reference counterpart: topo_stress_interior_H in two passes over
+ * caller-owned scratch `rates` (9 arrays of the 3-D field size, back to
+ * back): the strain rates first, then the streaming material and memory-
+ * variable update; the same results bit for bit, faster. */
+void awp_topo_stress_interior_rates_H(topo_t* T, float* rates);
 /* no reference counterpart: the same updates with 16-bit wavefield storage
  * (the wavefield pointers of T address uint16_t; fmt and scale as for
- * awp_dvelcx_s16) */
-void awp_topo_velocity_s16(awp_topo_t* T, int fmt, const float* scale);
-void awp_topo_stress_s16(awp_topo_t* T, int fmt, const float* scale);
+ * awp_dvelcx_s16; rates NULL for one pass, or the two-pass scratch) */
+void awp_topo_velocity_s16(topo_t* T, int fmt, const float* scale);
+void awp_topo_stress_s16(topo_t* T, int fmt, const float* scale, float* rates);
 /* geom_custom, src/topography/geometry/geometry.c:188-210: f = 1 + elevation
  * / (h (nz-2)) in place on the f grid (f holds the elevation on entry) */
 void awp_topo_geom_custom(float* f, int nx, int ny, int nz, float h, hipStream_t St);
 /* metrics_build_f (src/topography/metrics/metrics.c:35-41, 141-170): the
  * nine interpolants and derivatives of f */
-void awp_topo_metrics_f(const awp_f_grid_t* mf, int nx, int ny, float h, hipStream_t St);
+void awp_topo_metrics_f(const f_grid_t* mf, int nx, int ny, float h, hipStream_t St);
 /* geom_no_grid_stretching, geometry.c:33-42: g(k) = h k below the top ghost */
 void awp_topo_geom_no_stretching(float* g, int nz, float h, hipStream_t St);
 /* metrics_build_g (metrics.c:223-229, 293-318): g_c, g3, g3_c from g */
-void awp_topo_metrics_g(const awp_g_grid_t* mg, int nz, float h, hipStream_t St);
+void awp_topo_metrics_g(const g_grid_t* mg, int nz, float h, hipStream_t St);
 
 /* ---- Lagrange-interpolated receivers and sources ---------------------------
- * The reference keeps one cu_interp_t per grid and receiver/source set
- * (include/interpolation/interpolation.cuh:21-33).  awp_interp_t keeps its
- * members under the same names; its grid3_t becomes awp_grid3_t, the three
- * members of grid3_t that the path reads plus the field size for bounds
- * checks.  Point (i, j, k) of the grid's coordinate arrays is field element
- *   off_z + k + (off_y + j)*line + (off_x + i)*slice
- * (grid_index, include/grid/grid_3d.h:113-115). */
-typedef struct {
-  int off_x, off_y, off_z;    /* grid3_t.offset1 */
-  int line, slice;            /* grid3_t.line, grid3_t.slice */
-  int size_x, size_y, size_z; /* grid_boundary_size(grid): coordinate array lengths */
-  long long numel;            /* floats in one field on this grid */
-} awp_grid3_t;
-
+ * cu_interp_t, include/interpolation/interpolation.cuh:21-32, restated
+ * member for member (prec is float).  cuinterp_htod also builds, library-
+ * side, the map the source launchers gather over: the grid points the
+ * queries touch, each with its (query, basis point) terms in ascending query
+ * order, kept under the struct's d_ix and released by cuinterp_finalize.
+ * The launchers run on the default stream, as the reference's do, unless
+ * awp_interp_set_stream names another for this struct. */
 typedef struct {
   int num_basis; /* degree + 1 */
   int num_query;
   size_t size_l, size_i;
-  float *lx, *ly, *lz; /* host, num_query x num_basis */
+  float *lx, *ly, *lz;      /* host, num_query x num_basis */
   int *ix, *iy, *iz, *ridx; /* host, num_query */
   float *d_lx, *d_ly, *d_lz;
   int *d_ix, *d_iy, *d_iz, *d_ridx;
-  awp_grid3_t grid;
-  /* no reference counterpart.  Stream the launchers use (0 by default). */
-  hipStream_t stream;
-  /* Written by cuinterp_htod: the grid points the queries touch, each with
-   * its (query, basis point) terms in ascending query order.  The source
-   * launchers run one thread per grid point over this map, so points shared
-   * by several queries receive their terms in query order, deterministically
-   * and without atomics. */
-  int num_cells;
-  long long* d_cell_pos;
-  int* d_cell_ptr;  /* num_cells + 1 */
-  int* d_cell_term; /* ((q*nb + i)*nb + j)*nb + k for basis point (i, j, k) */
-  int ix_max, iy_max, iz_max; /* largest index + num_basis - 1 uploaded */
-} awp_interp_t;
+  grid3_t grid;
+} cu_interp_t;
+
+/* no reference counterpart: the stream the launchers of this struct use
+ * (after cuinterp_htod; 0 = the default stream).  Returns 0, or 400 if the
+ * struct has no map yet. */
+int awp_interp_set_stream(const cu_interp_t* interp, hipStream_t stream);
+/* no reference counterpart: the number of grid points in the struct's
+ * source map (-1 before cuinterp_htod) */
+int awp_interp_num_cells(const cu_interp_t* interp);
 
 /* cuinterp_init, src/interpolation/interpolation.cu:13-40: allocate, compute
  * the Lagrange coefficients of num_query query points (qx, qy, qz) on the
- * grid coordinates x, y, z (grid.size_x, ... entries) and upload them.
+ * grid coordinates x, y, z (grid_boundary_size(grid) entries) and upload
+ * them.
  * indices become ridx (receivers: output slot; sources: input column).
  * Returns 0 or the reference's error code (800: num_query <= 0,
  * 900: allocation; 400 here also when an index leaves the field). */
-int cuinterp_init(awp_interp_t* out, const float* x, const float* y, const float* z,
-                  awp_grid3_t grid, const float* qx, const float* qy, const float* qz,
+int cuinterp_init(cu_interp_t* out, const float* x, const float* y, const float* z,
+                  grid3_t grid, const float* qx, const float* qy, const float* qz,
                   const int* indices, int num_query, int degree);
 /* cuinterp_lagrange_h, interpolation.cu:42-70 (host only) */
-int cuinterp_lagrange_h(awp_interp_t* host, const float* x, const float* y,
-                        const float* z, awp_grid3_t grid, const float* qx,
+int cuinterp_lagrange_h(cu_interp_t* host, const float* x, const float* y,
+                        const float* z, grid3_t grid, const float* qx,
                         const float* qy, const float* qz);
 /* cuinterp_htod / cuinterp_dtoh / cuinterp_malloc / cuinterp_finalize,
  * interpolation.cu:72-162.  htod also rebuilds the grid-point map and checks
  * every index against the field (400 if one leaves it). */
-int cuinterp_htod(awp_interp_t* interp);
-int cuinterp_dtoh(awp_interp_t* interp);
-int cuinterp_malloc(awp_interp_t* interp);
-void cuinterp_finalize(awp_interp_t* interp);
+int cuinterp_htod(cu_interp_t* interp);
+int cuinterp_dtoh(cu_interp_t* interp);
+int cuinterp_malloc(cu_interp_t* interp);
+void cuinterp_finalize(cu_interp_t* interp);
 /* cuinterp_interp_H, interpolation.cu:164-175: out[ridx[q]] = sum over the
  * num_basis^3 stencil of lx ly lz in, one thread per query */
-void cuinterp_interp_H(const awp_interp_t* I, float* out, const float* in);
+void cuinterp_interp_H(const cu_interp_t* I, float* out, const float* in);
 /* cusource_add_cartesian_H, src/topography/sources/source.cu:11-22:
  * out += -dt/h^3 lx ly lz in[ridx[q]] */
-void cusource_add_cartesian_H(const awp_interp_t* I, float* out, const float* in,
+void cusource_add_cartesian_H(const cu_interp_t* I, float* out, const float* in,
                               float h, float dt);
 /* cusource_add_curvilinear_H, source.cu:51-63: the same divided by the
  * Jacobian f(i, j) dg(k) of the topography metrics (f: the (nx+20) x (ny+84)
  * metric layout, dg: a g-grid array) */
-void cusource_add_curvilinear_H(const awp_interp_t* I, float* out, const float* in,
+void cusource_add_curvilinear_H(const cu_interp_t* I, float* out, const float* in,
                                 float h, float dt, const float* f, int ny, const float* dg);
 /* cusource_add_force_H, source.cu:97-113: a body force on a velocity field,
  * scaled by -quad_weight / (f dg rho) with rho from d1 (a 3-D field) */
-void cusource_add_force_H(const awp_interp_t* I, float* out, const float* in,
+void cusource_add_force_H(const cu_interp_t* I, float* out, const float* in,
                           const float* d1, float h, float dt, float quad_weight,
                           const float* f, int nx, int ny, int nz, const float* dg);
 

@@ -31,9 +31,9 @@ def test_neighbour_ranks_are_rejected():
     refuse a neighbour (no GPU call is made)."""
     import ctypes as C
     from awp_amd import _lib
-    from awp_amd.topography import AwpTopo
+    from awp_amd.refstructs import Topo
     lib = _lib.load()
-    T = AwpTopo()
+    T = Topo()
     T.use = 1
     T.x_rank_l = T.x_rank_r = T.y_rank_f = T.y_rank_b = -1
     lib.awp_clear_error()
@@ -114,7 +114,7 @@ def test_gpu_topography_run_bit_exact(gpu, oracle_lib, two_pass):
     topo = w.params["topography"]
     g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
     if not two_pass:
-        g.topo.T.rates = None
+        g.topo.rates = None
     g.run(w.nt, check_every=10)
     o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
     o.run(w.nt)

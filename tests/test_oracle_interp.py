@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 from awp_amd.grid import ALIGN, NGSL, Dims
-from awp_amd.interpolation import AwpGrid3, grid_coordinates, metric_grid
+from awp_amd.interpolation import GridDesc, grid_coordinates, metric_grid
 
 FLTOL = 1e-5  # include/awp/definitions.h:43
 
@@ -261,7 +261,7 @@ def test_coincident_sources_add_in_query_order(oi):
 def test_library_coefficients_match_oracle(oi, deg):
     from awp_amd.interpolation import lagrange_h
     rng = np.random.default_rng(30 + deg)
-    grid = AwpGrid3(2, 2, ALIGN, 104, 104 * 60, 37, 29, 41, 10 ** 7)
+    grid = GridDesc.plain(2, 2, ALIGN, 104, 104 * 60, 37, 29, 41, 10 ** 7)
     x = _nonuniform(rng, grid.size_x, 0.1) - 1.0
     y = grid_coordinates(grid.size_y, 0.1, shift=1, padding=NGSL)
     z = _nonuniform(rng, grid.size_z, 0.05)
