@@ -120,7 +120,7 @@ class Solver(PmclStepOrder):
                  receivers: Receivers | None = None, nt_record: int = 0,
                  device: str = "cuda", variant: str = "dstrqc", NPC: int = 0,
                  plastic: PlasticMedium | Sequence | None = None, topography=None,
-                 kinematic=None):
+                 kinematic=None, pointio: dict | None = None):
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a GPU: the HIP path has no CPU fallback")
         self.lib = _lib.load()
@@ -187,6 +187,18 @@ class Solver(PmclStepOrder):
                                  **dev),
                 b=None if kinematic.b is None else np.ascontiguousarray(kinematic.b, np.float64),
                 a=None if kinematic.a is None else np.ascontiguousarray(kinematic.a, np.float64))
+        # input-file point sets: pointio = {"nt": run length, "directory": ...,
+        # "sources"/"receivers"/"forces"/"sgt": input-file names}
+        self.pointio = None
+        if pointio is not None:
+            from .pointsets import PointIO, grids_init
+            opts = dict(pointio)
+            self.pointio_steps = int(opts.pop("nt"))
+            stack = [grids_init(L.dims.nxt, L.dims.nyt, L.dims.nzt,
+                                int(p == 0 and self.topo is not None), L.medium.DH)
+                     for p, L in enumerate(self.levels)]
+            f_host = self.topo.metrics()["f"] if self.topo is not None else None
+            self.pointio = PointIO(stack, f_host=f_host, **opts)
         self.cur_step = 0
 
     # grid-0 views kept for single-grid callers

@@ -15,11 +15,12 @@ free_surface and record on grid 0.  This mixin sequences them as pmcl3d does:
                  NVE = 3: the inner x strip outside [yls2, yre2]      :1858-1877
                  then the left/right strips of every grid              :1895-1920
   dstrqc2_H      every grid above the lowest, [xss1, xse3] x [yls, yre] :1936-1941
+  input-file sources (sources_add_*), forces on a topography grid 0     :1987-2006
   addsrc_H       every grid with sources, while cur_step < NST         :2007-2015
   intp3d_H, swap_H again                                               :2045-2092
   NVE = 3: drprecpc_calc front/back; dstrqc_H on the inner box;
            drprecpc_calc left/right, inside; drprecpc_app              :2109-2209
-  fstr_H on grid 0, receivers                                          :2243-2246
+  fstr_H on grid 0, receivers; input-file receivers and SGT output      :2243-2249
 On one rank the NVE = 1 strips cover the stress region exactly once, so a
 single dstrqc_H over all of it is the same update.  In the NVE = 3 order the
 inner box's stress update runs after the source injection and after the
@@ -38,16 +39,26 @@ class PmclStepOrder:
     plastic = None  # None, or one plastic medium per grid
 
     kinematic = None  # a KinematicFault on grid 0 (IFAULT = 5), or None
+    # input-file sources, forces, receivers and SGT output
+    # (awp_amd.pointsets.PointIO), or None
+    pointio = None
+    pointio_steps = 0   # the run length the point-set file names are sized for
 
     def start(self) -> None:
         """Initial source sample before the loop (pmcl3d.c:1175-1186; only
-        for point sources, IFAULT < 4)."""
+        for point sources, IFAULT < 4); the input-file point sets sample
+        step 0 and read their first source data (pmcl3d.c:1560-1608)."""
+        if self.pointio is not None:
+            self.pointio.start(self, self.pointio_steps)
         if self.kinematic is not None:
             return
         for p in range(self.ngrids):
             self.add_source(p, 1)
 
     def _sources(self) -> None:
+        if self.pointio is not None:
+            # sources_read / forces_add / sources_add_* (pmcl3d.c:1987-2006)
+            self.pointio.inject(self)
         if self.kinematic is not None:
             # IFAULT = 5: the subfaults of grid 0 every step (pmcl3d.c:2025-2034)
             self.add_kinematic(0)
@@ -108,3 +119,6 @@ class PmclStepOrder:
                 self.plastic_app(p, R["xls"], R["xre"], R["yls"], R["yre"])
         self.free_surface()
         self.record(self.cur_step - 1)
+        if self.pointio is not None:
+            # receivers_write, sgt_write (pmcl3d.c:2245-2249)
+            self.pointio.output(self)

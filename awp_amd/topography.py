@@ -117,12 +117,17 @@ class Topography:
 
 # ---------------------------------------------------------------------------
 def f_grid(arrays: dict, nx: int, ny: int, h: float) -> FGrid:
-    """An f_grid_t over device arrays {name: tensor}, with its sizes as
-    metrics_init_f sets them (src/topography/metrics/metrics.c:8-33)."""
+    """An f_grid_t over device arrays {name: tensor}, with its sizes, bounds
+    and offsets as metrics_init_f sets them (src/topography/metrics/
+    metrics.c:14-33)."""
     fg = FGrid()
     rows, cols = f_shape(nx, ny)
     fg.size[:] = [nx, ny, 1]
     fg.mem[:] = [rows, cols, 1]
+    fg.bounds_x[:] = [-NGSL, nx + NGSL]
+    fg.bounds_y[:] = [-NGSL, ny + NGSL]
+    fg.bounds_stress_x[:] = [-NGSL // 2, nx + NGSL // 2]
+    fg.bounds_stress_y[:] = [-NGSL // 2, ny + NGSL // 2]
     fg.offset[:] = [NGSL + 2, NGSL + ALIGN + 2, 0]
     fg.hi = float(np.float32(1.0 / h))
     fg.line, fg.slice = 1, cols
@@ -222,6 +227,10 @@ class TopoDevice:
         self.T.stream_i = stream
         self.lib.awp_topo_stress_s16(C.byref(self.T), fmt, scales,
                                      self.rates.data_ptr() if self.rates is not None else None)
+
+    def metric_tensors(self) -> dict:
+        """The metric arrays on the device, by name (f .. f2_c, g .. g_c)."""
+        return {**self.mf, **self.mg}
 
     def metrics(self) -> dict:
         """Host copies of the metric arrays."""
