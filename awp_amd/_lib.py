@@ -57,6 +57,7 @@ SIGNATURES: dict[str, list] = {
     "awp_adjoint_inject": [_fp] * 4 + [_i, _fp, _s],
     "awp_adjoint_flush": [_fp, _fp, C.c_longlong, _s],
     "awp_misfit": [_fp, _fp, C.c_longlong, _fp, _fp, _fp, _s],
+    "awp_misfit_noalloc": [_fp, _fp, C.c_longlong, _fp, _fp, _s],
     "awp_adjoint_map": [_fp] * 11 + [_s],
     "awp_dvelcx_oop": [_fp] * 16 + [_i, _i, _s, _i, _i, _i, _i],
     "awp_dstrqc_oop": [_fp] * 24 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,

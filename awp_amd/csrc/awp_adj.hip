@@ -15,6 +15,8 @@
 // over the steps (binary32 inside a checkpoint segment, added into binary64
 // at its end) and mapped to the arrays lam (1/lambda), mu (1/mu), d1 (rho)
 // once, in binary64.
+#include <mutex>
+
 #include "awp_common.h"
 #include "../../include/awp_hip.h"
 
@@ -637,7 +639,70 @@ void awp_misfit(const float* traces, const float* obs, long long n, double* res,
                 double* work, hipStream_t St) {
   const char* W = "awp_misfit";
   if (n <= 0) return;
-  const int nb = 256;
+  const int nb = AWP_MISFIT_PARTS;
+  hipLaunchKernelGGL(misfit_kernel, dim3(nb), dim3(256), 0, St, traces, obs, n, res, work);
+  hipLaunchKernelGGL(misfit_final_kernel, dim3(1), dim3(64), 0, St, work, nb, J);
+  awp_check_launch(W);
+}
+
+}  // extern "C"
+
+// Scratch of awp_misfit_noalloc: one partial sum per block of misfit_kernel.
+// Every GPU the library runs on holds its own copy until the process ends.
+// Its size does not depend on n (the blocks stride over the samples), and
+// every call overwrites all AWP_MISFIT_PARTS entries before the final sum
+// reads them, so nothing needs clearing between calls.
+static __device__ __align__(256) double scratch[AWP_MISFIT_PARTS];
+
+namespace {
+
+// Device address of `scratch`, looked up once per device with that device
+// current.  Host code uses only these addresses: the array's own name is a
+// host-side stand-in, not a device address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+double* misfit_scratch(const char* W) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    awp_set_error(AWP_ERR_HIP, "%s: hipGetDevice: %s", W, hipGetErrorString(e));
+    return nullptr;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    awp_set_error(AWP_ERR_ARG, "%s: device %d outside [0,%d)", W, dev, kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      awp_set_error(AWP_ERR_HIP, "%s: hipGetSymbolAddress(scratch) on device %d: %s", W, dev,
+                    hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+void awp_misfit_noalloc(const float* traces, const float* obs, long long n, double* res,
+                        double* J, hipStream_t St) {
+  const char* W = "awp_misfit_noalloc";
+  if (n < 0) {
+    awp_set_error(AWP_ERR_ARG, "%s: n=%lld is negative", W, n);
+    return;
+  }
+  if (n == 0) return;
+  double* work = misfit_scratch(W);
+  if (!work) return;
+  const int nb = AWP_MISFIT_PARTS;
   hipLaunchKernelGGL(misfit_kernel, dim3(nb), dim3(256), 0, St, traces, obs, n, res, work);
   hipLaunchKernelGGL(misfit_final_kernel, dim3(1), dim3(64), 0, St, work, nb, J);
   awp_check_launch(W);

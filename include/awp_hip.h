@@ -275,6 +275,17 @@ void awp_adjoint_flush(float* g32, double* g64, long long n, hipStream_t St);
 #define AWP_MISFIT_PARTS 256
 void awp_misfit(const float* traces, const float* obs, long long n, double* res, double* J,
                 double* work, hipStream_t St);
+/* awp_misfit without the work argument and without any allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partial sums go to a library-owned device array of
+ * AWP_MISFIT_PARTS doubles (2 KiB per GPU, held until the process ends),
+ * looked up on the first call on each device.  Any n >= 0 fits (the scratch
+ * size does not grow with n); n < 0 is rejected with AWP_ERR_ARG.  Results are
+ * those of awp_misfit bit for bit.  All calls on one GPU share that array:
+ * calls that overlap on one GPU (different streams, or threads) race. */
+void awp_misfit_noalloc(const float* traces, const float* obs, long long n, double* res,
+                        double* J, hipStream_t St);
 /* maps the binary64 coefficient gradients gcoef[8] and grad_lam_mu to the
  * gradients in lam (1/lambda), mu (1/mu) and d1 (rho) of every padded cell */
 void awp_adjoint_map(const double* const* gcoef, const double* grad_lam_mu,
