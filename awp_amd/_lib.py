@@ -61,6 +61,7 @@ SIGNATURES: dict[str, list] = {
     "awp_adjoint_map": [_fp] * 11 + [_s],
     "awp_dvelcx_oop": [_fp] * 16 + [_i, _i, _s, _i, _i, _i, _i],
     "awp_dstrqc_oop": [_fp] * 24 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,
+    "awp_dstrqc_tex": [_fp] * 23 + [_i, _i, _s] + [_fp] * 3 + [_i] * 9,
     # topo_t* travels as a plain address (ctypes.byref of awp_amd.refstructs.Topo)
     "topo_set_constants": [_fp],
     "topo_velocity_interior_H": [_fp],

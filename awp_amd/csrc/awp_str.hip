@@ -59,7 +59,24 @@ struct CStress6 {
   const void* f[6];
 };
 
-template <int BJ, bool NEWLIT, bool OOP, class ST>
+// Relaxation texture.  init_texture (reference src/awp/mesh.c:1103-1137)
+// gives every cell of the stress region one of 8 mechanisms, ww = 1..8, and
+// vx1, vx2, wwo are functions of the mechanism alone.  TEX selects where the
+// stress kernel takes them from:
+//   TEX_ARRAYS  the per-cell vx1, vx2, wwo and ww arrays (dstrqc_H);
+//   TEX_TABLE   the per-cell ww array and an 8-entry table (12 B/cell less);
+//   TEX_RULE    ww from init_texture's parity rule and the table (16 B/cell
+//               less): ni = i - xls, nj = j - yls, nk = k - align count
+//               from the region origin, and
+//               itx = (ni+1)&1, ity = (ni*NJ+nj+1)&1,
+//               itz = (nzt + (ni*NJ+nj)*nzt + nk)&1, ww = 1 + 4itx + 2ity + itz.
+enum { TEX_ARRAYS = 0, TEX_TABLE = 1, TEX_RULE = 2 };
+struct TexTable {
+  float vx1[8], vx2[8], wwo[8];  // indexed by ww - 1
+  int xls, yls, nj, nzt;         // TEX_RULE only
+};
+
+template <int BJ, bool NEWLIT, bool OOP, class ST, int TEX = TEX_ARRAYS>
 __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     typename ST::T* __restrict__ xx, typename ST::T* __restrict__ yy,
     typename ST::T* __restrict__ zz, typename ST::T* __restrict__ xy,
@@ -74,10 +91,13 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     const float* __restrict__ dcrjx, const float* __restrict__ dcrjy,
     const float* __restrict__ dcrjz, const float* __restrict__ vx1,
     const float* __restrict__ vx2, const int* __restrict__ ww, const float* __restrict__ wwo,
-    const StrLaunch L, const WaveScales SC) {
+    const StrLaunch L, const WaveScales SC, const TexTable TT) {
   __shared__ float s_coeff[16];
+  __shared__ float s_tex[24];
   const int tid = threadIdx.y * 64 + threadIdx.x;
   if (tid < 16) s_coeff[tid] = coeff[tid];
+  if (TEX != TEX_ARRAYS && tid < 24)
+    s_tex[tid] = tid < 8 ? TT.vx1[tid] : tid < 16 ? TT.vx2[tid - 8] : TT.wwo[tid - 16];
   __syncthreads();
 
   const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
@@ -144,8 +164,22 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     const float w_jp1 = ST::ld(wp, o + Y, sw), w_jp2 = ST::ld(wp, o + 2 * Y, sw), w_jm1 = ST::ld(wp, o - Y, sw);
 
     const long long p = pl + o;
-    const float f_vx1 = vx1[p], f_vx2 = vx2[p], f_wwo = wwo[p];
-    const int f_ww = ww[p];
+    float f_vx1, f_vx2, f_wwo;
+    int f_ww;
+    if (TEX == TEX_ARRAYS) {
+      f_vx1 = vx1[p]; f_vx2 = vx2[p]; f_wwo = wwo[p];
+      f_ww = ww[p];
+    } else {
+      if (TEX == TEX_TABLE) {
+        f_ww = ww[p];
+      } else {
+        const int ni = i - TT.xls, col = ni * TT.nj + (j - TT.yls);
+        f_ww = 1 + 4 * ((ni + 1) & 1) + 2 * ((col + 1) & 1) +
+               ((TT.nzt + col * TT.nzt + (k - AWP_ALIGN)) & 1);
+      }
+      const int t = (f_ww - 1) & 7;
+      f_vx1 = s_tex[t]; f_vx2 = s_tex[8 + t]; f_wwo = s_tex[16 + t];
+    }
     const float ca = s_coeff[(f_ww * 2 - 2) & 15], cb = s_coeff[(f_ww * 2 - 1) & 15];
     const float f_dcrj = dcrjx[i] * dy * dz;
 
@@ -330,13 +364,13 @@ __global__ __launch_bounds__(256) void fs_ghost_kernel(typename ST::T* __restric
 
 // f15: xx yy zz xy xz yz r1..r6 u1 v1 w1 in the storage type of ST; with OOP,
 // S0 holds the stresses before the step and f15's stresses receive the update.
-template <bool NEWLIT, bool OOP, class ST>
+template <bool NEWLIT, bool OOP, class ST, int TEX = TEX_ARRAYS>
 void dstrqc_common(const char* W, void* const* f15, const CStress6& S0, float* lam, float* mu,
                    float* qp,
                    float* coeff, float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt,
                    int nzt, hipStream_t St, float* lam_mu, float* vx1, float* vx2, int* ww,
                    float* wwo, int NX, int NPC, int rankx, int ranky, int s_i, int e_i, int s_j,
-                   int e_j, int d_i, const WaveScales& SC) {
+                   int e_j, int d_i, const WaveScales& SC, const TexTable& TT = TexTable{}) {
   typedef typename ST::T T;
   T* const* A = reinterpret_cast<T* const*>(f15);
   if (!awp_grid_ok(d_i, W)) return;
@@ -388,10 +422,10 @@ void dstrqc_common(const char* W, void* const* f15, const CStress6& S0, float* l
   L.nblocks = L.nkt * L.njt * nic;
 #define AWP_STR_ARGS                                                                          \
   A[0], A[1], A[2], A[3], A[4], A[5], S0, A[6], A[7], A[8], A[9], A[10], A[11], A[12], A[13], \
-      A[14], lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, vx1, vx2, ww, wwo, L, SC
-#define AWP_STR_LAUNCH(B)                                                                     \
-  hipLaunchKernelGGL((dstrqc_kernel<B, NEWLIT, OOP, ST>), dim3(L.nblocks), dim3(64, B), 0, St, \
-                     AWP_STR_ARGS)
+      A[14], lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, vx1, vx2, ww, wwo, L, SC, TT
+#define AWP_STR_LAUNCH(B)                                                                \
+  hipLaunchKernelGGL((dstrqc_kernel<B, NEWLIT, OOP, ST, TEX>), dim3(L.nblocks), dim3(64, B), 0, \
+                     St, AWP_STR_ARGS)
   switch (bj) {
     case 1: AWP_STR_LAUNCH(1); break;
     case 2: AWP_STR_LAUNCH(2); break;
@@ -533,6 +567,44 @@ void awp_dstrqc_oop(float* xx, float* yy, float* zz, float* xy, float* xz, float
   dstrqc_common<false, true, StF32>(W, f15, S0, lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, nyt,
                                     nzt, St, lam_mu, vx1, vx2, ww, wwo, NX, NPC, rankx, ranky,
                                     s_i, e_i, s_j, e_j, d_i, SC);
+}
+
+void awp_dstrqc_tex(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                    float* r1, float* r2, float* r3, float* r4, float* r5, float* r6, float* u1,
+                    float* v1, float* w1, float* lam, float* mu, float* qp, float* coeff,
+                    float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt, int nzt,
+                    hipStream_t St, float* lam_mu, const float* tex, int* ww, int NX, int NPC,
+                    int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i) {
+  const char* W = "awp_dstrqc_tex";
+  if (!tex) {
+    awp_set_error(AWP_ERR_ARG, "%s: the texture table is NULL", W);
+    return;
+  }
+  if (!awp_grid_ok(d_i, W)) return;
+  const AwpGrid& g = awp_state().grid[d_i];
+  TexTable TT;
+  for (int m = 0; m < 8; ++m) {
+    TT.vx1[m] = tex[m];
+    TT.vx2[m] = tex[8 + m];
+    TT.wwo[m] = tex[16 + m];
+  }
+  // init_texture's region on one rank: the interior columns, all levels
+  TT.xls = 2 + AWP_NGSL;
+  TT.yls = 2 + AWP_NGSL;
+  TT.nj = g.nyt;
+  TT.nzt = g.nzt;
+  void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
+  WaveScales SC = {};
+  if (ww)
+    dstrqc_common<false, false, StF32, TEX_TABLE>(W, f15, CStress6{}, lam, mu, qp, coeff, qs,
+                                                  dcrjx, dcrjy, dcrjz, nyt, nzt, St, lam_mu,
+                                                  nullptr, nullptr, ww, nullptr, NX, NPC, rankx,
+                                                  ranky, s_i, e_i, s_j, e_j, d_i, SC, TT);
+  else
+    dstrqc_common<false, false, StF32, TEX_RULE>(W, f15, CStress6{}, lam, mu, qp, coeff, qs,
+                                                 dcrjx, dcrjy, dcrjz, nyt, nzt, St, lam_mu,
+                                                 nullptr, nullptr, nullptr, nullptr, NX, NPC,
+                                                 rankx, ranky, s_i, e_i, s_j, e_j, d_i, SC, TT);
 }
 
 void fstr_H(float* zz, float* xz, float* yz, hipStream_t St, int s_i, int e_i, int s_j,

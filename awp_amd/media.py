@@ -243,6 +243,38 @@ def texture(dims: Dims, qm: QModel, DT: float) -> tuple[np.ndarray, ...]:
     return vx1, vx2, wwo, ww
 
 
+def texture_table(medium: "Medium", rule: bool = False) -> np.ndarray:
+    """The 8-entry form of the relaxation texture: vx1[8], vx2[8], wwo[8]
+    (float32) indexed by ww - 1, for awp_dstrqc_tex.  Raises ValueError
+    unless every cell with mechanism ww holds those values in the per-cell
+    arrays, and, with rule=True, unless ww itself follows init_texture's
+    parity rule over the interior (mesh.c:1103-1137), which the kernel then
+    evaluates instead of reading ww."""
+    ww = np.asarray(medium.ww)
+    tab = np.zeros(24, dtype=np.float32)
+    for m in range(1, 9):
+        sel = ww == m
+        if not sel.any():
+            continue
+        for c, name in enumerate(("vx1", "vx2", "wwo")):
+            v = np.asarray(getattr(medium, name), dtype=np.float32)[sel]
+            if not np.all(v == v[0]):
+                raise ValueError(f"{name} is not a function of the mechanism ww={m}")
+            tab[8 * c + m - 1] = v[0]
+    if rule:
+        d = medium.dims
+        (xls, xre), (yls, yre) = d.i_range, d.j_range
+        ni = np.arange(xre - xls + 1)[:, None, None]
+        nj = np.arange(yre - yls + 1)[None, :, None]
+        nk = np.arange(d.nzt)[None, None, :]
+        col = ni * (yre - yls + 1) + nj
+        want = 1 + 4 * ((ni + 1) & 1) + 2 * ((col + 1) & 1) + ((d.nzt + col * d.nzt + nk) & 1)
+        got = ww[xls:xre + 1, yls:yre + 1, ALIGN:ALIGN + d.nzt]
+        if not np.array_equal(got, want):
+            raise ValueError("ww does not follow init_texture's parity rule")
+    return tab
+
+
 def cerjan(dims: Dims, ND: int = 20, ARBC: float = 0.92, islowest: bool = True,
            NPC: int = 0, NX: int | None = None, NY: int | None = None,
            coords=(0, 0)) -> tuple[np.ndarray, np.ndarray, np.ndarray]:

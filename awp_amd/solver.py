@@ -120,7 +120,7 @@ class Solver(PmclStepOrder):
                  receivers: Receivers | None = None, nt_record: int = 0,
                  device: str = "cuda", variant: str = "dstrqc", NPC: int = 0,
                  plastic: PlasticMedium | Sequence | None = None, topography=None,
-                 kinematic=None, pointio: dict | None = None):
+                 kinematic=None, pointio: dict | None = None, texture: str = "arrays"):
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a GPU: the HIP path has no CPU fallback")
         self.lib = _lib.load()
@@ -137,6 +137,18 @@ class Solver(PmclStepOrder):
             raise ValueError("plasticity needs a plastic medium on every grid")
         self.plastic = pls if plastic is not None else None
         self.levels = [_Level(media[p], srcs[p], pls[p], self.device) for p in range(G)]
+        # relaxation texture of the stress update: the per-cell arrays
+        # (dstrqc_H), or the 8-entry table with the ww array ("table") or with
+        # init_texture's parity rule ("rule") through awp_dstrqc_tex
+        if texture not in ("arrays", "table", "rule"):
+            raise ValueError(f"unknown texture mode {texture!r}")
+        if texture != "arrays" and variant != "dstrqc":
+            raise ValueError("the texture table applies to the dstrqc variant")
+        self.texture = texture
+        if texture != "arrays":
+            from .media import texture_table
+            for L in self.levels:
+                L.tex = (C.c_float * 24)(*texture_table(L.medium, rule=texture == "rule").tolist())
         DH = (C.c_float * G)(*[m.DH for m in media])
         nxt = (C.c_int * G)(*[m.dims.nxt for m in media])
         nyt = (C.c_int * G)(*[m.dims.nyt for m in media])
@@ -278,6 +290,16 @@ class Solver(PmclStepOrder):
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:
+            return
+        if self.texture != "arrays":
+            self.lib.awp_dstrqc_tex(
+                *(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                       "r5", "r6", "u1", "v1", "w1")),
+                _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["coeff"]), _ptr(m["qs"]),
+                _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nyt, d.nzt, self.stream,
+                _ptr(m["lam_mu"]), C.addressof(L.tex),
+                _ptr(m["ww"]) if self.texture == "table" else None,
+                d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, p)
             return
         fn = self.lib.dstrqc_H if self.variant == "dstrqc" else self.lib.dstrqc_H_new
         fn(*(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",

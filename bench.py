@@ -37,6 +37,8 @@ HBM_PEAK_GBS = 8000.0
 BYTES_VEL = 52.0      # 10 reads + 3 writes of 4 B per cell
 BYTES_STR = 140.0     # 23 reads + 12 writes of 4 B per cell
 BYTES_STEP = BYTES_VEL + BYTES_STR
+# awp_dstrqc_tex: vx1, vx2, wwo from a table (and ww from init_texture's rule)
+TEXTURE_SAVES = {"arrays": 0.0, "table": 12.0, "rule": 16.0}
 # 16-bit wavefield storage: 9 x 2 B + d1 4 B read, 3 x 2 B written (velocity);
 # 15 x 2 B + 8 medium x 4 B read, 12 x 2 B written (stress)
 BYTES16_VEL = 28.0
@@ -89,6 +91,10 @@ def parse():
     ap.add_argument("--storage", choices=("f32", "bf16", "fp16"), default="f32",
                     help="wavefield storage of the anelastic workload (16-bit: per-field "
                          "power-of-two scales, binary32 arithmetic)")
+    ap.add_argument("--texture", choices=("arrays", "table", "rule"), default="arrays",
+                    help="relaxation texture of the FP32 stress update: per-cell vx1/vx2/wwo "
+                         "arrays (dstrqc_H), or awp_dstrqc_tex's 8-entry table with the ww "
+                         "array (-12 B/cell) or with init_texture's parity rule (-16 B/cell)")
     ap.add_argument("--nt", type=int, default=2000, help="adjoint workload: time steps")
     ap.add_argument("--checkpoint-every", type=int, default=50)
     ap.add_argument("--fd-directions", type=int, default=5,
@@ -249,8 +255,9 @@ def main():
     med = w.medium()
     setup_s = time.perf_counter() - t0
     if args.storage == "f32":
-        sol = Solver(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1)
-        b_vel, b_str = BYTES_VEL, BYTES_STR
+        sol = Solver(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1,
+                     texture=args.texture)
+        b_vel, b_str = BYTES_VEL, BYTES_STR - TEXTURE_SAVES[args.texture]
     else:
         from awp_amd.solver import Solver16
         sol = Solver16(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1,
@@ -331,7 +338,8 @@ def main():
 
     traffic = None
     tfile = ROOT / "profiles" / "dstrqc_anelastic_512x512x256_traffic.json"
-    if tfile.exists() and (args.nx, args.ny, args.nz) == (512, 512, 256) and args.storage == "f32":
+    if (tfile.exists() and (args.nx, args.ny, args.nz) == (512, 512, 256) and args.storage == "f32"
+            and args.texture == "arrays"):
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
@@ -353,10 +361,15 @@ def main():
         "config": {"workload": w.name, "storage": args.storage, "nx": w.dims.nxt, "ny": w.dims.nyt, "nz": w.dims.nzt,
                    "h_m": w.DH, "dt_s": w.DT, "attenuation": "Qs=50vs[km/s], Qp=2Qs, 8 mechanisms",
                    "medium": "layered vs 1500-4500 m/s + 5% von Karman (a=500 m, H=0.5), seed 7",
-                   "step": "dvelcx_H_opt, dstrqc_H (+FS ghosts), addsrc_H, fstr_H, receivers"
+                   "texture": args.texture,
+                   "step": ("dvelcx_H_opt, " + ("dstrqc_H" if args.texture == "arrays" else
+                                                f"awp_dstrqc_tex ({args.texture})")
+                            + " (+FS ghosts), addsrc_H, fstr_H, receivers")
                            + (f", free-surface volume output every {args.volume_output} steps"
                               if args.volume_output else "")},
-        "roofline": {"bound": "hbm", "kernel": "dstrqc_H", "achieved": str_gbs,
+        "roofline": {"bound": "hbm",
+                     "kernel": "dstrqc_H" if args.texture == "arrays" else "awp_dstrqc_tex",
+                     "achieved": str_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "traffic_source": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and "

@@ -314,6 +314,21 @@ void awp_dstrqc_oop(float* xx, float* yy, float* zz, float* xy, float* xz, float
                     float* vx2, int* ww, float* wwo, int NX, int NPC, int rankx, int ranky,
                     int s_i, int e_i, int s_j, int e_j, int d_i);
 
+/* dstrqc_H (reference src/awp/kernel.cu:1201-1239) with the relaxation
+ * texture as a table.  init_texture (src/awp/mesh.c:1103-1137) assigns every
+ * cell one of 8 mechanisms ww = 1..8, and vx1, vx2 and wwo are functions of
+ * the mechanism alone: tex (host, 24 floats) holds vx1[8], vx2[8], wwo[8]
+ * indexed by ww - 1, replacing the three per-cell arrays (12 B/cell less).
+ * ww NULL takes the mechanism from init_texture's parity rule over this
+ * rank's interior (16 B/cell less).  Results equal dstrqc_H's bit for bit
+ * whenever the arrays dstrqc_H would read hold tex[ww - 1]. */
+void awp_dstrqc_tex(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                    float* r1, float* r2, float* r3, float* r4, float* r5, float* r6, float* u1,
+                    float* v1, float* w1, float* lam, float* mu, float* qp, float* coeff,
+                    float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt, int nzt,
+                    hipStream_t St, float* lam_mu, const float* tex, int* ww, int NX, int NPC,
+                    int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i);
+
 /* ---- The reference's structs at the topography and Lagrange boundary -----
  * The topography and Lagrange launchers take the reference's own structs by
  * pointer.  They are restated here member for member, in the reference's

@@ -117,16 +117,24 @@ def test_anelastic_512_random_state(torch_gpu, oracle_lib, many_threads):
     med = w.medium()
     g = Solver(med, w.sources, w.receivers, nt_record=4)
     o = OracleSolver(med, w.sources, w.receivers, nt_record=4, dtype=np.float32)
+    # the same step with the texture from init_texture's rule (awp_dstrqc_tex)
+    gt = Solver(med, w.sources, w.receivers, nt_record=4, texture="rule")
     del med
     _seed_state(g, o, seed=512)
+    for n in FIELDS:
+        gt.f[n].copy_(g.f[n])
     g.start()
     o.start()
+    gt.start()
     _steps(g, 3)
     _steps(o, 3)
+    _steps(gt, 3)
     assert np.array_equal(g.traces(), o.traces())
+    assert np.array_equal(gt.traces(), o.traces())
     for n in FIELDS:
         a = g.field(n)
         assert np.array_equal(a, o.f[n]), (n, int((a != o.f[n]).sum()))
+        assert np.array_equal(gt.field(n), a), n
         del a
 
 

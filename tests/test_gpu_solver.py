@@ -72,3 +72,28 @@ def test_plastic_solver_matches_oracle(torch_gpu, oracle_lib):
     assert np.array_equal(g.traces(), o32.traces())
     for n in ("u1", "w1", "xx", "xz", "yz", "yldfac", "neta"):
         assert np.array_equal(g.field(n), o32.f[n]), n
+
+
+@pytest.mark.parametrize("texture", ["table", "rule"])
+def test_texture_table_matches_arrays(torch_gpu, oracle_lib, texture):
+    """awp_dstrqc_tex (vx1, vx2, wwo from an 8-entry table keyed by the
+    mechanism, ww read or from init_texture's rule) against dstrqc_H and the
+    oracle on odd and even grid sizes."""
+    from awp_amd.solver import Solver
+    from oracle.oracle import OracleSolver
+    for nx, ny, nz in ((47, 41, 36), (40, 38, 32)):
+        w = configs.anelastic(nx=nx, ny=ny, nz=nz, nt=40, DH=200.0, DT=0.01)
+        print(f"texture {texture}: {w.name}", flush=True)
+        med = w.medium()
+        g = Solver(med, w.sources, w.receivers, nt_record=w.nt, texture=texture)
+        g.run(w.nt, check_every=10)
+        a = Solver(med, w.sources, w.receivers, nt_record=w.nt)
+        a.run(w.nt)
+        o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, dtype=np.float32)
+        o.run(w.nt)
+        assert np.abs(g.traces()).max() > 0
+        assert np.array_equal(g.traces(), o.traces())
+        for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                  "r5", "r6"):
+            assert np.array_equal(g.field(n), a.field(n)), n
+            assert np.array_equal(g.field(n), o.f[n]), n
