@@ -40,6 +40,7 @@ SIGNATURES: dict[str, list] = {
     "drprecpc_calc_H_opt": [_fp] * 13 + [_i] * 5 + [_s, _i],
     "awp_drprecpc_calc_tv": [_fp] * 13 + [_i] * 5 + [_s, _i, _f],
     "drprecpc_app_H": [_fp] * 9 + [_i] * 5 + [_s, _i],
+    "awp_dvelcx_dm": [_fp] * 13 + [_i, _i, _s, _i, _i, _i, _i],
     "dvelc2_H": [_fp] * 13 + [_i, _i, _s, _i],
     "dstrqc2_H": [_fp] * 22 + [_i, _i, _s] + [_fp] * 5 + [_i] * 5,
     "intp3d_H": [_fp] * 18 + [_i, _i, _i, _s, _i],
@@ -62,6 +63,7 @@ SIGNATURES: dict[str, list] = {
     "awp_dvelcx_oop": [_fp] * 16 + [_i, _i, _s, _i, _i, _i, _i],
     "awp_dstrqc_oop": [_fp] * 24 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,
     "awp_dstrqc_tex": [_fp] * 23 + [_i, _i, _s] + [_fp] * 3 + [_i] * 9,
+    "awp_step": [_fp] * 11 + [_i, _i, _i, _s],
     # topo_t* travels as a plain address (ctypes.byref of awp_amd.refstructs.Topo)
     "topo_set_constants": [_fp],
     "topo_velocity_interior_H": [_fp],

@@ -276,12 +276,20 @@ __constant__ SwapPass c_swap_pass[9][2] = {
     {{0, 3, 3, 3}, {0, 3, 3, 3}},  // yz: kh align+4
 };
 
+// One block per (field, tile): blockIdx.x = f * ntiles + tile, so the
+// blocks of one field run together and neighbouring tiles share their halo
+// columns in L2.  vec: each fine column's levels align .. align+8 arrive in
+// two 16-byte loads and one 4-byte load (yline a multiple of 4 floats and
+// 16-byte aligned field bases), else one 4-byte load per level.
 __global__ __launch_bounds__(64) void swap3_kernel(Fields9 L9, const Fields9 H9,
-                                                   const Coupling C, int ntj) {
+                                                   const Coupling C, int ntj, int ntiles,
+                                                   int vec) {
   __shared__ float slab[SW_FI * SW_FJ * SW_CP];
   __shared__ double wt[125];
   const int lane = (int)threadIdx.x;
-  const int bi = (int)blockIdx.x / ntj, bj = (int)blockIdx.x % ntj;
+  const int f = (int)blockIdx.x / ntiles;
+  const int tile = (int)blockIdx.x % ntiles;
+  const int bi = tile / ntj, bj = tile % ntj;
   const int ti = lane / SW_TJ, tj = lane % SW_TJ;
   const int i0 = C.ci0 + bi * SW_TI, j0 = C.cj0 + bj * SW_TJ;
   const int i = i0 + ti, j = j0 + tj;
@@ -303,36 +311,44 @@ __global__ __launch_bounds__(64) void swap3_kernel(Fields9 L9, const Fields9 H9,
             (1. + SW_WWL - abs(kk));
   }
 
-  const long long pc = (long long)i * C.slc + (long long)j * C.ylc + C.kc;
-  for (int f = 0; f < 9; ++f) {
-    __syncthreads();
-    const float* src = H9.f[f];
-    for (int t = lane; t < SW_FI * SW_FJ * SW_NL; t += 64) {
-      const int l = t & (SW_NL - 1);
-      const int col = t >> 3;
-      const int b = col % SW_FJ, a = col / SW_FJ;
-      const int fi = fi0 + a, fj = fj0 + b;
-      if (fi < C.nxf_pad && fj < C.nyf_pad)
-        slab[(a * SW_FJ + b) * SW_CP + l] =
-            src[(long long)fi * C.slf + (long long)fj * C.ylf + AWP_ALIGN + 1 + l];
-    }
-    __syncthreads();
-    if (!active) continue;
-    const int np = c_swap_np[f];
-    for (int q = 0; q < np; ++q) {
-      const SwapPass P = c_swap_pass[f][q];
-      const int a0 = 3 * ti + P.oi - SW_WWL, b0 = 3 * tj + P.oj - SW_WWL, l0 = P.kh - SW_WWL;
-      float sum = 0.0f;
-      int n = 0;
-      for (int jj = 0; jj < 5; ++jj)
-        for (int ii = 0; ii < 5; ++ii) {
-          const float* col = &slab[((a0 + ii) * SW_FJ + (b0 + jj)) * SW_CP + l0];
+  // the fine slab of this field: one lane per column, levels align+1 .. +8
+  const float* src = H9.f[f];
+  for (int col = lane; col < SW_FI * SW_FJ; col += 64) {
+    const int a = col / SW_FJ, b = col % SW_FJ;
+    const int fi = fi0 + a, fj = fj0 + b;
+    if (fi < C.nxf_pad && fj < C.nyf_pad) {
+      const float* q = src + (long long)fi * C.slf + (long long)fj * C.ylf + AWP_ALIGN;
+      float* d = &slab[col * SW_CP];
+      if (vec) {
+        const float4 lo = *reinterpret_cast<const float4*>(q);
+        const float4 hi = *reinterpret_cast<const float4*>(q + 4);
+        const float x8 = q[8];
+        d[0] = lo.y; d[1] = lo.z; d[2] = lo.w;
+        d[3] = hi.x; d[4] = hi.y; d[5] = hi.z; d[6] = hi.w;
+        d[7] = x8;
+      } else {
 #pragma unroll
-          for (int kk = 0; kk < 5; ++kk, ++n)
-            sum = (float)((double)sum + wt[n] * (double)col[kk]);
-        }
-      L9.f[f][pc + P.kc] = sum;
+        for (int l = 0; l < SW_NL; ++l) d[l] = q[1 + l];
+      }
     }
+  }
+  __syncthreads();
+  if (!active) return;
+  const long long pc = (long long)i * C.slc + (long long)j * C.ylc + C.kc;
+  const int np = c_swap_np[f];
+  for (int q = 0; q < np; ++q) {
+    const SwapPass P = c_swap_pass[f][q];
+    const int a0 = 3 * ti + P.oi - SW_WWL, b0 = 3 * tj + P.oj - SW_WWL, l0 = P.kh - SW_WWL;
+    float sum = 0.0f;
+    int n = 0;
+    for (int jj = 0; jj < 5; ++jj)
+      for (int ii = 0; ii < 5; ++ii) {
+        const float* col = &slab[((a0 + ii) * SW_FJ + (b0 + jj)) * SW_CP + l0];
+#pragma unroll
+        for (int kk = 0; kk < 5; ++kk, ++n)
+          sum = (float)((double)sum + wt[n] * (double)col[kk]);
+      }
+    L9.f[f][pc + P.kc] = sum;
   }
 }
 
@@ -486,7 +502,11 @@ void swap_H(float* xxl, float* yyl, float* zzl, float* xyl, float* xzl, float* y
   Fields9 H9 = {{u1h, v1h, w1h, xxh, yyh, zzh, xyh, xzh, yzh}};
   const int nti = awp_cdiv(C.ci1 - C.ci0 + 1, SW_TI);
   const int ntj = awp_cdiv(C.cj1 - C.cj0 + 1, SW_TJ);
-  hipLaunchKernelGGL(swap3_kernel, dim3(nti * ntj), dim3(64), 0, St, L9, H9, C, ntj);
+  int vec = (C.ylf % 4 == 0) ? 1 : 0;
+  for (int f = 0; f < 9; ++f)
+    if (reinterpret_cast<uintptr_t>(H9.f[f]) % 16 != 0) vec = 0;
+  hipLaunchKernelGGL(swap3_kernel, dim3(9 * nti * ntj), dim3(64), 0, St, L9, H9, C, ntj,
+                     nti * ntj, vec);
   awp_check_launch(W);
 }
 

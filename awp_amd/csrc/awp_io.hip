@@ -10,7 +10,7 @@
 
 // Defaults from sweeps on MI355X at 512x512x256: velocity 4 rows x 2 i-chunks
 // (all workgroups resident), stress 16 rows x ~32-plane i-chunks.
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1};
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 0, 2048};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -196,6 +196,11 @@ int awp_set_tuning(const char* key, int value) {
   } else if (!strcmp(key, "topo_q_ichunk")) {
     if (value < 1 || value > 4096) return -1;
     t.topo_q_ichunk = value;
+  } else if (!strcmp(key, "step_ichunk")) {
+    t.step_ichunk = value < 0 ? 0 : value;
+  } else if (!strcmp(key, "step_target_wgs")) {
+    if (value < 1) return -1;
+    t.step_target_wgs = value;
   } else {
     return -1;
   }

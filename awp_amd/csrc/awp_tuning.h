@@ -13,6 +13,8 @@ struct AwpTuning {
   int topo_vel_ichunk;  // planes per march of the curvilinear velocity kernel
   int topo_str_ichunk;  // same, curvilinear stress kernel
   int topo_q_ichunk;    // planes per march of the two-pass stress update
+  int step_ichunk;      // >0 forces the i-chunk length of the fused step (awp_step)
+  int step_target_wgs;  // fused step: workgroups to aim for when splitting the i march
 };
 
 AwpTuning& awp_tuning();

@@ -28,7 +28,11 @@ struct VelLaunch {
 // OOP: the velocities before the step are read from u0/v0/w0 and the
 // updated ones written to u1/v1/w1 (distinct arrays); otherwise u1/v1/w1 are
 // updated in place and u0/v0/w0 are unused.
-template <int BJ, bool OOP, class ST>
+// DM2: the grid sits above a coarser one, and the lane at k = align+2 (which
+// the 4th-order update skips there) applies dvelc2's 2nd-order update
+// (kernel.cu:2289-2356) from the operands it has loaded anyway, so one
+// launch does dvelcx_H_opt + dvelc2_H.
+template <int BJ, bool OOP, class ST, bool DM2 = false>
 __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     typename ST::T* __restrict__ u1, typename ST::T* __restrict__ v1,
     typename ST::T* __restrict__ w1, const typename ST::T* __restrict__ u0,
@@ -47,7 +51,8 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
   const int j = L.j0 + jt * BJ + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const int k = L.kb + kt * 64 + (int)threadIdx.x;
   if (j > L.j1) return;
-  if (k < L.k0 || k > L.k1) return;
+  const bool dm2 = DM2 && k == AWP_ALIGN + 2;
+  if ((k < L.k0 && !dm2) || k > L.k1) return;
   const int ia = L.i0 + ic * L.ichunk;
   int nsteps = L.ni - ic * L.ichunk;
   if (nsteps > L.ichunk) nsteps = L.ichunk;
@@ -112,6 +117,28 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
                 v = ST::ldb(OOP ? RP(v0, i, 0, 0) : V, kw, SC.s[AWP_SV]),
                 w = ST::ldb(OOP ? RP(w0, i, 0, 0) : W, kw, SC.s[AWP_SW]);
 
+    if (DM2 && dm2) {
+      // dvelc2: density means as float sums scaled by the double 0.25, the
+      // sponge factor in the order dcrjz * dcrjx * dcrjy
+      const float f_dcrj = dz * dcrjx[i] * dy;
+      float g3 = (float)(0.25 * (double)(dc + ec + djm + ejm));
+      g3 = dth / g3;
+      ST::stb(W, kw, (w + g3 * ((xz_p1 - xz_0) + (yz_0 - yz_jm1) + (zz_kp1 - zz_0))) * f_dcrj,
+              SC.inv[AWP_SW]);
+      float g1 = (float)(0.25 * (double)(dc + djm + dkm + djk));
+      float g2 = (float)(0.25 * (double)(dc + ec + dkm + ekm));
+      g1 = dth / g1;
+      g2 = dth / g2;
+      ST::stb(U, kw, (u + g1 * ((xx_0 - xx_m1) + (xy_0 - xy_jm1) + (xz_0 - xz_km1))) * f_dcrj,
+              SC.inv[AWP_SU]);
+      ST::stb(V, kw, (v + g2 * ((xy_p1 - xy_0) + (yy_jp1 - yy_0) + (yz_0 - yz_km1))) * f_dcrj,
+              SC.inv[AWP_SV]);
+      xx_m2 = xx_m1; xx_m1 = xx_0; xx_0 = xx_p1;
+      xy_m1 = xy_0; xy_0 = xy_p1; xy_p1 = xy_p2;
+      xz_m1 = xz_0; xz_0 = xz_p1; xz_p1 = xz_p2;
+      dc = ec; djm = ejm; dkm = ekm; djk = ejk;
+      continue;
+    }
     const float dcrj = dcrjx[i] * dy * dz;
     float f_d1 = 0.25f * (dc + djm + dkm + djk);
     float f_d2 = 0.25f * (dc + ec + dkm + ekm);
@@ -139,7 +166,7 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
 #undef RP
 }
 
-template <int BJ, class ST>
+template <int BJ, class ST, bool DM2 = false>
 void launch_vel(void* const* f9, void* const* v0, float* dcrjx, float* dcrjy, float* dcrjz,
                 float* d_1, hipStream_t St, VelLaunch L, const WaveScales& SC) {
   typedef typename ST::T T;
@@ -148,7 +175,12 @@ void launch_vel(void* const* f9, void* const* v0, float* dcrjx, float* dcrjy, fl
   awp_split_i(L.ni, nkj, awp_tuning().vel_ichunk, awp_tuning().vel_target_wgs, &nic, &ichunk);
   L.ichunk = ichunk;
   L.nblocks = nkj * nic;
-  if (v0)
+  if (DM2)
+    hipLaunchKernelGGL((dvelcx_kernel<BJ, false, ST, true>), dim3(L.nblocks), dim3(64, BJ), 0,
+                       St, (T*)f9[0], (T*)f9[1], (T*)f9[2], nullptr, nullptr, nullptr,
+                       (const T*)f9[3], (const T*)f9[4], (const T*)f9[5], (const T*)f9[6],
+                       (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz, d_1, L, SC);
+  else if (v0)
     hipLaunchKernelGGL((dvelcx_kernel<BJ, true, ST>), dim3(L.nblocks), dim3(64, BJ), 0, St,
                        (T*)f9[0], (T*)f9[1], (T*)f9[2], (const T*)v0[0], (const T*)v0[1],
                        (const T*)v0[2], (const T*)f9[3], (const T*)f9[4], (const T*)f9[5],
@@ -163,7 +195,7 @@ void launch_vel(void* const* f9, void* const* v0, float* dcrjx, float* dcrjy, fl
 
 // f9: u1 v1 w1 xx yy zz xy xz yz in the storage type of ST; v0: null (u1 v1
 // w1 updated in place) or the velocities before the step (u0 v0 w0).
-template <class ST>
+template <class ST, bool DM2 = false>
 void dvelcx_common(const char* W, void* const* f9, void* const* v0, float* dcrjx, float* dcrjy,
                    float* dcrjz, float* d_1, int nyt, int nzt, hipStream_t St, int s_i, int e_i,
                    int d_i, int ngrids, const WaveScales& SC) {
@@ -202,11 +234,11 @@ void dvelcx_common(const char* W, void* const* f9, void* const* v0, float* dcrjx
   L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
   if (L.k1 < L.k0) return;
   switch (bj) {
-    case 1: launch_vel<1, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 2: launch_vel<2, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 8: launch_vel<8, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 16: launch_vel<16, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    default: launch_vel<4, ST>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 1: launch_vel<1, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 2: launch_vel<2, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 8: launch_vel<8, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    case 16: launch_vel<16, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+    default: launch_vel<4, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
   }
   awp_check_launch(W);
 }
@@ -221,6 +253,39 @@ extern "C" void dvelcx_H_opt(float* u1, float* v1, float* w1, float* xx, float* 
   WaveScales SC = {};
   dvelcx_common<StF32>("dvelcx_H_opt", f9, nullptr, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St,
                        s_i, e_i, d_i, ngrids, SC);
+}
+
+extern "C" void awp_dvelcx_dm(float* u1, float* v1, float* w1, float* xx, float* yy,
+                              float* zz, float* xy, float* xz, float* yz, float* dcrjx,
+                              float* dcrjy, float* dcrjz, float* d_1, int nyt, int nzt,
+                              hipStream_t St, int s_i, int e_i, int d_i, int ngrids) {
+  const char* W = "awp_dvelcx_dm";
+  void* f9[9] = {u1, v1, w1, xx, yy, zz, xy, xz, yz};
+  WaveScales SC = {};
+  if (d_i >= ngrids - 1) {
+    // the lowest grid has no dvelc2 level
+    dvelcx_common<StF32>(W, f9, nullptr, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i, d_i,
+                         ngrids, SC);
+    return;
+  }
+  if (!awp_grid_ok(d_i, W)) return;
+  const AwpGrid& g = awp_state().grid[d_i];
+  // dvelc2_H covers the whole interior (kernel.cu:2301): so must the sweep
+  if (s_i != 2 + AWP_NGSL || e_i != g.nxt + AWP_NGSL + 1) {
+    awp_set_error(AWP_ERR_ARG, "%s: i range [%d, %d] is not grid %d's interior [%d, %d]", W, s_i,
+                  e_i, d_i, 2 + AWP_NGSL, g.nxt + AWP_NGSL + 1);
+    return;
+  }
+  if (g.nzt < 3) {
+    awp_set_error(AWP_ERR_ARG, "%s: grid %d has nzt=%d < 3", W, d_i, g.nzt);
+    return;
+  }
+  if (!awp_tuning().kalign) {
+    awp_set_error(AWP_ERR_STATE, "%s: needs k tiles that start at k = align (kalign=1)", W);
+    return;
+  }
+  dvelcx_common<StF32, true>(W, f9, nullptr, dcrjx, dcrjy, dcrjz, d_1, nyt, nzt, St, s_i, e_i,
+                             d_i, ngrids, SC);
 }
 
 extern "C" void awp_dvelcx_s16(uint16_t* u1, uint16_t* v1, uint16_t* w1, uint16_t* xx,

@@ -74,6 +74,10 @@ def check_dm_stack(media: Sequence[Medium]) -> None:
             raise ValueError("all grids share one time step")
 
 
+# the state awp_step reads from one set and writes to the other
+STEP_FIELDS = ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")
+
+
 class _Level:
     """Device state of one grid: wavefield, medium, sources, plasticity."""
 
@@ -120,7 +124,8 @@ class Solver(PmclStepOrder):
                  receivers: Receivers | None = None, nt_record: int = 0,
                  device: str = "cuda", variant: str = "dstrqc", NPC: int = 0,
                  plastic: PlasticMedium | Sequence | None = None, topography=None,
-                 kinematic=None, pointio: dict | None = None, texture: str = "arrays"):
+                 kinematic=None, pointio: dict | None = None, texture: str = "arrays",
+                 fused: bool = False, dm_fused: bool = True):
         if not torch.cuda.is_available():
             raise RuntimeError("Solver needs a GPU: the HIP path has no CPU fallback")
         self.lib = _lib.load()
@@ -145,10 +150,23 @@ class Solver(PmclStepOrder):
         if texture != "arrays" and variant != "dstrqc":
             raise ValueError("the texture table applies to the dstrqc variant")
         self.texture = texture
-        if texture != "arrays":
+        # on a grid above a coarser one, dvelc2_H's level in the velocity
+        # sweep (awp_dvelcx_dm) instead of a launch of its own
+        self.dm_fused = bool(dm_fused)
+        # fused: the whole NVE = 1 step of one grid in one sweep (awp_step),
+        # from the state in f into a second state set; the two swap roles
+        # every step
+        self.fused = bool(fused)
+        if self.fused and (G != 1 or plastic is not None or topography is not None
+                           or variant != "dstrqc" or NPC != 0):
+            raise ValueError("the fused step runs one Cartesian grid, NVE = 1, dstrqc, NPC = 0")
+        if texture != "arrays" or self.fused:
             from .media import texture_table
             for L in self.levels:
                 L.tex = (C.c_float * 24)(*texture_table(L.medium, rule=texture == "rule").tolist())
+        if self.fused:
+            L = self.levels[0]
+            self.f_next = {n: torch.empty_like(L.f[n]) for n in STEP_FIELDS}
         DH = (C.c_float * G)(*[m.DH for m in media])
         nxt = (C.c_int * G)(*[m.dims.nxt for m in media])
         nyt = (C.c_int * G)(*[m.dims.nyt for m in media])
@@ -249,13 +267,16 @@ class Solver(PmclStepOrder):
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         s_i, e_i = d.i_range
-        self.lib.dvelcx_H_opt(_ptr(f["u1"]), _ptr(f["v1"]), _ptr(f["w1"]), _ptr(f["xx"]),
-                              _ptr(f["yy"]), _ptr(f["zz"]), _ptr(f["xy"]), _ptr(f["xz"]),
-                              _ptr(f["yz"]), _ptr(m["dcrjx"]), _ptr(m["dcrjy"]),
-                              _ptr(m["dcrjz"]), _ptr(m["d1"]), d.nyt, d.nzt, self.stream, s_i,
-                              e_i, p, self.ngrids)
+        fn = (self.lib.awp_dvelcx_dm if self.dm_fused and p < self.ngrids - 1
+              else self.lib.dvelcx_H_opt)
+        fn(_ptr(f["u1"]), _ptr(f["v1"]), _ptr(f["w1"]), _ptr(f["xx"]), _ptr(f["yy"]),
+           _ptr(f["zz"]), _ptr(f["xy"]), _ptr(f["xz"]), _ptr(f["yz"]), _ptr(m["dcrjx"]),
+           _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), _ptr(m["d1"]), d.nyt, d.nzt, self.stream, s_i,
+           e_i, p, self.ngrids)
 
     def velocity_dm(self, p: int) -> None:
+        if self.dm_fused:
+            return  # done in the velocity sweep (awp_dvelcx_dm)
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         self.lib.dvelc2_H(*(_ptr(f[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz",
@@ -378,6 +399,43 @@ class Solver(PmclStepOrder):
         self.lib.awp_receivers_gather(_ptr(f["u1"]), _ptr(f["v1"]), _ptr(f["w1"]),
                                       _ptr(r["idx"]), r["n"], _ptr(r["out"]), slot, self.stream,
                                       0)
+
+    def start(self) -> None:
+        if self.fused:
+            # the second state set starts as a copy of the first: the step
+            # leaves the cells it does not update as they are in both
+            for n in STEP_FIELDS:
+                self.f_next[n].copy_(self.levels[0].f[n])
+        super().start()
+
+    def step(self) -> None:
+        if not self.fused:
+            return super().step()
+        # pmcl3d's NVE = 1 order on one grid: velocity, stress (with the
+        # free-surface ghosts), sources, fstr, receivers
+        self.cur_step += 1
+        self.fused_step()
+        self._sources()
+        self.free_surface()
+        self.record(self.cur_step - 1)
+        if self.pointio is not None:
+            self.pointio.output(self)
+
+    def fused_step(self) -> None:
+        """awp_step from the current state into the second set, then swap
+        the two so that f holds the state after the step."""
+        L = self.levels[0]
+        d, f, m = L.dims, L.f, L.m
+        a9 = (C.c_void_p * 9)(*[_ptr(f[n]) for n in STEP_FIELDS])
+        b9 = (C.c_void_p * 9)(*[_ptr(self.f_next[n]) for n in STEP_FIELDS])
+        r6 = (C.c_void_p * 6)(*[_ptr(f[n]) for n in ("r1", "r2", "r3", "r4", "r5", "r6")])
+        med = (C.c_void_p * 5)(*[_ptr(m[n]) for n in ("d1", "lam", "mu", "qp", "qs")])
+        self.lib.awp_step(a9, b9, r6, med, _ptr(m["coeff"]), C.addressof(L.tex),
+                          _ptr(m["ww"]) if self.texture != "rule" else None,
+                          _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]),
+                          _ptr(m["lam_mu"]), d.nxt, d.nyt, d.nzt, self.stream)
+        for n in STEP_FIELDS:
+            f[n], self.f_next[n] = self.f_next[n], f[n]
 
     def run(self, nt: int, check_every: int = 0) -> None:
         if self.cur_step == 0:

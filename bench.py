@@ -37,6 +37,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_VEL = 52.0      # 10 reads + 3 writes of 4 B per cell
 BYTES_STR = 140.0     # 23 reads + 12 writes of 4 B per cell
 BYTES_STEP = BYTES_VEL + BYTES_STR
+# awp_step: u v w and the six stresses read once (36 B) and written once
+# (36 B), memory variables read and written (48 B), d1 lam mu qp qs (20 B),
+# vx1 vx2 ww wwo (16 B)
+BYTES_FUSED = 156.0
 # awp_dstrqc_tex: vx1, vx2, wwo from a table (and ww from init_texture's rule)
 TEXTURE_SAVES = {"arrays": 0.0, "table": 12.0, "rule": 16.0}
 # 16-bit wavefield storage: 9 x 2 B + d1 4 B read, 3 x 2 B written (velocity);
@@ -95,6 +99,9 @@ def parse():
                     help="relaxation texture of the FP32 stress update: per-cell vx1/vx2/wwo "
                          "arrays (dstrqc_H), or awp_dstrqc_tex's 8-entry table with the ww "
                          "array (-12 B/cell) or with init_texture's parity rule (-16 B/cell)")
+    ap.add_argument("--fused", action="store_true",
+                    help="anelastic workload: the whole step in one sweep (awp_step, state A -> "
+                         "state B, texture table; 144 B/cell, 140 B with --texture rule)")
     ap.add_argument("--nt", type=int, default=2000, help="adjoint workload: time steps")
     ap.add_argument("--checkpoint-every", type=int, default=50)
     ap.add_argument("--fd-directions", type=int, default=5,
@@ -256,8 +263,12 @@ def main():
     setup_s = time.perf_counter() - t0
     if args.storage == "f32":
         sol = Solver(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1,
-                     texture=args.texture)
+                     texture=args.texture, fused=args.fused)
         b_vel, b_str = BYTES_VEL, BYTES_STR - TEXTURE_SAVES[args.texture]
+        if args.fused:
+            # one sweep: the state read once and written once, the texture
+            # from the table (ww read, or from the parity rule)
+            b_vel, b_str = 0.0, BYTES_FUSED - (16.0 if args.texture == "rule" else 12.0)
     else:
         from awp_amd.solver import Solver16
         sol = Solver16(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1,
@@ -305,9 +316,13 @@ def main():
         e = ev[n]
         sol.cur_step += 1
         e[0].record(stream)
-        sol.velocity()
-        e[1].record(stream)
-        sol.stress()
+        if args.fused:
+            sol.fused_step()
+            e[1].record(stream)
+        else:
+            sol.velocity()
+            e[1].record(stream)
+            sol.stress()
         e[2].record(stream)
         if sol.cur_step < sol.source_steps(0):
             sol.add_source(0, sol.cur_step + 1)
@@ -328,10 +343,13 @@ def main():
     step_ms = [e[0].elapsed_time(e[3]) for e in ev]
     vel_avg = float(np.mean(vel_ms))
     str_avg = float(np.mean(str_ms))
+    if args.fused:
+        # one launch between the first two events
+        str_avg, vel_avg = vel_avg, 0.0
     ms_per_step = wall * 1e3 / args.steps
     value = cells * args.steps / wall
     str_gbs = b_str * cells / (str_avg * 1e-3) / 1e9
-    vel_gbs = b_vel * cells / (vel_avg * 1e-3) / 1e9
+    vel_gbs = b_vel * cells / (vel_avg * 1e-3) / 1e9 if vel_avg > 0 else 0.0
     step_gbs = (b_vel + b_str) * cells / (ms_per_step * 1e-3) / 1e9
     tr = sol.traces()
     finite = bool(np.isfinite(tr).all())
@@ -339,7 +357,7 @@ def main():
     traffic = None
     tfile = ROOT / "profiles" / "dstrqc_anelastic_512x512x256_traffic.json"
     if (tfile.exists() and (args.nx, args.ny, args.nz) == (512, 512, 256) and args.storage == "f32"
-            and args.texture == "arrays"):
+            and args.texture == "arrays" and not args.fused):
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
@@ -362,13 +380,16 @@ def main():
                    "h_m": w.DH, "dt_s": w.DT, "attenuation": "Qs=50vs[km/s], Qp=2Qs, 8 mechanisms",
                    "medium": "layered vs 1500-4500 m/s + 5% von Karman (a=500 m, H=0.5), seed 7",
                    "texture": args.texture,
-                   "step": ("dvelcx_H_opt, " + ("dstrqc_H" if args.texture == "arrays" else
-                                                f"awp_dstrqc_tex ({args.texture})")
-                            + " (+FS ghosts), addsrc_H, fstr_H, receivers")
+                   "step": (("awp_step (velocity, FS ghosts and stress in one sweep)"
+                             if args.fused else
+                             "dvelcx_H_opt, " + ("dstrqc_H" if args.texture == "arrays" else
+                                                 f"awp_dstrqc_tex ({args.texture})")
+                             + " (+FS ghosts)") + ", addsrc_H, fstr_H, receivers")
                            + (f", free-surface volume output every {args.volume_output} steps"
                               if args.volume_output else "")},
         "roofline": {"bound": "hbm",
-                     "kernel": "dstrqc_H" if args.texture == "arrays" else "awp_dstrqc_tex",
+                     "kernel": ("awp_step" if args.fused else
+                                "dstrqc_H" if args.texture == "arrays" else "awp_dstrqc_tex"),
                      "achieved": str_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
                      "traffic": traffic,
@@ -379,11 +400,14 @@ def main():
                                         "run") if traffic is not None else None,
                      "algorithmic_bytes_per_cell": b_str,
                      "cells_per_launch": cells, "avg_launch_ms": str_avg},
-        "kernels": {"dvelcx_H_opt": {"avg_ms": vel_avg, "GBps": vel_gbs,
-                                     "frac": vel_gbs / HBM_PEAK_GBS},
-                    "dstrqc_H": {"avg_ms": str_avg, "GBps": str_gbs,
-                                 "frac": str_gbs / HBM_PEAK_GBS},
-                    "step_events_avg_ms": float(np.mean(step_ms))},
+        "kernels": ({"awp_step": {"avg_ms": str_avg, "GBps": str_gbs,
+                                  "frac": str_gbs / HBM_PEAK_GBS},
+                     "step_events_avg_ms": float(np.mean(step_ms))} if args.fused else
+                    {"dvelcx_H_opt": {"avg_ms": vel_avg, "GBps": vel_gbs,
+                                      "frac": vel_gbs / HBM_PEAK_GBS},
+                     "dstrqc_H" if args.texture == "arrays" else "awp_dstrqc_tex":
+                         {"avg_ms": str_avg, "GBps": str_gbs, "frac": str_gbs / HBM_PEAK_GBS},
+                     "step_events_avg_ms": float(np.mean(step_ms))}),
         "step_roofline": {"bytes_per_cell": b_vel + b_str, "achieved_GBps": step_gbs,
                           "frac": step_gbs / HBM_PEAK_GBS},
         "traces_finite": finite,

@@ -41,8 +41,9 @@ void awp_clear_error(void);
  * not depend on them.  Keys: vel_bj, str_bj (rows per workgroup: 1, 2, 4, 8,
  * 16), vel_target_wgs, str_target_wgs, vel_ichunk, str_ichunk (i-march
  * chunking, 0 = automatic), xcd_remap, kalign (0/1), topo_vel_ichunk,
- * topo_str_ichunk, topo_q_ichunk (planes per march, 1..4096).  Returns 0, or
- * -1 for an unknown key or a value out of range. */
+ * topo_str_ichunk, topo_q_ichunk (planes per march, 1..4096), step_ichunk
+ * (0 = automatic), step_target_wgs (awp_step's i-march chunking).  Returns 0,
+ * or -1 for an unknown key or a value out of range. */
 int awp_set_tuning(const char* key, int value);
 
 /* Replaces SetDeviceConstValue, include/awp/kernel_launch.h:24-26
@@ -167,6 +168,17 @@ void drprecpc_app_H(float* xx, float* yy, float* zz, float* xy, float* xz, float
  * array coordinates, writing out[step_slot*3*npts + 3*n + c]. */
 void awp_receivers_gather(const float* u1, const float* v1, const float* w1, const int* idx,
                           int npts, float* out, int step_slot, hipStream_t St, int d_i);
+
+/* dvelcx_H_opt followed by dvelc2_H in one launch (no reference
+ * counterpart; it combines kernel.cu:298-316 and :2289-2356): on a grid above
+ * a coarser one (d_i < ngrids-1) the lane at k = align+2, which the 4th-order
+ * update skips there, applies dvelc2's 2nd-order update from the operands
+ * the sweep loads anyway.  s_i, e_i must be the grid's interior (dvelc2_H's
+ * range); on the lowest grid it is dvelcx_H_opt.  Bit for bit the two
+ * launches' result. */
+void awp_dvelcx_dm(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz, float* xy,
+                   float* xz, float* yz, float* dcrjx, float* dcrjy, float* dcrjz, float* d_1,
+                   int nyt, int nzt, hipStream_t St, int s_i, int e_i, int d_i, int ngrids);
 
 /* Replaces dvelc2_H, include/awp/kernel_launch.h:92-95
  * (src/awp/kernel.cu:2289-2356).  Discontinuous mesh, fine grid d_i: velocity
@@ -328,6 +340,23 @@ void awp_dstrqc_tex(float* xx, float* yy, float* zz, float* xy, float* xz, float
                     float* qs, float* dcrjx, float* dcrjy, float* dcrjz, int nyt, int nzt,
                     hipStream_t St, float* lam_mu, const float* tex, int* ww, int NX, int NPC,
                     int rankx, int ranky, int s_i, int e_i, int s_j, int e_j, int d_i);
+
+/* The extended entry point: one NVE = 1 step of a single Cartesian grid
+ * (grid 0 of SetDeviceConstValue, ngrids = 1, one rank: NX = nxt, NPC = 0)
+ * in one sweep, equal bit for bit to dvelcx_H_opt over the interior
+ * (kernel.cu:298-316) followed by dstrqc_H over the interior (kernel.cu:
+ * 1201-1239, its free-surface ghosts included).  It reads the state before
+ * the step from a9 (u1 v1 w1 xx yy zz xy xz yz) and writes the state after
+ * it to b9 (same order; distinct arrays); r6 (r1..r6) are updated in place.
+ * b9 must hold a9's values in every cell the step does not write: give both
+ * the same initial state, and pass the arrays of the last b9 as the next a9
+ * (addsrc_H, fstr_H and the receivers act on b9 in between, as on the
+ * in-place arrays).  med = d1 lam mu qp qs; tex and ww are the relaxation
+ * texture as for awp_dstrqc_tex (ww NULL: init_texture's parity rule). */
+void awp_step(const float* const* a9, float* const* b9, float* const* r6,
+              const float* const* med, const float* coeff, const float* tex, const int* ww,
+              const float* dcrjx, const float* dcrjy, const float* dcrjz, const float* lam_mu,
+              int nxt, int nyt, int nzt, hipStream_t St);
 
 /* ---- The reference's structs at the topography and Lagrange boundary -----
  * The topography and Lagrange launchers take the reference's own structs by
