@@ -10,7 +10,7 @@
 
 // Defaults from sweeps on MI355X at 512x512x256: velocity 4 rows x 2 i-chunks
 // (all workgroups resident), stress 16 rows x ~32-plane i-chunks.
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 0, 2048};
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {

@@ -26,6 +26,8 @@
 // Every value is computed with the expression trees of dvelcx_H_opt,
 // dstrqc_H's free-surface ghosts and dstrqc_H (built with -ffp-contract=off),
 // so state B equals what dvelcx_H_opt + dstrqc_H leave in place, bit for bit.
+#include <type_traits>
+
 #include "awp_common.h"
 #include "awp_tuning.h"
 #include "../../include/awp_hip.h"
@@ -81,51 +83,77 @@ struct Fields {
       *__restrict__ d1;
 };
 
-__device__ __forceinline__ void vw_prime(VelWin& W, const Fields& F, long long p, long long S,
-                                         int Y) {
-  W.xx_m2 = F.xx[p - 2 * S]; W.xx_m1 = F.xx[p - S]; W.xx_0 = F.xx[p];
-  W.xy_m1 = F.xy[p - S]; W.xy_0 = F.xy[p]; W.xy_p1 = F.xy[p + S];
-  W.xz_m1 = F.xz[p - S]; W.xz_0 = F.xz[p]; W.xz_p1 = F.xz[p + S];
-  W.dc = F.d1[p]; W.djm = F.d1[p - Y]; W.dkm = F.d1[p - 1]; W.djk = F.d1[p - Y - 1];
+// Field access at (plane m + di, row j + dj, level k + dk) as a 32-bit byte
+// offset from the field's base: global_load v, v_off, s[base] with the base
+// a kernel argument in scalar registers.  The offsets of the stencil (di, dj)
+// shifts are the same for every field, so they are formed once per plane;
+// dk shifts go into the instruction's immediate.  The launcher checks that a
+// field spans less than 4 GiB.
+template <bool UNI>
+struct Acc {
+  unsigned b;       // byte offset of (m, j, k)
+  unsigned S4, Y4;  // plane and row strides in bytes
+  __device__ __forceinline__ Acc(int m, int j, int k, long long S, int Y)
+      : b((unsigned)(((long long)m * S + (long long)j * Y + k) * 4)), S4((unsigned)(S * 4)),
+        Y4((unsigned)Y * 4u) {}
+  __device__ __forceinline__ unsigned off(int di, int dj, int dk) const {
+    return b + (unsigned)di * S4 + (unsigned)dj * Y4 + (unsigned)dk * 4u;
+  }
+  __device__ __forceinline__ float operator()(const float* __restrict__ a, int di, int dj,
+                                              int dk) const {
+    return awp_ld(a, off(di, dj, dk));
+  }
+};
+
+template <bool UNI>
+__device__ __forceinline__ void vw_prime(VelWin& W, const Fields& F, const Acc<UNI>& X) {
+  W.xx_m2 = X(F.xx, -2, 0, 0); W.xx_m1 = X(F.xx, -1, 0, 0); W.xx_0 = X(F.xx, 0, 0, 0);
+  W.xy_m1 = X(F.xy, -1, 0, 0); W.xy_0 = X(F.xy, 0, 0, 0); W.xy_p1 = X(F.xy, 1, 0, 0);
+  W.xz_m1 = X(F.xz, -1, 0, 0); W.xz_0 = X(F.xz, 0, 0, 0); W.xz_p1 = X(F.xz, 1, 0, 0);
+  W.dc = X(F.d1, 0, 0, 0); W.djm = X(F.d1, 0, -1, 0); W.dkm = X(F.d1, 0, 0, -1);
+  W.djk = X(F.d1, 0, -1, -1);
 }
 
-// Velocities at p (plane m) when upd, else the stored (unchanging) values;
-// the window advances to plane m+1 either way.
-__device__ __forceinline__ void vw_step(VelWin& W, const Fields& F, long long p, long long S,
-                                        int Y, bool upd, float c1, float c2, float dth,
-                                        float dcrj, float& u, float& v, float& w) {
-  const float xx_p1 = F.xx[p + S];
-  const float xy_p2 = F.xy[p + 2 * S];
-  const float xz_p2 = F.xz[p + 2 * S];
-  const float ec = F.d1[p + S], ejm = F.d1[p + S - Y], ekm = F.d1[p + S - 1],
-              ejk = F.d1[p + S - Y - 1];
+// Velocities at plane m (X's plane) when upd, else the stored (unchanging)
+// values; the window advances to plane m+1 either way.
+template <bool UNI>
+__device__ __forceinline__ void vw_step(VelWin& W, const Fields& F, const Acc<UNI>& X, bool upd,
+                                        float c1, float c2, float dth, float dcrj, float& u,
+                                        float& v, float& w) {
+  const float xx_p1 = X(F.xx, 1, 0, 0);
+  const float xy_p2 = X(F.xy, 2, 0, 0);
+  const float xz_p2 = X(F.xz, 2, 0, 0);
+  const float ec = X(F.d1, 1, 0, 0), ejm = X(F.d1, 1, -1, 0), ekm = X(F.d1, 1, 0, -1),
+              ejk = X(F.d1, 1, -1, -1);
   if (upd) {
-    const float xy_jm1 = F.xy[p - Y], xy_jp1 = F.xy[p + Y], xy_jm2 = F.xy[p - 2 * Y];
-    const float xz_km1 = F.xz[p - 1], xz_kp1 = F.xz[p + 1], xz_km2 = F.xz[p - 2];
-    const float yy_0 = F.yy[p], yy_jp1 = F.yy[p + Y], yy_jp2 = F.yy[p + 2 * Y],
-                yy_jm1 = F.yy[p - Y];
-    const float yz_0 = F.yz[p], yz_km1 = F.yz[p - 1], yz_kp1 = F.yz[p + 1], yz_km2 = F.yz[p - 2];
-    const float yz_jm1 = F.yz[p - Y], yz_jp1 = F.yz[p + Y], yz_jm2 = F.yz[p - 2 * Y];
-    const float zz_0 = F.zz[p], zz_kp1 = F.zz[p + 1], zz_kp2 = F.zz[p + 2], zz_km1 = F.zz[p - 1];
+    const float xy_jm1 = X(F.xy, 0, -1, 0), xy_jp1 = X(F.xy, 0, 1, 0), xy_jm2 = X(F.xy, 0, -2, 0);
+    const float xz_km1 = X(F.xz, 0, 0, -1), xz_kp1 = X(F.xz, 0, 0, 1), xz_km2 = X(F.xz, 0, 0, -2);
+    const float yy_0 = X(F.yy, 0, 0, 0), yy_jp1 = X(F.yy, 0, 1, 0), yy_jp2 = X(F.yy, 0, 2, 0),
+                yy_jm1 = X(F.yy, 0, -1, 0);
+    const float yz_0 = X(F.yz, 0, 0, 0), yz_km1 = X(F.yz, 0, 0, -1), yz_kp1 = X(F.yz, 0, 0, 1),
+                yz_km2 = X(F.yz, 0, 0, -2);
+    const float yz_jm1 = X(F.yz, 0, -1, 0), yz_jp1 = X(F.yz, 0, 1, 0), yz_jm2 = X(F.yz, 0, -2, 0);
+    const float zz_0 = X(F.zz, 0, 0, 0), zz_kp1 = X(F.zz, 0, 0, 1), zz_kp2 = X(F.zz, 0, 0, 2),
+                zz_km1 = X(F.zz, 0, 0, -1);
     float f_d1 = 0.25f * (W.dc + W.djm + W.dkm + W.djk);
     float f_d2 = 0.25f * (W.dc + ec + W.dkm + ekm);
     float f_d3 = 0.25f * (W.dc + ec + W.djm + ejm);
     f_d1 = dth / f_d1;
     f_d2 = dth / f_d2;
     f_d3 = dth / f_d3;
-    u = (F.u[p] + f_d1 * (c1 * (W.xx_0 - W.xx_m1) + c2 * (xx_p1 - W.xx_m2) +
-                          c1 * (W.xy_0 - xy_jm1) + c2 * (xy_jp1 - xy_jm2) +
-                          c1 * (W.xz_0 - xz_km1) + c2 * (xz_kp1 - xz_km2))) * dcrj;
-    v = (F.v[p] + f_d2 * (c1 * (W.xy_p1 - W.xy_0) + c2 * (xy_p2 - W.xy_m1) +
-                          c1 * (yy_jp1 - yy_0) + c2 * (yy_jp2 - yy_jm1) + c1 * (yz_0 - yz_km1) +
-                          c2 * (yz_kp1 - yz_km2))) * dcrj;
-    w = (F.w[p] + f_d3 * (c1 * (W.xz_p1 - W.xz_0) + c2 * (xz_p2 - W.xz_m1) +
-                          c1 * (yz_0 - yz_jm1) + c2 * (yz_jp1 - yz_jm2) + c1 * (zz_kp1 - zz_0) +
-                          c2 * (zz_kp2 - zz_km1))) * dcrj;
+    u = (X(F.u, 0, 0, 0) + f_d1 * (c1 * (W.xx_0 - W.xx_m1) + c2 * (xx_p1 - W.xx_m2) +
+                                   c1 * (W.xy_0 - xy_jm1) + c2 * (xy_jp1 - xy_jm2) +
+                                   c1 * (W.xz_0 - xz_km1) + c2 * (xz_kp1 - xz_km2))) * dcrj;
+    v = (X(F.v, 0, 0, 0) + f_d2 * (c1 * (W.xy_p1 - W.xy_0) + c2 * (xy_p2 - W.xy_m1) +
+                                   c1 * (yy_jp1 - yy_0) + c2 * (yy_jp2 - yy_jm1) +
+                                   c1 * (yz_0 - yz_km1) + c2 * (yz_kp1 - yz_km2))) * dcrj;
+    w = (X(F.w, 0, 0, 0) + f_d3 * (c1 * (W.xz_p1 - W.xz_0) + c2 * (xz_p2 - W.xz_m1) +
+                                   c1 * (yz_0 - yz_jm1) + c2 * (yz_jp1 - yz_jm2) +
+                                   c1 * (zz_kp1 - zz_0) + c2 * (zz_kp2 - zz_km1))) * dcrj;
   } else {
-    u = F.u[p];
-    v = F.v[p];
-    w = F.w[p];
+    u = X(F.u, 0, 0, 0);
+    v = X(F.v, 0, 0, 0);
+    w = X(F.w, 0, 0, 0);
   }
   W.xx_m2 = W.xx_m1; W.xx_m1 = W.xx_0; W.xx_0 = xx_p1;
   W.xy_m1 = W.xy_0; W.xy_0 = W.xy_p1; W.xy_p1 = xy_p2;
@@ -206,39 +234,42 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
   // One velocity item at plane m: the update where the item is in the
   // velocity region (and m in [xls, xre]), else the stored value; into the
   // LDS ring at (row r, level c), and returned.
-  auto vel_item = [&](VelWin& Wn, bool need, bool sp, long long o, float dy, float dz, int r,
-                      int c, int m, float& u, float& v, float& w) {
+  auto vel_item = [&](auto uni, VelWin& Wn, bool need, bool sp, int jj, int kk, float dy,
+                      float dz, int r, int c, int m, float& u, float& v, float& w) {
+    constexpr bool UNI = decltype(uni)::value;
     if (!need) return;
-    const long long p = (long long)m * S + o;
+    const Acc<UNI> X(m, jj, kk, S, Y);
     if (sp) {
       const bool in_x = m >= L.xls && m <= L.xre;
       const float dcrj = A.dcrjx[m] * dy * dz;
-      vw_step(Wn, F, p, S, Y, in_x, c1, c2, dth, dcrj, u, v, w);
+      vw_step<UNI>(Wn, F, X, in_x, c1, c2, dth, dcrj, u, v, w);
     } else {
-      u = F.u[p]; v = F.v[p]; w = F.w[p];
+      u = X(F.u, 0, 0, 0); v = X(F.v, 0, 0, 0); w = X(F.w, 0, 0, 0);
     }
     const int slot = m & (FSL - 1);
     vl[lidx(slot, 0, r, c)] = u;
     vl[lidx(slot, 1, r, c)] = v;
     vl[lidx(slot, 2, r, c)] = w;
   };
+  using UniT = std::integral_constant<bool, true>;
+  using LaneT = std::integral_constant<bool, false>;
 
   if (!core) {
     // halo waves: their row item and (lanes 0..FBJ-1) one level column of
     // the core rows, planes ia .. ib+2; the same barriers as the core waves
     VelWin W0, W1;
-    if (sp0) vw_prime(W0, F, (long long)ia * S + o0, S, Y);
-    if (sp1) vw_prime(W1, F, (long long)ia * S + o1, S, Y);
+    if (sp0) vw_prime<true>(W0, F, Acc<true>(ia, j, k, S, Y));
+    if (sp1) vw_prime<false>(W1, F, Acc<false>(ia, j1, k1, S, Y));
     float u, v, w;
     for (int m = ia; m <= ia + 1; ++m) {
-      vel_item(W0, need0, sp0, o0, dy0, dz0, r0, c0, m, u, v, w);
-      vel_item(W1, need1, sp1, o1, dy1, dz1, r1, hc, m, u, v, w);
+      vel_item(UniT(), W0, need0, sp0, j, k, dy0, dz0, r0, c0, m, u, v, w);
+      vel_item(LaneT(), W1, need1, sp1, j1, k1, dy1, dz1, r1, hc, m, u, v, w);
     }
     __syncthreads();
     for (int s = 0; s < nsteps; ++s) {
       const int m = __builtin_amdgcn_readfirstlane(ia + s + 2);
-      vel_item(W0, need0, sp0, o0, dy0, dz0, r0, c0, m, u, v, w);
-      vel_item(W1, need1, sp1, o1, dy1, dz1, r1, hc, m, u, v, w);
+      vel_item(UniT(), W0, need0, sp0, j, k, dy0, dz0, r0, c0, m, u, v, w);
+      vel_item(LaneT(), W1, need1, sp1, j1, k1, dy1, dz1, r1, hc, m, u, v, w);
       __syncthreads();
     }
     return;
@@ -247,13 +278,13 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
   // core waves: velocity of the own column, planes ia-2 .. ib+2, and the
   // stress of planes ia .. ib
   VelWin W0;
-  if (sp0) vw_prime(W0, F, (long long)(ia - 2) * S + o0, S, Y);
+  if (sp0) vw_prime<true>(W0, F, Acc<true>(ia - 2, j, k, S, Y));
   // the core row's velocities at plane m; owned cells are written to B
   auto vel_core = [&](int m, float& u, float& v, float& w) {
-    vel_item(W0, need0, sp0, o0, dy0, dz0, r0, c0, m, u, v, w);
+    vel_item(UniT(), W0, need0, sp0, j, k, dy0, dz0, r0, c0, m, u, v, w);
     if (sp0 && m >= ia && m <= ib) {
-      const long long p = (long long)m * S + o0;
-      A.b[0][p] = u; A.b[1][p] = v; A.b[2][p] = w;
+      const unsigned ov = Acc<true>(m, j, k, S, Y).off(0, 0, 0);
+      awp_st(A.b[0], ov, u); awp_st(A.b[1], ov, v); awp_st(A.b[2], ov, w);
     }
   };
 
@@ -279,15 +310,30 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
   float lc = 0.f, ljm = 0.f, lkm = 0.f, ljk = 0.f, mc = 0.f, mjm = 0.f, mkm = 0.f, mjk = 0.f;
   float pc = 0.f, pjm = 0.f, pkm = 0.f, pjk = 0.f, sc = 0.f, sjm = 0.f, skm = 0.f, sjk = 0.f;
   if (sact) {
-    const long long p0 = (long long)ia * S + so;
-    lc = A.lam[p0]; ljm = A.lam[p0 - Y]; lkm = A.lam[p0 - 1]; ljk = A.lam[p0 - Y - 1];
-    mc = A.mu[p0]; mjm = A.mu[p0 - Y]; mkm = A.mu[p0 - 1]; mjk = A.mu[p0 - Y - 1];
-    pc = A.qp[p0]; pjm = A.qp[p0 - Y]; pkm = A.qp[p0 - 1]; pjk = A.qp[p0 - Y - 1];
-    sc = A.qs[p0]; sjm = A.qs[p0 - Y]; skm = A.qs[p0 - 1]; sjk = A.qs[p0 - Y - 1];
+    const Acc<true> X(ia, j, k, S, Y);
+    lc = X(A.lam, 0, 0, 0); ljm = X(A.lam, 0, -1, 0); lkm = X(A.lam, 0, 0, -1);
+    ljk = X(A.lam, 0, -1, -1);
+    mc = X(A.mu, 0, 0, 0); mjm = X(A.mu, 0, -1, 0); mkm = X(A.mu, 0, 0, -1);
+    mjk = X(A.mu, 0, -1, -1);
+    pc = X(A.qp, 0, 0, 0); pjm = X(A.qp, 0, -1, 0); pkm = X(A.qp, 0, 0, -1);
+    pjk = X(A.qp, 0, -1, -1);
+    sc = X(A.qs, 0, 0, 0); sjm = X(A.qs, 0, -1, 0); skm = X(A.qs, 0, 0, -1);
+    sjk = X(A.qs, 0, -1, -1);
   }
 
   for (int s = 0; s < nsteps; ++s) {
     const int i = __builtin_amdgcn_readfirstlane(ia + s);
+    // the stress operands that do not depend on the new velocities are
+    // loaded before the velocity update, so both loads overlap
+    float o_s[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, o_r[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (sact) {
+      const Acc<true> X(i, j, k, S, Y);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        o_s[c] = X(A.a[3 + c], 0, 0, 0);
+        o_r[c] = X(A.r[c], 0, 0, 0);
+      }
+    }
     float u_p2 = 0.f, v_n = 0.f, w_n = 0.f;
     vel_core(i + 2, u_p2, v_n, w_n);
     __syncthreads();
@@ -314,20 +360,20 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
       vl[lidx(slot, 0, r0, c0 + 1)] = ug;
       vl[lidx(slot, 1, r0, c0 + 1)] = vg;
       vl[lidx(slot, 2, r0, c0 + 1)] = wg;
-      const long long p = pl + so + 1;
-      A.b[0][p] = ug; A.b[1][p] = vg; A.b[2][p] = wg;
+      const unsigned og = Acc<true>(i, j, k + 1, S, Y).off(0, 0, 0);
+      awp_st(A.b[0], og, ug); awp_st(A.b[1], og, vg); awp_st(A.b[2], og, wg);
     }
     if (sact) {
-      const long long pn = pl + S;
-      const long long p = pl + so;
-      const float Lc = A.lam[pn + so], Ljm = A.lam[pn + so - Y], Lkm = A.lam[pn + so - 1],
-                  Ljk = A.lam[pn + so - Y - 1];
-      const float Mc = A.mu[pn + so], Mjm = A.mu[pn + so - Y], Mkm = A.mu[pn + so - 1],
-                  Mjk = A.mu[pn + so - Y - 1];
-      const float Pc = A.qp[pn + so], Pjm = A.qp[pn + so - Y], Pkm = A.qp[pn + so - 1],
-                  Pjk = A.qp[pn + so - Y - 1];
-      const float Sc = A.qs[pn + so], Sjm = A.qs[pn + so - Y], Skm = A.qs[pn + so - 1],
-                  Sjk = A.qs[pn + so - Y - 1];
+      const Acc<true> X(i, j, k, S, Y);
+      const unsigned ob = X.off(0, 0, 0);
+      const float Lc = X(A.lam, 1, 0, 0), Ljm = X(A.lam, 1, -1, 0), Lkm = X(A.lam, 1, 0, -1),
+                  Ljk = X(A.lam, 1, -1, -1);
+      const float Mc = X(A.mu, 1, 0, 0), Mjm = X(A.mu, 1, -1, 0), Mkm = X(A.mu, 1, 0, -1),
+                  Mjk = X(A.mu, 1, -1, -1);
+      const float Pc = X(A.qp, 1, 0, 0), Pjm = X(A.qp, 1, -1, 0), Pkm = X(A.qp, 1, 0, -1),
+                  Pjk = X(A.qp, 1, -1, -1);
+      const float Sc = X(A.qs, 1, 0, 0), Sjm = X(A.qs, 1, -1, 0), Skm = X(A.qs, 1, 0, -1),
+                  Sjk = X(A.qs, 1, -1, -1);
       // in-plane velocity neighbours from the ring
       const float u_jp1 = vl[lidx(slot, 0, r0 + 1, c0)], u_jp2 = vl[lidx(slot, 0, r0 + 2, c0)],
                   u_jm1 = vl[lidx(slot, 0, r0 - 1, c0)];
@@ -348,7 +394,7 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
         f_ww = 1 + 4 * ((ni + 1) & 1) + 2 * ((col + 1) & 1) +
                ((L.tex_nzt + col * L.tex_nzt + (k - AWP_ALIGN)) & 1);
       } else {
-        f_ww = A.ww[p];
+        f_ww = awp_ldi(A.ww, ob);
       }
       const int tt = (f_ww - 1) & 7;
       const float f_vx1 = s_tex[tt], f_vx2 = s_tex[8 + tt], f_wwo = s_tex[16 + tt];
@@ -407,53 +453,53 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
       const float one_m_vx1 = 1 - f_vx1;
       const float wwo_m1 = f_wwo - 1;
 
-      f_r = A.r[0][p];
+      f_r = o_r[0];
       f_rtmp = -h * (vs2 + vs3) + a1;
-      t = A.a[3][p] + tmp - xm * (vs2 + vs3) + vx1p * f_r;
-      A.r[0][p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      t = o_s[0] + tmp - xm * (vs2 + vs3) + vx1p * f_r;
+      awp_st(A.r[0], ob, f_vx2 * f_r + f_wwo * f_rtmp);
       f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-      A.b[3][p] = (t + DT * f_rtmp) * f_dcrj;
+      awp_st(A.b[3], ob, (t + DT * f_rtmp) * f_dcrj);
 
-      f_r = A.r[1][p];
+      f_r = o_r[1];
       f_rtmp = -h * (vs1 + vs3) + a1;
-      t = (A.a[4][p] + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
-      A.r[1][p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      t = (o_s[1] + tmp - xm * (vs1 + vs3) + vx1p * f_r) * f_dcrj;
+      awp_st(A.r[1], ob, f_vx2 * f_r + f_wwo * f_rtmp);
       f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-      A.b[4][p] = (t + DT * f_rtmp) * f_dcrj;
+      awp_st(A.b[4], ob, (t + DT * f_rtmp) * f_dcrj);
 
-      f_r = A.r[2][p];
+      f_r = o_r[2];
       f_rtmp = -h * (vs1 + vs2) + a1;
-      t = (A.a[5][p] + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
-      A.r[2][p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      t = (o_s[2] + tmp - xm * (vs1 + vs2) + vx1p * f_r) * f_dcrj;
+      awp_st(A.r[2], ob, f_vx2 * f_r + f_wwo * f_rtmp);
       f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-      A.b[5][p] = (t + DT * f_rtmp) * f_dcrj;
+      awp_st(A.b[5], ob, (t + DT * f_rtmp) * f_dcrj);
 
       vs1 = c1 * (u_jp1 - u_0) + c2 * (u_jp2 - u_jm1);
       vs2 = c1 * (v_0 - v_m1) + c2 * (v_p1 - v_m2);
-      f_r = A.r[3][p];
+      f_r = o_r[3];
       f_rtmp = h1 * (vs1 + vs2);
-      t = A.a[6][p] + xmu1 * (vs1 + vs2) + vx1p * f_r;
-      A.r[3][p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      t = o_s[3] + xmu1 * (vs1 + vs2) + vx1p * f_r;
+      awp_st(A.r[3], ob, f_vx2 * f_r + f_wwo * f_rtmp);
       f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-      A.b[6][p] = (t + DT * f_rtmp) * f_dcrj;
+      awp_st(A.b[6], ob, (t + DT * f_rtmp) * f_dcrj);
 
       vs1 = c1 * (u_kp1 - u_0) + c2 * (u_kp2 - u_km1);
       vs2 = c1 * (w_0 - w_m1) + c2 * (w_p1 - w_m2);
-      f_r = A.r[4][p];
+      f_r = o_r[4];
       f_rtmp = h2 * (vs1 + vs2);
-      t = A.a[7][p] + xmu2 * (vs1 + vs2) + vx1p * f_r;
-      A.r[4][p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      t = o_s[4] + xmu2 * (vs1 + vs2) + vx1p * f_r;
+      awp_st(A.r[4], ob, f_vx2 * f_r + f_wwo * f_rtmp);
       f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-      A.b[7][p] = (t + DT * f_rtmp) * f_dcrj;
+      awp_st(A.b[7], ob, (t + DT * f_rtmp) * f_dcrj);
 
       vs1 = c1 * (v_kp1 - v_0) + c2 * (v_kp2 - v_km1);
       vs2 = c1 * (w_jp1 - w_0) + c2 * (w_jp2 - w_jm1);
-      f_r = A.r[5][p];
+      f_r = o_r[5];
       f_rtmp = h3 * (vs1 + vs2);
-      t = A.a[8][p] + xmu3 * (vs1 + vs2) + vx1p * f_r;
-      A.r[5][p] = f_vx2 * f_r + f_wwo * f_rtmp;
+      t = o_s[5] + xmu3 * (vs1 + vs2) + vx1p * f_r;
+      awp_st(A.r[5], ob, f_vx2 * f_r + f_wwo * f_rtmp);
       f_rtmp = f_rtmp * wwo_m1 + f_vx2 * f_r * one_m_vx1;
-      A.b[8][p] = (t + DT * f_rtmp) * f_dcrj;
+      awp_st(A.b[8], ob, (t + DT * f_rtmp) * f_dcrj);
 
       lc = Lc; ljm = Ljm; lkm = Lkm; ljk = Ljk;
       mc = Mc; mjm = Mjm; mkm = Mkm; mjk = Mjk;
@@ -506,6 +552,12 @@ void awp_step(const float* const* a9, float* const* b9, float* const* r6,
   A.coeff = coeff; A.dcrjx = dcrjx; A.dcrjy = dcrjy; A.dcrjz = dcrjz; A.lam_mu = lam_mu;
   A.ww = ww;
 
+  // 32-bit byte offsets inside a field
+  if ((long long)(nxt + 4 + AWP_NGSL2) * g.slice * 4 >= (1ll << 32)) {
+    awp_set_error(AWP_ERR_ARG, "%s: a field of %d x %d x %d cells spans 4 GiB or more", W, nxt,
+                  nyt, nzt);
+    return;
+  }
   StepLaunch L;
   L.xls = 2 + AWP_NGSL; L.xre = nxt + AWP_NGSL + 1;
   L.yls = 2 + AWP_NGSL; L.yre = nyt + AWP_NGSL + 1;
