@@ -55,6 +55,7 @@ SIGNATURES: dict[str, list] = {
     "awp_s16_rescale": [_fp, C.c_longlong, _i, _i, _s],
     "awp_s16_absmax": [_fp, C.c_longlong, _i, _fp, _s],
     "awp_adjoint_step": [_fp] * 13 + [_s],
+    "awp_adjoint_settle": [_fp, _fp, _s],
     "awp_adjoint_inject": [_fp] * 4 + [_i, _fp, _s],
     "awp_adjoint_flush": [_fp, _fp, C.c_longlong, _s],
     "awp_misfit": [_fp, _fp, C.c_longlong, _fp, _fp, _fp, _s],

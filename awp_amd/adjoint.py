@@ -40,7 +40,8 @@ class AdjointProblem:
     """One grid, NVE = 1: forward with checkpoints, misfit, adjoint sweep."""
 
     def __init__(self, medium: Medium, sources, receivers, nt: int, observed: np.ndarray,
-                 checkpoint_every: int = 50, device: str = "cuda"):
+                 checkpoint_every: int = 50, device: str = "cuda",
+                 resident_steps: int | None = None, hbm_reserve_gib: float = 16.0):
         self.solver = Solver(medium, sources, receivers, nt_record=nt, device=device)
         if self.solver.rec is None:
             raise ValueError("the misfit needs receivers")
@@ -85,8 +86,27 @@ class AdjointProblem:
                   & (kk <= d.ktop))
             if not ok.all():
                 raise ValueError("adjoint sources must lie in the stress update region")
-        # ring of K+1 velocity + stress states (slot 0 holds the checkpoint)
-        self.ring = [{x: torch.zeros(sh, **f32) for x in VEL + STRESS} for _ in range(self.K + 1)]
+        # Velocity + stress states in HBM.  The last segments of the run keep
+        # every state from the forward pass (slots[n - n_tail] for step n >=
+        # n_tail), so the backward sweep recomputes only the segments before
+        # n_tail; once those resident segments are done, slots[0..K] serve
+        # as the recompute ring of the others.  resident_steps bounds the
+        # tail (None: as many whole segments as the free HBM holds, less
+        # hbm_reserve_gib; 0: recompute every segment).
+        slot_bytes = len(VEL + STRESS) * int(np.prod(sh)) * 4
+        if resident_steps is None:
+            free, _ = torch.cuda.mem_get_info(self.dev)
+            avail = free - int(hbm_reserve_gib * 2**30)
+            resident_steps = max(0, avail // slot_bytes - 1)
+        nseg = (self.nt + self.K - 1) // self.K
+        tail_segs = min(nseg, max(0, int(resident_steps)) // self.K)
+        # n_tail: the first step of the resident segments (a segment start)
+        self.n_tail = (nseg - tail_segs) * self.K if tail_segs < nseg else 0
+        if tail_segs == 0:
+            self.n_tail = self.nt + 1  # nothing resident
+        nslots = max(self.K + 1, self.nt - self.n_tail + 1 if tail_segs else 0)
+        self.slots = [{x: torch.zeros(sh, **f32) for x in VEL + STRESS} for _ in range(nslots)]
+        self.ring = self.slots[:self.K + 1]
 
     # -- medium --
     def set_medium(self, lam, mu, rho) -> None:
@@ -129,11 +149,29 @@ class AdjointProblem:
         self._reset()
         self.checkpoints = {}
         s.start()
-        self._save(0)
-        for n in range(1, self.nt + 1):
+        if self.n_tail > 0:
+            self._save(0)
+        for n in range(1, min(self.n_tail, self.nt + 1)):
             s.step()
-            if n % self.K == 0 and n < self.nt:
+            if n % self.K == 0 and n < min(self.n_tail, self.nt):
                 self._save(n)
+        if self.n_tail <= self.nt:
+            # the resident tail: the state at n_tail into slot 0, then every
+            # step out of place into the next slot, recording the receivers
+            # from it
+            if self.n_tail > 0:
+                s.step()
+            for x in VEL + STRESS:
+                self.slots[0][x].copy_(s.f[x])
+            rec = s.rec
+            for n in range(self.n_tail + 1, self.nt + 1):
+                fout = self.slots[n - self.n_tail]
+                self._recompute_step(self.slots[n - 1 - self.n_tail], fout)
+                if n - 1 < rec["out"].shape[0]:
+                    self.lib.awp_receivers_gather(fout["u1"].data_ptr(), fout["v1"].data_ptr(),
+                                                  fout["w1"].data_ptr(), rec["idx"].data_ptr(),
+                                                  rec["n"], rec["out"].data_ptr(), n - 1,
+                                                  s.stream, 0)
         self.lib.awp_misfit(s.rec["out"].data_ptr(), self.obs.data_ptr(), self.obs.numel(),
                             self.res.data_ptr(), self.J.data_ptr(), self.work.data_ptr(),
                             s.stream)
@@ -193,22 +231,28 @@ class AdjointProblem:
         g32 = _pp(self.g32)
         rec = s.rec
         npts = rec["n"]
-        starts = sorted(self.checkpoints)
+        starts = sorted(set(self.checkpoints) |
+                        set(range(max(self.n_tail, 0), self.nt, self.K)))
         for n0 in reversed(starts):
             n1 = min(n0 + self.K, self.nt)
-            # recompute the segment from its checkpoint into the ring
-            for x, t in zip(FIELDS, self.checkpoints[n0]):
-                (self.ring[0][x] if x in self.ring[0] else s.f[x]).copy_(t)
-            s.cur_step = n0
-            for n in range(n0 + 1, n1 + 1):
-                self._recompute_step(self.ring[n - 1 - n0], self.ring[n - n0])
+            if n0 >= self.n_tail:
+                # resident from the forward pass
+                ring = self.slots[n0 - self.n_tail:n1 - self.n_tail + 1]
+            else:
+                # recompute the segment from its checkpoint into the ring
+                ring = self.ring
+                for x, t in zip(FIELDS, self.checkpoints[n0]):
+                    (ring[0][x] if x in ring[0] else s.f[x]).copy_(t)
+                s.cur_step = n0
+                for n in range(n0 + 1, n1 + 1):
+                    self._recompute_step(ring[n - 1 - n0], ring[n - n0])
             for n in range(n1, n0, -1):
                 res_n = self.res[n - 1]
                 lib.awp_adjoint_inject(self.adj["u1"].data_ptr(), self.adj["v1"].data_ptr(),
                                        self.adj["w1"].data_ptr(), rec["idx"].data_ptr(), npts,
                                        res_n.data_ptr(), st)
-                fv = _pp([self.ring[n - n0][x] for x in VEL])
-                fs = _pp([self.ring[n - 1 - n0][x] for x in STRESS])
+                fv = _pp([ring[n - n0][x] for x in VEL])
+                fs = _pp([ring[n - 1 - n0][x] for x in STRESS])
                 lib.awp_adjoint_step(adj, fv, fs, med, m["ww"].data_ptr(), m["coeff"].data_ptr(),
                                      m["dcrjx"].data_ptr(), m["dcrjy"].data_ptr(),
                                      m["dcrjz"].data_ptr(), scratch, g32,
@@ -217,6 +261,8 @@ class AdjointProblem:
                 lib.awp_adjoint_flush(a.data_ptr(), b.data_ptr(), a.numel(), st)
             lib.awp_adjoint_flush(self.gl32.data_ptr(), self.gl64.data_ptr(), self.gl32.numel(), st)
             _lib.check()
+        # the adjoint state at step 0 complete (the last step's pending gather)
+        lib.awp_adjoint_settle(adj, scratch, st)
         sh = self.dims.shape
         f64 = dict(dtype=torch.float64, device=self.dev)
         glam, gmu, grho = (torch.zeros(sh, **f64) for _ in range(3))

@@ -102,13 +102,33 @@ struct Medium {
   const int* ww;
 };
 
+// The velocity^T gather at stress cell q: the six stress adjoints'
+// increments from the weighted velocity adjoints Aw (u, v, w).
+__device__ __forceinline__ void vel_gather6(const CPtr6& Aw, long long q, const AdjGrid& G,
+                                            float* g) {
+  const long long S = G.slice, Y = G.yline;
+  const float c1 = G.c1, c2 = G.c2;
+  const float *Au = Aw.f[0], *Av = Aw.f[1], *Aw_ = Aw.f[2];
+  g[0] = c1 * (Au[q] - Au[q + S]) + c2 * (Au[q - S] - Au[q + 2 * S]);
+  g[1] = c1 * (Av[q - Y] - Av[q]) + c2 * (Av[q - 2 * Y] - Av[q + Y]);
+  g[2] = c1 * (Aw_[q - 1] - Aw_[q]) + c2 * (Aw_[q - 2] - Aw_[q + 1]);
+  g[3] = c1 * (Au[q] - Au[q + Y]) + c2 * (Au[q - Y] - Au[q + 2 * Y]) +
+         c1 * (Av[q - S] - Av[q]) + c2 * (Av[q - 2 * S] - Av[q + S]);
+  g[4] = c1 * (Au[q] - Au[q + 1]) + c2 * (Au[q - 1] - Au[q + 2]) +
+         c1 * (Aw_[q - S] - Aw_[q]) + c2 * (Aw_[q - 2 * S] - Aw_[q + S]);
+  g[5] = c1 * (Av[q] - Av[q + 1]) + c2 * (Av[q - 1] - Av[q + 2]) +
+         c1 * (Aw_[q] - Aw_[q + Y]) + c2 * (Aw_[q - Y] - Aw_[q + 2 * Y]);
+}
+
 // stress^T, local part.  Per stress cell: the adjoint of the old stress and
 // memory variables, the adjoints of the six strain combinations (N1..N3 for
 // the normal strains, E4..E6 for the shear pairs) and the coefficient
 // gradients (strains of the forward velocity after the step).
+// Below the free-surface band (k <= top-3) the previous step's pending
+// velocity^T gather (weights Awp) is added to the stress adjoints first.
 __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es, CPtr3 V,
                                                                Medium M, Grad8 Gr,
-                                                               const AdjGrid G) {
+                                                               const AdjGrid G, CPtr6 Awp) {
   const Blk3 B = adj_block();
   const int k = AWP_ALIGN + 3 + (int)(B.x * 64 + threadIdx.x);
   const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
@@ -176,10 +196,13 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
   // normal components: sigma' = P (sigma + (XL + DT QPA) S - XM O + vx1p r)
   //                              + D DT ((wwo-1) frt + keep r),  P = D (xx), D^2 (yy, zz)
   //                     r' = vx2 r + wwo frt,  frt = -H O + QPA S
+  float pend[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  const bool has_pend = k <= G.top - 3;
+  if (has_pend) vel_gather6(Awp, p, G, pend);
   float Sbar = 0.0f, Obar[3], gxl = 0.0f, gxm = 0.0f;
   for (int c = 0; c < 3; ++c) {
     const float P = (c == 0) ? dcrj : dcrj * dcrj;
-    const float sb = A.f[c][p];
+    const float sb = has_pend ? A.f[c][p] + pend[c] : A.f[c][p];
     const float rb = A.f[6 + c][p];
     const float frtb = dcrj * DT * (f_wwo - 1) * sb + f_wwo * rb;
     A.f[c][p] = P * sb;
@@ -191,7 +214,7 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
   }
   float gmu[3];
   for (int c = 0; c < 3; ++c) {
-    const float sb = A.f[3 + c][p];
+    const float sb = has_pend ? A.f[3 + c][p] + pend[3 + c] : A.f[3 + c][p];
     const float rb = A.f[9 + c][p];
     const float frtb = dcrj * DT * (f_wwo - 1) * sb + f_wwo * rb;
     A.f[3 + c][p] = dcrj * sb;
@@ -211,29 +234,44 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
 
 // stress^T, gather part: velocity adjoints from the strain adjoints of the
 // stress cells whose stencils read them (interior columns, k up to top+1).
+// The strain -> velocity gather at cell c (the velocity adjoints' increments).
+struct Vec3 {
+  float u, v, w;
+};
+__device__ __forceinline__ Vec3 strain_gather(const CPtr6& E, long long c, long long S,
+                                              long long Y, float c1, float c2) {
+  const float *N1 = E.f[0], *N2 = E.f[1], *N3 = E.f[2], *E4 = E.f[3], *E5 = E.f[4],
+              *E6 = E.f[5];
+  Vec3 r;
+  r.u = c1 * (N1[c - S] - N1[c]) + c2 * (N1[c - 2 * S] - N1[c + S]) +
+        c1 * (E4[c - Y] - E4[c]) + c2 * (E4[c - 2 * Y] - E4[c + Y]) +
+        c1 * (E5[c - 1] - E5[c]) + c2 * (E5[c - 2] - E5[c + 1]);
+  r.v = c1 * (N2[c] - N2[c + Y]) + c2 * (N2[c - Y] - N2[c + 2 * Y]) +
+        c1 * (E4[c] - E4[c + S]) + c2 * (E4[c - S] - E4[c + 2 * S]) +
+        c1 * (E6[c - 1] - E6[c]) + c2 * (E6[c - 2] - E6[c + 1]);
+  r.w = c1 * (N3[c] - N3[c + 1]) + c2 * (N3[c - 1] - N3[c + 2]) +
+        c1 * (E5[c] - E5[c + S]) + c2 * (E5[c - S] - E5[c + 2 * S]) +
+        c1 * (E6[c - Y] - E6[c]) + c2 * (E6[c - 2 * Y] - E6[c + Y]);
+  return r;
+}
+
+// stress^T, gather part on levels [k0, k1] (interior columns).
 __global__ __launch_bounds__(256) void adj_strain_gather_kernel(float* __restrict__ ub,
                                                                 float* __restrict__ vb,
                                                                 float* __restrict__ wb,
-                                                                CPtr6 E, const AdjGrid G) {
+                                                                CPtr6 E, const AdjGrid G,
+                                                                int k0, int k1) {
   const Blk3 B = adj_block();
-  const int k = AWP_ALIGN + (int)(B.x * 64 + threadIdx.x);
+  const int k = k0 + (int)(B.x * 64 + threadIdx.x);
   const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
   const int i = 2 + AWP_NGSL + B.z;
-  if (k > G.top + 1 || j >= G.ny + 2 + AWP_NGSL) return;
+  if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
   const long long S = G.slice, Y = G.yline;
   const long long c = aidx(G, i, j, k);
-  const float c1 = G.c1, c2 = G.c2;
-  const float *N1 = E.f[0], *N2 = E.f[1], *N3 = E.f[2], *E4 = E.f[3], *E5 = E.f[4],
-              *E6 = E.f[5];
-  ub[c] += c1 * (N1[c - S] - N1[c]) + c2 * (N1[c - 2 * S] - N1[c + S]) +
-           c1 * (E4[c - Y] - E4[c]) + c2 * (E4[c - 2 * Y] - E4[c + Y]) +
-           c1 * (E5[c - 1] - E5[c]) + c2 * (E5[c - 2] - E5[c + 1]);
-  vb[c] += c1 * (N2[c] - N2[c + Y]) + c2 * (N2[c - Y] - N2[c + 2 * Y]) +
-           c1 * (E4[c] - E4[c + S]) + c2 * (E4[c - S] - E4[c + 2 * S]) +
-           c1 * (E6[c - 1] - E6[c]) + c2 * (E6[c - 2] - E6[c + 1]);
-  wb[c] += c1 * (N3[c] - N3[c + 1]) + c2 * (N3[c - 1] - N3[c + 2]) +
-           c1 * (E5[c] - E5[c + S]) + c2 * (E5[c - S] - E5[c + 2 * S]) +
-           c1 * (E6[c - Y] - E6[c]) + c2 * (E6[c - 2 * Y] - E6[c + Y]);
+  const Vec3 g = strain_gather(E, c, S, Y, G.c1, G.c2);
+  ub[c] += g.u;
+  vb[c] += g.v;
+  wb[c] += g.w;
 }
 
 // ghost^T, part 1 (per column): the adjoints of ug, vg and of the bracket of
@@ -316,16 +354,21 @@ __global__ __launch_bounds__(256) void adj_ghost2_kernel(float* __restrict__ ub,
 // velocity^T, local part: A = dcrj * f_d * adjoint (the weights the stress
 // adjoints gather), the velocity adjoint scaled by dcrj, and the f_d
 // gradients with the stress differences of the state before the step.
+// GATHER: the strain -> velocity gather of the same cell first (levels that
+// the free-surface ghosts do not touch), so the velocity adjoints make one
+// trip through HBM instead of two.
+template <bool GATHER>
 __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ ub,
                                                             float* __restrict__ vb,
                                                             float* __restrict__ wb, Ptr6 Aw,
                                                             CPtr6 Sg, Medium M, Grad8 Gr,
-                                                            const AdjGrid G) {
+                                                            const AdjGrid G, CPtr6 E, int k0,
+                                                            int k1) {
   const Blk3 B = adj_block();
-  const int k = AWP_ALIGN + (int)(B.x * 64 + threadIdx.x);
+  const int k = k0 + (int)(B.x * 64 + threadIdx.x);
   const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
   const int i = 2 + AWP_NGSL + B.z;
-  if (k > G.top || j >= G.ny + 2 + AWP_NGSL) return;
+  if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
   const long long S = G.slice, Y = G.yline;
   const long long p = aidx(G, i, j, k);
   const float* d1 = M.d1;
@@ -345,7 +388,13 @@ __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ 
   const float Xw = c1 * (xz[p + S] - xz[p]) + c2 * (xz[p + 2 * S] - xz[p - S]) +
                    c1 * (yz[p] - yz[p - Y]) + c2 * (yz[p + Y] - yz[p - 2 * Y]) +
                    c1 * (zz[p + 1] - zz[p]) + c2 * (zz[p + 2] - zz[p - 1]);
-  const float a = ub[p], b = vb[p], c = wb[p];
+  float a = ub[p], b = vb[p], c = wb[p];
+  if (GATHER) {
+    const Vec3 g = strain_gather(E, p, S, Y, c1, c2);
+    a += g.u;
+    b += g.v;
+    c += g.w;
+  }
   Gr.f[0][p] += a * dcrj * Xu;
   Gr.f[1][p] += b * dcrj * Xv;
   Gr.f[2][p] += c * dcrj * Xw;
@@ -358,28 +407,39 @@ __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ 
 }
 
 // velocity^T, gather part: stress adjoints from the weighted velocity
-// adjoints of the velocity cells whose stencils read them (interior columns,
-// k from align+3 to top+2; A is zero outside the velocity interior).
+// adjoints of the velocity cells whose stencils read them (A is zero outside
+// the velocity interior).  On the levels below the free-surface band
+// (k <= top-3) this gather is left pending at the end of a step and applied
+// by the next step's adj_stress_local_kernel, which reads those stress
+// adjoints anyway (adj_settle_kernel applies it after the last step); the
+// band, which fstr^T touches, is gathered at once.
 __global__ __launch_bounds__(256) void adj_vel_gather_kernel(Adj15 Ab, CPtr6 Aw,
-                                                             const AdjGrid G) {
+                                                             const AdjGrid G, int k0, int k1) {
+  const Blk3 B = adj_block();
+  const int k = k0 + (int)(B.x * 64 + threadIdx.x);
+  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
+  const int i = 2 + AWP_NGSL + B.z;
+  if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
+  const long long q = aidx(G, i, j, k);
+  float g[6];
+  vel_gather6(Aw, q, G, g);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) Ab.f[c][q] += g[c];
+}
+
+// The pending velocity^T gather after the last step (levels align+3 ..
+// top-3).
+__global__ __launch_bounds__(256) void adj_settle_kernel(Adj15 Ab, CPtr6 Aw, const AdjGrid G) {
   const Blk3 B = adj_block();
   const int k = AWP_ALIGN + 3 + (int)(B.x * 64 + threadIdx.x);
   const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
   const int i = 2 + AWP_NGSL + B.z;
-  if (k > G.top + 2 || j >= G.ny + 2 + AWP_NGSL) return;
-  const long long S = G.slice, Y = G.yline;
+  if (k > G.top - 3 || j >= G.ny + 2 + AWP_NGSL) return;
   const long long q = aidx(G, i, j, k);
-  const float c1 = G.c1, c2 = G.c2;
-  const float *Au = Aw.f[0], *Av = Aw.f[1], *Aw_ = Aw.f[2];
-  Ab.f[0][q] += c1 * (Au[q] - Au[q + S]) + c2 * (Au[q - S] - Au[q + 2 * S]);
-  Ab.f[1][q] += c1 * (Av[q - Y] - Av[q]) + c2 * (Av[q - 2 * Y] - Av[q + Y]);
-  Ab.f[2][q] += c1 * (Aw_[q - 1] - Aw_[q]) + c2 * (Aw_[q - 2] - Aw_[q + 1]);
-  Ab.f[3][q] += c1 * (Au[q] - Au[q + Y]) + c2 * (Au[q - Y] - Au[q + 2 * Y]) +
-                c1 * (Av[q - S] - Av[q]) + c2 * (Av[q - 2 * S] - Av[q + S]);
-  Ab.f[4][q] += c1 * (Au[q] - Au[q + 1]) + c2 * (Au[q - 1] - Au[q + 2]) +
-                c1 * (Aw_[q - S] - Aw_[q]) + c2 * (Aw_[q - 2 * S] - Aw_[q + S]);
-  Ab.f[5][q] += c1 * (Av[q] - Av[q + 1]) + c2 * (Av[q - 1] - Av[q + 2]) +
-                c1 * (Aw_[q] - Aw_[q + Y]) + c2 * (Aw_[q - Y] - Aw_[q + 2 * Y]);
+  float g[6];
+  vel_gather6(Aw, q, G, g);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) Ab.f[c][q] += g[c];
 }
 
 __global__ __launch_bounds__(256) void adj_inject_kernel(float* __restrict__ ub,
@@ -600,18 +660,40 @@ void awp_adjoint_step(float* const* adj, const float* const* fwd_v, const float*
   hipLaunchKernelGGL(adj_fstr_kernel, g2, b2, 0, St, A.f[2], A.f[4], A.f[5], G);
   const int nks = G.top - (AWP_ALIGN + 3) + 1;
   hipLaunchKernelGGL(adj_stress_local_kernel, dim3(awp_cdiv(nks, 64), awp_cdiv(G.ny, 4), G.nx),
-                     dim3(64, 4), 0, St, A, E, V, M, Gr, G);
-  const int nkv = G.top + 1 - AWP_ALIGN + 1;
-  hipLaunchKernelGGL(adj_strain_gather_kernel, dim3(awp_cdiv(nkv, 64), awp_cdiv(G.ny, 4), G.nx),
-                     dim3(64, 4), 0, St, A.f[12], A.f[13], A.f[14], Ec, G);
+                     dim3(64, 4), 0, St, A, E, V, M, Gr, G, Awc);
+  // levels align .. top-2: strain gather and velocity^T in one pass (the
+  // free-surface ghosts' transpose reads and writes only top-1 .. top+1)
+  const int kf1 = G.top - 2;
+  const int nkf = kf1 - AWP_ALIGN + 1;
+  hipLaunchKernelGGL((adj_vel_local_kernel<true>),
+                     dim3(awp_cdiv(nkf, 64), awp_cdiv(G.ny, 4), G.nx), dim3(64, 4), 0, St,
+                     A.f[12], A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec, AWP_ALIGN, kf1);
+  // levels top-1 .. top+1: gather, ghosts^T, then velocity^T
+  hipLaunchKernelGGL(adj_strain_gather_kernel, dim3(1, awp_cdiv(G.ny, 4), G.nx), dim3(64, 4), 0,
+                     St, A.f[12], A.f[13], A.f[14], Ec, G, G.top - 1, G.top + 1);
   hipLaunchKernelGGL(adj_ghost1_kernel, g2, b2, 0, St, A.f[12], A.f[13], A.f[14], V, M.lam_mu,
                      col, grad_lam_mu, G);
   hipLaunchKernelGGL(adj_ghost2_kernel, g2, b2, 0, St, A.f[12], A.f[13], A.f[14], col, G);
-  const int nkl = G.top - AWP_ALIGN + 1;
-  hipLaunchKernelGGL(adj_vel_local_kernel, dim3(awp_cdiv(nkl, 64), awp_cdiv(G.ny, 4), G.nx),
-                     dim3(64, 4), 0, St, A.f[12], A.f[13], A.f[14], Aw, Sg, M, Gr, G);
-  const int nkg = G.top + 2 - (AWP_ALIGN + 3) + 1;
-  hipLaunchKernelGGL(adj_vel_gather_kernel, dim3(awp_cdiv(nkg, 64), awp_cdiv(G.ny, 4), G.nx),
+  hipLaunchKernelGGL((adj_vel_local_kernel<false>), dim3(1, awp_cdiv(G.ny, 4), G.nx),
+                     dim3(64, 4), 0, St, A.f[12], A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec,
+                     G.top - 1, G.top);
+  // the free-surface band now; the levels below stay pending for the next
+  // step (or awp_adjoint_settle)
+  hipLaunchKernelGGL(adj_vel_gather_kernel, dim3(1, awp_cdiv(G.ny, 4), G.nx), dim3(64, 4), 0,
+                     St, A, Awc, G, G.top - 2, G.top + 2);
+  awp_check_launch(W);
+}
+
+void awp_adjoint_settle(float* const* adj, float* const* scratch, hipStream_t St) {
+  const char* W = "awp_adjoint_settle";
+  if (!awp_grid_ok(0, W)) return;
+  const AdjGrid G = adj_grid(0);
+  Adj15 A;
+  for (int f = 0; f < 15; ++f) A.f[f] = adj[f];
+  CPtr6 Awc = {{scratch[6], scratch[7], scratch[8], nullptr, nullptr, nullptr}};
+  const int nk = G.top - 3 - (AWP_ALIGN + 3) + 1;
+  if (nk <= 0) return;
+  hipLaunchKernelGGL(adj_settle_kernel, dim3(awp_cdiv(nk, 64), awp_cdiv(G.ny, 4), G.nx),
                      dim3(64, 4), 0, St, A, Awc, G);
   awp_check_launch(W);
 }

@@ -270,12 +270,18 @@ void awp_s16_absmax(const uint16_t* q, long long n, int fmt, float* out, hipStre
  * vx1 vx2 wwo lam_mu; scratch[9] are zero-initialised full-size arrays that
  * the call reuses; grad[8] (f_d1 f_d2 f_d3 xl xm xmu1 xmu2 xmu3) and
  * grad_lam_mu accumulate coefficient gradients in binary32; col holds
- * 3 x (nxt+20) x (nyt+20) floats of per-column scratch. */
+ * 3 x (nxt+20) x (nyt+20) floats of per-column scratch.  The last part of
+ * the transposed velocity update (its gather into the six stress adjoints)
+ * is applied below the free-surface band (k <= top-3) by the next call, which
+ * reads those adjoints anyway; scratch[6..8] carry its weights between calls.
+ * After the last step, awp_adjoint_settle applies it, if the caller needs
+ * the adjoint state itself (the gradients do not). */
 void awp_adjoint_step(float* const* adj, const float* const* fwd_v, const float* const* fwd_s,
                       const float* const* med, const int* ww, const float* coeff,
                       const float* dcrjx, const float* dcrjy, const float* dcrjz,
                       float* const* scratch, float* const* grad, float* grad_lam_mu,
                       float* col, hipStream_t St);
+void awp_adjoint_settle(float* const* adj, float* const* scratch, hipStream_t St);
 /* adds the binary64 residuals res[3*n + c] of receiver n (the awp_receivers_
  * gather sampling) to the adjoint velocities */
 void awp_adjoint_inject(float* ub, float* vb, float* wb, const int* idx, int npts,

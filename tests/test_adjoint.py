@@ -80,10 +80,11 @@ def gpu():
     return True
 
 
-def _gpu_problem(case, K):
+def _gpu_problem(case, K, resident_steps=0):
     from awp_amd.adjoint import AdjointProblem
     bg, true, src, rec, tf, obs = case[:6]
-    return AdjointProblem(bg, src, rec, NT, obs.numpy().astype(np.float32), checkpoint_every=K)
+    return AdjointProblem(bg, src, rec, NT, obs.numpy().astype(np.float32), checkpoint_every=K,
+                          resident_steps=resident_steps)
 
 
 def _rel(a, b):
@@ -183,3 +184,26 @@ def test_gpu_out_of_place_recompute_matches_in_place(gpu, case):
                 assert torch.equal(P.ring[n + 1][x], ref[n][x]), (n0, n, x)
         for x in R:
             assert torch.equal(s.f[x], ref_r[x]), (n0, x)
+
+
+@pytest.mark.gpu
+def test_gpu_resident_tail_gives_the_same_gradient(gpu, case):
+    """States of the last segments kept from the forward pass (no recompute
+    for them) against recomputing every segment: the same states, so the
+    same misfit, traces and gradients bit for bit; a partial tail (2 of 6
+    segments) and a fully resident run."""
+    bg = case[0]
+    out = []
+    for resident in (0, 25, None):
+        P = _gpu_problem(case, 10, resident_steps=resident)
+        print(f"resident_steps={resident}: first resident step {P.n_tail}", flush=True)
+        P.set_medium(*_arrays(bg))
+        J = P.forward()
+        tr = P.traces()
+        out.append((J, tr, [g.cpu() for g in P.backward()], P.n_tail))
+    assert out[0][3] > NT and out[1][3] == 40 and out[2][3] == 0
+    for J, tr, g, _ in out[1:]:
+        assert J == out[0][0]
+        assert np.array_equal(tr, out[0][1])
+        for a, b in zip(g, out[0][2]):
+            assert torch.equal(a, b)
