@@ -63,6 +63,21 @@ __device__ __forceinline__ int awp_xcd_remap(int b, int nblocks) {
 
 static inline int awp_cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// (k tile, j tile, i chunk) of logical block b: k tiles fastest, or with
+// jfast the j tiles fastest, so that consecutive blocks (one XCD's L2 after
+// awp_xcd_remap) are j neighbours and share their halo rows.
+__device__ __forceinline__ void awp_tile3(int b, int nkt, int njt, int jfast, int& kt, int& jt,
+                                          int& ic) {
+  if (jfast) {
+    jt = b % njt;
+    kt = (b / njt) % nkt;
+  } else {
+    kt = b % nkt;
+    jt = (b / nkt) % njt;
+  }
+  ic = b / (nkt * njt);
+}
+
 // Loads and stores through a wave-uniform row pointer and a per-lane 32-bit
 // byte offset.  With one wave per j row the row pointer lives in scalar
 // registers and every access of a row lowers to

@@ -10,7 +10,7 @@
 
 // Defaults from sweeps on MI355X at 512x512x256: velocity 4 rows x 2 i-chunks
 // (all workgroups resident), stress 16 rows x ~32-plane i-chunks.
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048};
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -198,6 +198,12 @@ int awp_set_tuning(const char* key, int value) {
     t.topo_q_ichunk = value;
   } else if (!strcmp(key, "step_ichunk")) {
     t.step_ichunk = value < 0 ? 0 : value;
+  } else if (!strcmp(key, "vel_jfast")) {
+    t.vel_jfast = value ? 1 : 0;
+  } else if (!strcmp(key, "str_jfast")) {
+    t.str_jfast = value ? 1 : 0;
+  } else if (!strcmp(key, "step_jfast")) {
+    t.step_jfast = value ? 1 : 0;
   } else if (!strcmp(key, "step_target_wgs")) {
     if (value < 1) return -1;
     t.step_target_wgs = value;

@@ -51,7 +51,7 @@ struct StepArgs {
 
 struct StepLaunch {
   int xls, xre, yls, yre, top;  // update region: i, j in [ls, re], k in [align, top]
-  int nkt, njt, ichunk, ni, nblocks, remap;
+  int nkt, njt, ichunk, ni, nblocks, remap, jfast;
   int yline, lam_mu_ld, nxt;
   long long slice;
   float c1, c2, dth, dh1, DT;
@@ -181,9 +181,8 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
   }
 
   const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
-  const int kt = b % L.nkt;
-  const int jt = (b / L.nkt) % L.njt;
-  const int ic = b / (L.nkt * L.njt);
+  int kt, jt, ic;
+  awp_tile3(b, L.nkt, L.njt, L.jfast, kt, jt, ic);
   const int K0 = AWP_ALIGN + 64 * kt;
   const int j0 = L.yls + jt * FBJ;
   const int ia = L.xls + ic * L.ichunk;
@@ -574,6 +573,7 @@ void awp_step(const float* const* a9, float* const* b9, float* const* r6,
   L.tex_nj = nyt;
   L.tex_nzt = nzt;
   L.remap = awp_tuning().xcd_remap;
+  L.jfast = awp_tuning().step_jfast;
   L.ni = L.xre - L.xls + 1;
   L.nkt = awp_cdiv(L.top - AWP_ALIGN + 1, 64);
   L.njt = awp_cdiv(L.yre - L.yls + 1, FBJ);

@@ -19,7 +19,7 @@ namespace {
 
 struct StrLaunch {
   int kb, k0, k1, j0, j1, i0;  // kb: k of lane 0 in tile 0; [k0, k1] active
-  int nkt, njt, ichunk, ni, nblocks, remap;
+  int nkt, njt, ichunk, ni, nblocks, remap, jfast;
   int yline;
   long long slice;
   float c1, c2, dth, dh1, DT;
@@ -101,9 +101,8 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
   __syncthreads();
 
   const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
-  const int kt = b % L.nkt;
-  const int jt = (b / L.nkt) % L.njt;
-  const int ic = b / (L.nkt * L.njt);
+  int kt, jt, ic;
+  awp_tile3(b, L.nkt, L.njt, L.jfast, kt, jt, ic);
   const int k = L.kb + kt * 64 + (int)threadIdx.x;
   const int j = L.j0 + jt * BJ + (int)threadIdx.y;
   if (k < L.k0 || k > L.k1 || j > L.j1) return;
@@ -412,6 +411,7 @@ void dstrqc_common(const char* W, void* const* f15, const CStress6& S0, float* l
   L.slice = g.slice;
   L.c1 = g.c1; L.c2 = g.c2; L.dth = g.dth; L.dh1 = g.dh1; L.DT = g.DT;
   L.remap = awp_tuning().xcd_remap;
+  L.jfast = awp_tuning().str_jfast;
   if (L.k1 < L.k0) return;
   const int bj = awp_tuning().str_bj;
   L.nkt = awp_cdiv(L.k1 - L.kb + 1, 64);

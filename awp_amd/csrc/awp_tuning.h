@@ -15,6 +15,9 @@ struct AwpTuning {
   int topo_q_ichunk;    // planes per march of the two-pass stress update
   int step_ichunk;      // >0 forces the i-chunk length of the fused step (awp_step)
   int step_target_wgs;  // fused step: workgroups to aim for when splitting the i march
+  int vel_jfast;        // 1: j tiles fastest in the block order (velocity)
+  int str_jfast;        // same, stress
+  int step_jfast;       // same, fused step
 };
 
 AwpTuning& awp_tuning();

@@ -19,7 +19,7 @@ namespace {
 
 struct VelLaunch {
   int kb, k0, k1, j0, j1, i0;  // kb: k of lane 0 in tile 0; [k0, k1] active
-  int nkt, njt, ichunk, ni, nblocks, remap;
+  int nkt, njt, ichunk, ni, nblocks, remap, jfast;
   int yline;
   long long slice;
   float c1, c2, dth;
@@ -44,9 +44,8 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     const float* __restrict__ dcrjy, const float* __restrict__ dcrjz,
     const float* __restrict__ d1, const VelLaunch L, const WaveScales SC) {
   const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
-  const int kt = b % L.nkt;
-  const int jt = (b / L.nkt) % L.njt;
-  const int ic = b / (L.nkt * L.njt);
+  int kt, jt, ic;
+  awp_tile3(b, L.nkt, L.njt, L.jfast, kt, jt, ic);
   // one wave per j row: j is wave-uniform, k is the lane
   const int j = L.j0 + jt * BJ + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const int k = L.kb + kt * 64 + (int)threadIdx.x;
@@ -117,28 +116,6 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
                 v = ST::ldb(OOP ? RP(v0, i, 0, 0) : V, kw, SC.s[AWP_SV]),
                 w = ST::ldb(OOP ? RP(w0, i, 0, 0) : W, kw, SC.s[AWP_SW]);
 
-    if (DM2 && dm2) {
-      // dvelc2: density means as float sums scaled by the double 0.25, the
-      // sponge factor in the order dcrjz * dcrjx * dcrjy
-      const float f_dcrj = dz * dcrjx[i] * dy;
-      float g3 = (float)(0.25 * (double)(dc + ec + djm + ejm));
-      g3 = dth / g3;
-      ST::stb(W, kw, (w + g3 * ((xz_p1 - xz_0) + (yz_0 - yz_jm1) + (zz_kp1 - zz_0))) * f_dcrj,
-              SC.inv[AWP_SW]);
-      float g1 = (float)(0.25 * (double)(dc + djm + dkm + djk));
-      float g2 = (float)(0.25 * (double)(dc + ec + dkm + ekm));
-      g1 = dth / g1;
-      g2 = dth / g2;
-      ST::stb(U, kw, (u + g1 * ((xx_0 - xx_m1) + (xy_0 - xy_jm1) + (xz_0 - xz_km1))) * f_dcrj,
-              SC.inv[AWP_SU]);
-      ST::stb(V, kw, (v + g2 * ((xy_p1 - xy_0) + (yy_jp1 - yy_0) + (yz_0 - yz_km1))) * f_dcrj,
-              SC.inv[AWP_SV]);
-      xx_m2 = xx_m1; xx_m1 = xx_0; xx_0 = xx_p1;
-      xy_m1 = xy_0; xy_0 = xy_p1; xy_p1 = xy_p2;
-      xz_m1 = xz_0; xz_0 = xz_p1; xz_p1 = xz_p2;
-      dc = ec; djm = ejm; dkm = ekm; djk = ejk;
-      continue;
-    }
     const float dcrj = dcrjx[i] * dy * dz;
     float f_d1 = 0.25f * (dc + djm + dkm + djk);
     float f_d2 = 0.25f * (dc + ec + dkm + ekm);
@@ -147,15 +124,32 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     f_d2 = dth / f_d2;
     f_d3 = dth / f_d3;
 
-    ST::stb(U, kw, (u + f_d1 * (c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) + c1 * (xy_0 - xy_jm1) +
-                               c2 * (xy_jp1 - xy_jm2) + c1 * (xz_0 - xz_km1) +
-                               c2 * (xz_kp1 - xz_km2))) * dcrj, SC.inv[AWP_SU]);
-    ST::stb(V, kw, (v + f_d2 * (c1 * (xy_p1 - xy_0) + c2 * (xy_p2 - xy_m1) + c1 * (yy_jp1 - yy_0) +
-                               c2 * (yy_jp2 - yy_jm1) + c1 * (yz_0 - yz_km1) +
-                               c2 * (yz_kp1 - yz_km2))) * dcrj, SC.inv[AWP_SV]);
-    ST::stb(W, kw, (w + f_d3 * (c1 * (xz_p1 - xz_0) + c2 * (xz_p2 - xz_m1) + c1 * (yz_0 - yz_jm1) +
-                               c2 * (yz_jp1 - yz_jm2) + c1 * (zz_kp1 - zz_0) +
-                               c2 * (zz_kp2 - zz_km1))) * dcrj, SC.inv[AWP_SW]);
+    float un = (u + f_d1 * (c1 * (xx_0 - xx_m1) + c2 * (xx_p1 - xx_m2) + c1 * (xy_0 - xy_jm1) +
+                            c2 * (xy_jp1 - xy_jm2) + c1 * (xz_0 - xz_km1) +
+                            c2 * (xz_kp1 - xz_km2))) * dcrj;
+    float vn = (v + f_d2 * (c1 * (xy_p1 - xy_0) + c2 * (xy_p2 - xy_m1) + c1 * (yy_jp1 - yy_0) +
+                            c2 * (yy_jp2 - yy_jm1) + c1 * (yz_0 - yz_km1) +
+                            c2 * (yz_kp1 - yz_km2))) * dcrj;
+    float wn = (w + f_d3 * (c1 * (xz_p1 - xz_0) + c2 * (xz_p2 - xz_m1) + c1 * (yz_0 - yz_jm1) +
+                            c2 * (yz_jp1 - yz_jm2) + c1 * (zz_kp1 - zz_0) +
+                            c2 * (zz_kp2 - zz_km1))) * dcrj;
+    if (DM2) {
+      // dvelc2 on its level, computed on every lane and selected, so the
+      // loop keeps one path.  Its density means are the same float sums
+      // scaled by the double 0.25; a power-of-two scale is exact in both
+      // precisions, so its dth/mean equal f_d1..3 bit for bit.  Its sponge
+      // factor multiplies in the order dcrjz * dcrjx * dcrjy.
+      const float f_dcrj = dz * dcrjx[i] * dy;
+      const float u2 = (u + f_d1 * ((xx_0 - xx_m1) + (xy_0 - xy_jm1) + (xz_0 - xz_km1))) * f_dcrj;
+      const float v2 = (v + f_d2 * ((xy_p1 - xy_0) + (yy_jp1 - yy_0) + (yz_0 - yz_km1))) * f_dcrj;
+      const float w2 = (w + f_d3 * ((xz_p1 - xz_0) + (yz_0 - yz_jm1) + (zz_kp1 - zz_0))) * f_dcrj;
+      un = dm2 ? u2 : un;
+      vn = dm2 ? v2 : vn;
+      wn = dm2 ? w2 : wn;
+    }
+    ST::stb(U, kw, un, SC.inv[AWP_SU]);
+    ST::stb(V, kw, vn, SC.inv[AWP_SV]);
+    ST::stb(W, kw, wn, SC.inv[AWP_SW]);
 
     // advance pipelines
     xx_m2 = xx_m1; xx_m1 = xx_0; xx_0 = xx_p1;
@@ -228,6 +222,7 @@ void dvelcx_common(const char* W, void* const* f9, void* const* v0, float* dcrjx
   L.c2 = g.c2;
   L.dth = g.dth;
   L.remap = awp_tuning().xcd_remap;
+  L.jfast = awp_tuning().vel_jfast;
   L.kb = awp_tuning().kalign ? (L.k0 & ~31) : L.k0;
   const int bj = awp_tuning().vel_bj;
   L.nkt = awp_cdiv(L.k1 - L.kb + 1, 64);

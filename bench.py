@@ -95,7 +95,7 @@ def parse():
     ap.add_argument("--storage", choices=("f32", "bf16", "fp16"), default="f32",
                     help="wavefield storage of the anelastic workload (16-bit: per-field "
                          "power-of-two scales, binary32 arithmetic)")
-    ap.add_argument("--texture", choices=("arrays", "table", "rule"), default="arrays",
+    ap.add_argument("--texture", choices=("arrays", "table", "rule"), default="rule",
                     help="relaxation texture of the FP32 stress update: per-cell vx1/vx2/wwo "
                          "arrays (dstrqc_H), or awp_dstrqc_tex's 8-entry table with the ww "
                          "array (-12 B/cell) or with init_texture's parity rule (-16 B/cell)")
@@ -355,9 +355,13 @@ def main():
     finite = bool(np.isfinite(tr).all())
 
     traffic = None
-    tfile = ROOT / "profiles" / "dstrqc_anelastic_512x512x256_traffic.json"
-    if (tfile.exists() and (args.nx, args.ny, args.nz) == (512, 512, 256) and args.storage == "f32"
-            and args.texture == "arrays" and not args.fused):
+    tname = {("arrays", False): "dstrqc_anelastic_512x512x256_traffic.json",
+             ("rule", False): "dstrqc_rule_anelastic_512x512x256_traffic.json",
+             ("rule", True): "awp_step_rule_anelastic_512x512x256_traffic.json"}.get(
+                 (args.texture, bool(args.fused)), "")
+    tfile = ROOT / "profiles" / tname
+    if (tname and tfile.exists() and (args.nx, args.ny, args.nz) == (512, 512, 256)
+            and args.storage == "f32"):
         try:
             traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
@@ -395,9 +399,9 @@ def main():
                      "traffic": traffic,
                      "traffic_source": ("HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and "
                                         "WRITE_SIZE of this kernel at this size, scaled by a copy "
-                                        "kernel's counts (profiles/dstrqc_anelastic_512x512x256_"
-                                        "traffic.json); read from that file, not counted in this "
-                                        "run") if traffic is not None else None,
+                                        f"kernel's counts (profiles/{tname}); read from that "
+                                        "file, not counted in this run")
+                     if traffic is not None else None,
                      "algorithmic_bytes_per_cell": b_str,
                      "cells_per_launch": cells, "avg_launch_ms": str_avg},
         "kernels": ({"awp_step": {"avg_ms": str_avg, "GBps": str_gbs,
