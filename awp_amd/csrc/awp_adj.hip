@@ -150,11 +150,11 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
                     M.qp[jk1] + M.qp[ijk1];
   const float ss8 = M.qs[p] + M.qs[ip1] + M.qs[jm1] + M.qs[ijk] + M.qs[km1] + M.qs[ik1] +
                     M.qs[jk1] + M.qs[ijk1];
-  const float xm = (float)(16.0 / (double)sm8);
-  const float xl = (float)(8.0 / (double)sl8) + xm;
-  const float xmu1 = (float)(2.0 / (double)(M.mu[p] + M.mu[km1]));
-  const float xmu2 = (float)(2.0 / (double)(M.mu[p] + M.mu[jm1]));
-  const float xmu3 = (float)(2.0 / (double)(M.mu[p] + M.mu[ip1]));
+  const float xm = awp_quot(16.0f, sm8);
+  const float xl = awp_quot(8.0f, sl8) + xm;
+  const float xmu1 = awp_quot(2.0f, M.mu[p] + M.mu[km1]);
+  const float xmu2 = awp_quot(2.0f, M.mu[p] + M.mu[jm1]);
+  const float xmu3 = awp_quot(2.0f, M.mu[p] + M.mu[ip1]);
   const float qpaw = adj_qw(0.0625f * sp8, ca, cb, f_wwo);
   const float hw = adj_qw(0.0625f * ss8, ca, cb, f_wwo);
   const float h1w = adj_qw(0.250f * (M.qs[p] + M.qs[km1]), ca, cb, f_wwo);

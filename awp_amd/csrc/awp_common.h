@@ -63,6 +63,16 @@ __device__ __forceinline__ int awp_xcd_remap(int b, int nblocks) {
 
 static inline int awp_cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// The reference's (float)(c / (double)x) for binary32 c and x.  Rounding the
+// binary64 quotient to binary32 gives the correctly rounded binary32 quotient:
+// double rounding is innocuous when the wider format has at least 2p+2
+// significand bits, and 53 >= 2*24+2 (S. A. Figueroa, "When is double rounding
+// innocuous?", ACM SIGNUM Newsletter 30(3), 1995).  So one binary32 division
+// (correctly rounded: hipcc's default) gives the same bits without binary64
+// arithmetic.  A quotient in the subnormal range flushes to zero either way in
+// this build.
+__device__ __forceinline__ float awp_quot(float c, float x) { return c / x; }
+
 // (k tile, j tile, i chunk) of logical block b: k tiles fastest, or with
 // jfast the j tiles fastest, so that consecutive blocks (one XCD's L2 after
 // awp_xcd_remap) are j neighbours and share their halo rows.

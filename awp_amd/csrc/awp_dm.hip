@@ -94,13 +94,13 @@ __global__ __launch_bounds__(256) void dstrqc2_kernel(
   const float f_dcrj = dcrjz[k] * dcrjx[i] * dcrjy[j];
   const float f_vx1 = vx1p[p], f_vx2 = vx2p[p], f_wwo = wwop[p];
   const int f_ww = wwp[p];
-  float xl = (float)(8.0 / (double)(lam[p] + lam[ip1] + lam[jm1] + lam[ijk] + lam[km1] +
-                                    lam[ik1] + lam[jk1] + lam[ijk1]));
-  float xm = (float)(16.0 / (double)(mu[p] + mu[ip1] + mu[jm1] + mu[ijk] + mu[km1] + mu[ik1] +
-                                     mu[jk1] + mu[ijk1]));
-  float xmu1 = (float)(2.0 / (double)(mu[p] + mu[km1]));
-  float xmu2 = (float)(2.0 / (double)(mu[p] + mu[jm1]));
-  float xmu3 = (float)(2.0 / (double)(mu[p] + mu[ip1]));
+  float xl = awp_quot(8.0f, lam[p] + lam[ip1] + lam[jm1] + lam[ijk] + lam[km1] + lam[ik1] +
+                               lam[jk1] + lam[ijk1]);
+  float xm = awp_quot(16.0f, mu[p] + mu[ip1] + mu[jm1] + mu[ijk] + mu[km1] + mu[ik1] + mu[jk1] +
+                                 mu[ijk1]);
+  float xmu1 = awp_quot(2.0f, mu[p] + mu[km1]);
+  float xmu2 = awp_quot(2.0f, mu[p] + mu[jm1]);
+  float xmu3 = awp_quot(2.0f, mu[p] + mu[ip1]);
   xl = xl + xm;
   float qpa = (float)(0.0625 * (double)(qp[p] + qp[ip1] + qp[jm1] + qp[ijk] + qp[km1] +
                                         qp[ik1] + qp[jk1] + qp[ijk1]));

@@ -404,11 +404,11 @@ __global__ __launch_bounds__(64 * FNW) void step_kernel(const StepArgs A, const 
       const float sm8 = mc + Mc + mjm + Mjm + mkm + Mkm + mjk + Mjk;
       const float sp8 = pc + Pc + pjm + Pjm + pkm + Pkm + pjk + Pjk;
       const float ss8 = sc + Sc + sjm + Sjm + skm + Skm + sjk + Sjk;
-      float xl = (float)(8.0 / (double)sl8);
-      float xm = (float)(16.0 / (double)sm8);
-      float xmu1 = (float)(2.0 / (double)(mc + mkm));
-      float xmu2 = (float)(2.0 / (double)(mc + mjm));
-      float xmu3 = (float)(2.0 / (double)(mc + Mc));
+      float xl = awp_quot(8.0f, sl8);
+      float xm = awp_quot(16.0f, sm8);
+      float xmu1 = awp_quot(2.0f, mc + mkm);
+      float xmu2 = awp_quot(2.0f, mc + mjm);
+      float xmu3 = awp_quot(2.0f, mc + Mc);
       xl = xl + xm;
       float qpa = 0.0625f * sp8;
       const float qpaw = qw(qpa, ca, cb, f_wwo);

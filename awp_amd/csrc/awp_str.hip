@@ -188,11 +188,11 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     const float ss8 = sc + Sc + sjm + Sjm + skm + Skm + sjk + Sjk;
     float xl, xm, xmu1, xmu2, xmu3, qpa, h, h1, h2, h3, qpaw, hw, h1w, h2w, h3w;
     if (!NEWLIT) {
-      xl = (float)(8.0 / (double)sl8);
-      xm = (float)(16.0 / (double)sm8);
-      xmu1 = (float)(2.0 / (double)(mc + mkm));
-      xmu2 = (float)(2.0 / (double)(mc + mjm));
-      xmu3 = (float)(2.0 / (double)(mc + Mc));
+      xl = awp_quot(8.0f, sl8);
+      xm = awp_quot(16.0f, sm8);
+      xmu1 = awp_quot(2.0f, mc + mkm);
+      xmu2 = awp_quot(2.0f, mc + mjm);
+      xmu3 = awp_quot(2.0f, mc + Mc);
       xl = xl + xm;
       qpa = 0.0625f * sp8;
       qpaw = qw_prod(qpa, ca, cb, f_wwo);
@@ -208,8 +208,8 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
       xl = 8.0f / sl8;
       xm = 16.0f / sm8;
       xmu1 = 2.0f / (mc + mkm);
-      xmu2 = (float)(2.0 / (double)(mc + mjm));
-      xmu3 = (float)(2.0 / (double)(mc + Mc));
+      xmu2 = awp_quot(2.0f, mc + mjm);
+      xmu3 = awp_quot(2.0f, mc + Mc);
       xl = xl + xm;
       qpa = 0.0625f * sp8;
       qpaw = qw_new(qpa, ca, cb, f_wwo, true);

@@ -83,9 +83,8 @@ struct TopoVelArgs {
   float a, nu;
 };
 
-__device__ __forceinline__ float dbl_quot(float num, float den) {
-  return (float)((double)num * 1.0 / (double)den);
-}
+// the reference's nu * 1.0 / A with binary32 nu and A (x 1.0 is exact)
+__device__ __forceinline__ float dbl_quot(float num, float den) { return awp_quot(num, den); }
 
 // ---------------------------------------------------------------------------
 // Velocity: dtopo_vel_111_unroll (velocity_unroll.cu:1-461) on levels
@@ -403,8 +402,8 @@ __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int po
   float xm = 16.0f / (mu[pos] + mu[pos_ip1] + mu[pos_jm1] + mu[pos_ijk] + mu[pos_km1] +
                       mu[pos_ik1] + mu[pos_jk1] + mu[pos_ijk1]);
   float xmu1 = 2.0f / (mu[pos] + mu[pos_km1]);
-  float xmu2 = (float)(2.0 / (double)(mu[pos] + mu[pos_jm1]));
-  float xmu3 = (float)(2.0 / (double)(mu[pos] + mu[pos_ip1]));
+  float xmu2 = awp_quot(2.0f, mu[pos] + mu[pos_jm1]);
+  float xmu3 = awp_quot(2.0f, mu[pos] + mu[pos_ip1]);
   xl = xl + xm;
   float qpa = 0.0625f * (qp[pos] + qp[pos_ip1] + qp[pos_jm1] + qp[pos_ijk] + qp[pos_km1] +
                          qp[pos_ik1] + qp[pos_jk1] + qp[pos_ijk1]);
@@ -647,7 +646,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
       float acc;
       // xx, yy, zz
       float Jii = M.f_c[q] * g3c_k;
-      Jii = (float)(1.0 * 1.0 / (double)Jii);
+      Jii = awp_quot(1.0f, Jii);
       if (!bnd) {
         acc = P4F(0) * M.f1_1[q - fr] * Zu[0];
         acc = acc + P4F(1) * M.f1_1[q] * Zu[1];
@@ -683,7 +682,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
       }
       // xy
       float J12i = M.f[q] * g3c_k6;
-      J12i = (float)(1.0 / (double)J12i);
+      J12i = awp_quot(1.0f, J12i);
       const float zu_o = bnd ? zrow(c_str_phdz4r, L[C_UO]) : zmid(phdz4, L[C_UO]);
       const float zu_m1 = bnd ? zrow(c_str_phdz4r, L[C_UM1]) : zmid(phdz4, L[C_UM1]);
       const float zu_p1 = bnd ? zrow(c_str_phdz4r, L[C_UP1]) : zmid(phdz4, L[C_UP1]);
@@ -702,7 +701,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
                          J12i * g_ck * acc;
       // xz
       float J13i = M.f_1[q] * g3_k;
-      J13i = (float)(1.0 * 1.0 / (double)J13i);
+      J13i = awp_quot(1.0f, J13i);
       float xzv1;
       if (!bnd) {
         const float* c = L[C_UO];
@@ -723,7 +722,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
                          J13i * g_k * acc;
       // yz
       float J23i = M.f_2[q] * g3_k;
-      J23i = (float)(1.0 * 1.0 / (double)J23i);
+      J23i = awp_quot(1.0f, J23i);
       float yzv1;
       if (!bnd) {
         const float* c = L[C_VO];

@@ -23,6 +23,7 @@ finite-difference direction as a sanity check.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -68,7 +69,7 @@ def parse():
                     help="run only this many steps with no event timing (for rocprofv3)")
     ap.add_argument("--workload",
                     choices=("anelastic", "dm", "adjoint", "topo", "lagrange", "kinematic",
-                             "cpu"),
+                             "cpu", "stream"),
                     default="anelastic",
                     help="anelastic: the headline single-grid workload; dm: discontinuous mesh "
                          "+ plasticity (fine 3nx x 3ny x nz-fine over coarse nx x ny x nz); "
@@ -77,7 +78,8 @@ def parse():
                          "lagrange: Lagrange-interpolated receivers and sources on the "
                          "512 x 512 x 256 layout; kinematic: finite-fault subfaults "
                          "(addkinsrc_H, Brune/Liu, 4th-order filter) on the same layout; cpu: "
-                         "the CPU baseline table only (no GPU)")
+                         "the CPU baseline table only (no GPU); stream: the HBM bandwidth of "
+                         "each kernel's access mix with no stencil (awp_stream_probe)")
     ap.add_argument("--receivers", type=int, default=20000,
                     help="lagrange workload: receivers (three components each)")
     ap.add_argument("--subfaults", type=int, default=4000,
@@ -239,6 +241,9 @@ def main():
         return
     if args.workload == "dm":
         bench_dm(args)
+        return
+    if args.workload == "stream":
+        bench_stream(args)
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
@@ -428,6 +433,65 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+# (name, reads, writes) per cell: the access mixes of the stencil kernels
+STREAM_MIXES = (("copy", 1, 1), ("dvelcx (52 B)", 10, 3), ("dstrqc_H (140 B)", 23, 12),
+                ("awp_dstrqc_tex, rule (124 B)", 19, 12), ("awp_step, rule (140 B)", 20, 15))
+
+
+def bench_stream(args) -> None:
+    """The bandwidth each access mix reaches with no stencil: awp_stream_probe
+    over arrays of the 512x512x256 field size (padded, 90.6 M floats), every
+    element read or written once, at 4, 8 and 16 bytes per lane, over a sweep
+    of grid sizes; the best of each is reported."""
+    import numpy as np
+    import torch
+
+    from awp_amd import _lib
+    from awp_amd.grid import Dims
+    lib = _lib.load()
+    d = Dims(512, 512, 256)
+    n = d.size
+    nin = max(m[1] for m in STREAM_MIXES)
+    nout = max(m[2] for m in STREAM_MIXES)
+    ins = [torch.full((n,), 1.0 + a, dtype=torch.float32, device="cuda") for a in range(nin)]
+    outs = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(nout)]
+    pin = (ctypes.c_void_p * nin)(*[t.data_ptr() for t in ins])
+    pout = (ctypes.c_void_p * nout)(*[t.data_ptr() for t in outs])
+    st = torch.cuda.current_stream()
+    rows = []
+    for name, r, w in STREAM_MIXES:
+        best = {}
+        for vec4 in (0, 2, 1):
+            for blocks in (1024, 2048, 4096, 8192, 16384):
+                def go():
+                    rc = lib.awp_stream_probe(pin, r, pout, w, n, vec4, blocks, st.cuda_stream)
+                    if rc:
+                        _lib.check()
+                for _ in range(2):
+                    go()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                reps = max(3, args.steps if args.steps < 50 else 10)
+                e0.record(st)
+                for _ in range(reps):
+                    go()
+                e1.record(st)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                gbs = (r + w) * 4.0 * n / (ms * 1e-3) / 1e9
+                key = {0: "scalar", 1: "vec4", 2: "vec2"}[vec4]
+                if gbs > best.get(key, {}).get("GBps", 0.0):
+                    best[key] = {"GBps": gbs, "ms": ms, "blocks": blocks}
+        ok = bool(np.isclose(float(outs[0][0].item()), float(sum(1.0 + a for a in range(r)))))
+        rows.append({"mix": name, "reads": r, "writes": w, "bytes_per_cell": 4 * (r + w),
+                     "scalar": best["scalar"], "vec2": best["vec2"], "vec4": best["vec4"],
+                     "checked": ok})
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    _lib.check()
+    print(json.dumps({"metric": "HBM streaming probe (awp_stream_probe)", "unit": "GB/s",
+                      "floats_per_array": n, "peak_GBps": HBM_PEAK_GBS, "rows": rows}))
 
 
 def copy_probe(nbytes: int) -> None:

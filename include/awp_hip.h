@@ -364,6 +364,15 @@ void awp_step(const float* const* a9, float* const* b9, float* const* r6,
               const float* dcrjx, const float* dcrjy, const float* dcrjz, const float* lam_mu,
               int nxt, int nyt, int nzt, hipStream_t St);
 
+/* HBM streaming probe (no reference counterpart): reads nin arrays and writes
+ * nout arrays of n floats, every element once, moving 4 bytes per lane and
+ * array (vec4 = 0), 16 bytes (vec4 = 1) or 8 bytes (vec4 = 2; n a multiple of
+ * the width, arrays aligned to it), over `blocks` workgroups of 256 lanes.  bench.py --workload stream
+ * uses it to measure the bandwidth an access mix reaches without stencils.
+ * Returns 0 or an AWP_ERR_* code (nin, nout <= 32). */
+int awp_stream_probe(const float* const* in, int nin, float* const* out, int nout, long long n,
+                     int vec4, int blocks, hipStream_t St);
+
 /* ---- The reference's structs at the topography and Lagrange boundary -----
  * The topography and Lagrange launchers take the reference's own structs by
  * pointer.  They are restated here member for member, in the reference's
