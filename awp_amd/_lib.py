@@ -65,7 +65,7 @@ SIGNATURES: dict[str, list] = {
     "awp_dstrqc_oop": [_fp] * 24 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9,
     "awp_dstrqc_tex": [_fp] * 23 + [_i, _i, _s] + [_fp] * 3 + [_i] * 9,
     "awp_step": [_fp] * 11 + [_i, _i, _i, _s],
-    "awp_stream_probe": [_fp, _i, _fp, _i, C.c_longlong, _i, _i, _s],
+    "awp_stream_probe": [_fp, _i, _fp, _i, C.c_longlong, _i, _i, _i, _s],
     # topo_t* travels as a plain address (ctypes.byref of awp_amd.refstructs.Topo)
     "topo_set_constants": [_fp],
     "topo_velocity_interior_H": [_fp],

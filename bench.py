@@ -443,8 +443,10 @@ STREAM_MIXES = (("copy", 1, 1), ("dvelcx (52 B)", 10, 3), ("dstrqc_H (140 B)", 2
 def bench_stream(args) -> None:
     """The bandwidth each access mix reaches with no stencil: awp_stream_probe
     over arrays of the 512x512x256 field size (padded, 90.6 M floats), every
-    element read or written once, at 4, 8 and 16 bytes per lane, over a sweep
-    of grid sizes; the best of each is reported."""
+    element read or written once, at 4, 8 and 16 bytes per lane, with each
+    array's load waited for before the next (serial) or all of a lane's loads
+    in flight together (batched), over a sweep of grid sizes; the best of
+    each is reported."""
     import numpy as np
     import torch
 
@@ -462,32 +464,36 @@ def bench_stream(args) -> None:
     st = torch.cuda.current_stream()
     rows = []
     for name, r, w in STREAM_MIXES:
-        best = {}
-        for vec4 in (0, 2, 1):
-            for blocks in (1024, 2048, 4096, 8192, 16384):
-                def go():
-                    rc = lib.awp_stream_probe(pin, r, pout, w, n, vec4, blocks, st.cuda_stream)
-                    if rc:
-                        _lib.check()
-                for _ in range(2):
-                    go()
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                reps = max(3, args.steps if args.steps < 50 else 10)
-                e0.record(st)
-                for _ in range(reps):
-                    go()
-                e1.record(st)
-                torch.cuda.synchronize()
-                ms = e0.elapsed_time(e1) / reps
-                gbs = (r + w) * 4.0 * n / (ms * 1e-3) / 1e9
-                key = {0: "scalar", 1: "vec4", 2: "vec2"}[vec4]
-                if gbs > best.get(key, {}).get("GBps", 0.0):
-                    best[key] = {"GBps": gbs, "ms": ms, "blocks": blocks}
-        ok = bool(np.isclose(float(outs[0][0].item()), float(sum(1.0 + a for a in range(r)))))
-        rows.append({"mix": name, "reads": r, "writes": w, "bytes_per_cell": 4 * (r + w),
-                     "scalar": best["scalar"], "vec2": best["vec2"], "vec4": best["vec4"],
-                     "checked": ok})
+        row = {"mix": name, "reads": r, "writes": w, "bytes_per_cell": 4 * (r + w)}
+        for sched, batched in (("serial", 0), ("batched", 1)):
+            best = {}
+            for vec4 in (0, 2, 1):
+                for blocks in (1024, 2048, 4096, 8192, 16384):
+                    def go():
+                        rc = lib.awp_stream_probe(pin, r, pout, w, n, vec4, blocks, batched,
+                                                  st.cuda_stream)
+                        if rc:
+                            _lib.check()
+                    for _ in range(2):
+                        go()
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    reps = max(3, args.steps if args.steps < 50 else 10)
+                    e0.record(st)
+                    for _ in range(reps):
+                        go()
+                    e1.record(st)
+                    torch.cuda.synchronize()
+                    ms = e0.elapsed_time(e1) / reps
+                    gbs = (r + w) * 4.0 * n / (ms * 1e-3) / 1e9
+                    key = {0: "4B", 1: "16B", 2: "8B"}[vec4]
+                    if gbs > best.get(key, {}).get("GBps", 0.0):
+                        best[key] = {"GBps": gbs, "ms": ms, "blocks": blocks}
+            want = float(sum(1.0 + a for a in range(r)))
+            ok = all(bool(np.isclose(float(outs[a][i].item()), want + a))
+                     for a in range(w) for i in (0, n - 1))
+            row[sched] = dict(best, checked=ok)
+        rows.append(row)
         print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
     _lib.check()
     print(json.dumps({"metric": "HBM streaming probe (awp_stream_probe)", "unit": "GB/s",

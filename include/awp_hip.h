@@ -367,11 +367,14 @@ void awp_step(const float* const* a9, float* const* b9, float* const* r6,
 /* HBM streaming probe (no reference counterpart): reads nin arrays and writes
  * nout arrays of n floats, every element once, moving 4 bytes per lane and
  * array (vec4 = 0), 16 bytes (vec4 = 1) or 8 bytes (vec4 = 2; n a multiple of
- * the width, arrays aligned to it), over `blocks` workgroups of 256 lanes.  bench.py --workload stream
- * uses it to measure the bandwidth an access mix reaches without stencils.
- * Returns 0 or an AWP_ERR_* code (nin, nout <= 32). */
+ * the width, arrays aligned to it), over `blocks` workgroups of 256 lanes.
+ * batched = 0: each array's load is waited for before the next is issued;
+ * batched = 1: a lane issues the loads of all nin arrays before using any.
+ * bench.py --workload stream uses it to measure the bandwidth an access mix
+ * reaches without stencils.  Returns 0 or an AWP_ERR_* code
+ * (nin, nout <= 32; nin >= 1 when batched). */
 int awp_stream_probe(const float* const* in, int nin, float* const* out, int nout, long long n,
-                     int vec4, int blocks, hipStream_t St);
+                     int vec4, int blocks, int batched, hipStream_t St);
 
 /* ---- The reference's structs at the topography and Lagrange boundary -----
  * The topography and Lagrange launchers take the reference's own structs by
