@@ -60,11 +60,12 @@ __global__ __launch_bounds__(256) void dvelc2_kernel(
 // ---------------------------------------------------------------------------
 // dstrqc2.  Q weights with this kernel's literal types: the qpa test is single
 // precision and the else branches carry no factor 2 (kernel.cu:2638-2683).
-__device__ __forceinline__ float dm_qweight(const float* coeff, int ww, float wwo, float q) {
+// ca, cb: coeff[ww*2-2], coeff[ww*2-1], loaded once per cell before the
+// branches.
+__device__ __forceinline__ float dm_qweight(float ca, float cb, float wwo, float q) {
   float w;
   if (1. / ((double)q * 2.0) <= 200.0)
-    w = (float)((double)coeff[ww * 2 - 2] * (2. * (double)q) * (2. * (double)q) +
-                (double)coeff[ww * 2 - 1] * (2. * (double)q));
+    w = (float)((double)ca * (2. * (double)q) * (2. * (double)q) + (double)cb * (2. * (double)q));
   else
     w = wwo * q;
   return w / wwo;
@@ -104,22 +105,23 @@ __global__ __launch_bounds__(256) void dstrqc2_kernel(
   xl = xl + xm;
   float qpa = (float)(0.0625 * (double)(qp[p] + qp[ip1] + qp[jm1] + qp[ijk] + qp[km1] +
                                         qp[ik1] + qp[jk1] + qp[ijk1]));
+  const float ca = coeff[f_ww * 2 - 2], cb = coeff[f_ww * 2 - 1];
   float qpaw;
   if (1.0f / (qpa * 2.0f) <= 200.0f)
-    qpaw = (float)((double)coeff[f_ww * 2 - 2] * (2. * (double)qpa) * (2. * (double)qpa) +
-                   (double)coeff[f_ww * 2 - 1] * (2. * (double)qpa));
+    qpaw = (float)((double)ca * (2. * (double)qpa) * (2. * (double)qpa) +
+                   (double)cb * (2. * (double)qpa));
   else
     qpaw = f_wwo * qpa;
   qpaw = qpaw / f_wwo;
   float h = 0.0625f * (qs[p] + qs[ip1] + qs[jm1] + qs[ijk] + qs[km1] + qs[ik1] + qs[jk1] +
                        qs[ijk1]);
-  const float hw = dm_qweight(coeff, f_ww, f_wwo, h);
+  const float hw = dm_qweight(ca, cb, f_wwo, h);
   float h1 = (float)(0.25 * (double)(qs[p] + qs[km1]));
-  const float h1w = dm_qweight(coeff, f_ww, f_wwo, h1);
+  const float h1w = dm_qweight(ca, cb, f_wwo, h1);
   float h2 = (float)(0.250 * (double)(qs[p] + qs[jm1]));
-  const float h2w = dm_qweight(coeff, f_ww, f_wwo, h2);
+  const float h2w = dm_qweight(ca, cb, f_wwo, h2);
   float h3 = (float)(0.250 * (double)(qs[p] + qs[ip1]));
-  const float h3w = dm_qweight(coeff, f_ww, f_wwo, h3);
+  const float h3w = dm_qweight(ca, cb, f_wwo, h3);
   h = -xm * hw * dh1;
   h1 = -xmu1 * h1w * dh1;
   h2 = -xmu2 * h2w * dh1;
@@ -214,35 +216,41 @@ struct Coupling {
   int nxf_pad, nyf_pad;   // fine padded extents nxt+4+ngsl2, nyt+4+ngsl2
 };
 
-// intp3d: per coarse column four corner values per field, 16 fine writes.
-// Fine offsets (added to (ii-1, jj-1)) per field, kernel.cu:2397-2557.
+// intp3d: one lane per fine cell of one field (blockIdx.z).  Each coarse
+// column (i, j) writes a 4x4 block of fine cells at (ii-1, jj-1) = 0..3 from
+// its base; neighbouring blocks overlap in their last and first rows.  In
+// the reference's loop order (i outer, j inner, kernel.cu:2379-2557 restated
+// sequentially) an overlapped cell keeps the value of the column with the
+// largest i and then the largest j, so the lane of fine offset t (0 .. 3*Ni+3
+// from the first block's base) takes coarse column min(t/3, Ni) and weight
+// index t - 3*that, and every fine cell is written once.  Fine offsets (ox,
+// oy) per field, kernel.cu:2397-2557.
 __constant__ int c_intp_ox[9] = {0, 1, 1, 1, 1, 1, 0, 0, 1};
 __constant__ int c_intp_oy[9] = {0, 1, 0, 0, 0, 0, 1, 0, 1};
 
+__device__ __forceinline__ float intp_w(int a) {
+  return a == 0 ? 1.0f : a == 1 ? (float)(2. / 3.) : a == 2 ? (float)(1. / 3.) : 0.0f;
+}
+
 __global__ __launch_bounds__(256) void intp3d_kernel(const Fields9 L9, Fields9 H9,
                                                      const Coupling C) {
-  const int j = C.cj0 + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int i = C.ci0 + (int)(blockIdx.y * 4 + threadIdx.y);
-  if (j > C.cj1 || i > C.ci1) return;
-  const float w[4] = {1.0f, (float)(2. / 3.), (float)(1. / 3.), 0.0f};
-  const long long pl = (long long)i * C.slc + (long long)j * C.ylc + C.kc;
-  const int ih = (i - 2 - AWP_NGSL) * 3 + 2 + AWP_NGSL;
-  const int jh = (j - 2 - AWP_NGSL) * 3 + 2 + AWP_NGSL;
-  const long long ph = (long long)ih * C.slf + (long long)jh * C.ylf + AWP_ALIGN + 1;
-#pragma unroll
-  for (int f = 0; f < 9; ++f) {
-    const float* a = L9.f[f];
-    const float v0 = a[pl], v1 = a[pl + C.slc], v2 = a[pl + C.ylc], v3 = a[pl + C.slc + C.ylc];
-    float* h = H9.f[f];
-    const long long base = ph + (long long)c_intp_ox[f] * C.slf + (long long)c_intp_oy[f] * C.ylf;
-#pragma unroll
-    for (int jj = 1; jj <= 4; ++jj)
-#pragma unroll
-      for (int ii = 1; ii <= 4; ++ii)
-        h[base + (long long)(ii - 1) * C.slf + (long long)(jj - 1) * C.ylf] =
-            v0 * w[ii - 1] * w[jj - 1] + v1 * w[4 - ii] * w[jj - 1] + v2 * w[ii - 1] * w[4 - jj] +
-            v3 * w[4 - ii] * w[4 - jj];
-  }
+  const int f = (int)blockIdx.z;
+  const int Ni = C.ci1 - C.ci0, Nj = C.cj1 - C.cj0;
+  const int tj = (int)(blockIdx.x * 64 + threadIdx.x);
+  const int ti = (int)(blockIdx.y * 4 + threadIdx.y);
+  if (tj > 3 * Nj + 3 || ti > 3 * Ni + 3) return;
+  const int ir = min(ti / 3, Ni), jr = min(tj / 3, Nj);
+  const int a = ti - 3 * ir, b = tj - 3 * jr;  // ii-1, jj-1
+  const long long pl =
+      (long long)(C.ci0 + ir) * C.slc + (long long)(C.cj0 + jr) * C.ylc + C.kc;
+  const float* A = L9.f[f];
+  const float v0 = A[pl], v1 = A[pl + C.slc], v2 = A[pl + C.ylc], v3 = A[pl + C.slc + C.ylc];
+  const float wa = intp_w(a), wb = intp_w(b), wa1 = intp_w(3 - a), wb1 = intp_w(3 - b);
+  // fine base of the first coarse column, 3*(ci0-2-ngsl) + 2 + ngsl
+  const int fi = (C.ci0 - 2 - AWP_NGSL) * 3 + 2 + AWP_NGSL + c_intp_ox[f] + ti;
+  const int fj = (C.cj0 - 2 - AWP_NGSL) * 3 + 2 + AWP_NGSL + c_intp_oy[f] + tj;
+  H9.f[f][(long long)fi * C.slf + (long long)fj * C.ylf + AWP_ALIGN + 1] =
+      v0 * wa * wb + v1 * wa1 * wb + v2 * wa * wb1 + v3 * wa1 * wb1;
 }
 
 // swap3.  One wave per tile of SW_TI x SW_TJ coarse columns.  For each field
@@ -255,6 +263,7 @@ constexpr int SW_TI = 4, SW_TJ = 16;
 constexpr int SW_FI = 3 * SW_TI + 3, SW_FJ = 3 * SW_TJ + 3, SW_NL = 8;
 constexpr int SW_CP = SW_NL + 1;  // column pitch in LDS (odd: spreads banks)
 constexpr int SW_WWL = 2;         // include/awp/pmcl3d_cons.h:92
+constexpr int SW_G = 4;           // slab columns per lane loaded together
 
 struct SwapPass {
   int kc;      // 0: coarse level nzt+align-2, 1: nzt+align-1
@@ -276,9 +285,9 @@ __constant__ SwapPass c_swap_pass[9][2] = {
     {{0, 3, 3, 3}, {0, 3, 3, 3}},  // yz: kh align+4
 };
 
-// One block per (field, tile): blockIdx.x = f * ntiles + tile, so the
-// blocks of one field run together and neighbouring tiles share their halo
-// columns in L2.  vec: each fine column's levels align .. align+8 arrive in
+// One block per (field, tile): block f * ntiles + tile of the XCD-remapped
+// order, so the blocks of one field run together and neighbouring tiles
+// share their halo columns in one L2.  vec: each fine column's levels align .. align+8 arrive in
 // two 16-byte loads and one 4-byte load (yline a multiple of 4 floats and
 // 16-byte aligned field bases), else one 4-byte load per level.
 __global__ __launch_bounds__(64) void swap3_kernel(Fields9 L9, const Fields9 H9,
@@ -287,8 +296,11 @@ __global__ __launch_bounds__(64) void swap3_kernel(Fields9 L9, const Fields9 H9,
   __shared__ float slab[SW_FI * SW_FJ * SW_CP];
   __shared__ double wt[125];
   const int lane = (int)threadIdx.x;
-  const int f = (int)blockIdx.x / ntiles;
-  const int tile = (int)blockIdx.x % ntiles;
+  // XCD-aware order: each XCD takes a contiguous range of (field, tile)
+  // blocks, so neighbouring tiles, which share halo columns, share an L2
+  const int blk = awp_xcd_remap((int)blockIdx.x, (int)gridDim.x);
+  const int f = blk / ntiles;
+  const int tile = blk % ntiles;
   const int bi = tile / ntj, bj = tile % ntj;
   const int ti = lane / SW_TJ, tj = lane % SW_TJ;
   const int i0 = C.ci0 + bi * SW_TI, j0 = C.cj0 + bj * SW_TJ;
@@ -311,26 +323,41 @@ __global__ __launch_bounds__(64) void swap3_kernel(Fields9 L9, const Fields9 H9,
             (1. + SW_WWL - abs(kk));
   }
 
-  // the fine slab of this field: one lane per column, levels align+1 .. +8
+  // the fine slab of this field: one lane per column, levels align+1 .. +8.
+  // The columns go in groups of SW_G per lane, each group's loads issued
+  // before any is stored to LDS; a column past the slab or the padded grid
+  // loads column 0 of the slab instead and stores nothing.
   const float* src = H9.f[f];
-  for (int col = lane; col < SW_FI * SW_FJ; col += 64) {
-    const int a = col / SW_FJ, b = col % SW_FJ;
-    const int fi = fi0 + a, fj = fj0 + b;
-    if (fi < C.nxf_pad && fj < C.nyf_pad) {
+  constexpr int NCOL = SW_FI * SW_FJ;
+  constexpr int NIT = (NCOL + 63) / 64;
+  for (int g0 = 0; g0 < NIT; g0 += SW_G) {
+    float v[SW_G][SW_NL];
+    bool ok[SW_G];
+#pragma unroll
+    for (int u = 0; u < SW_G; ++u) {
+      const int col = lane + (g0 + u) * 64;
+      const int a = col / SW_FJ, b = col % SW_FJ;
+      ok[u] = g0 + u < NIT && col < NCOL && fi0 + a < C.nxf_pad && fj0 + b < C.nyf_pad;
+      const int fi = ok[u] ? fi0 + a : fi0, fj = ok[u] ? fj0 + b : fj0;
       const float* q = src + (long long)fi * C.slf + (long long)fj * C.ylf + AWP_ALIGN;
-      float* d = &slab[col * SW_CP];
       if (vec) {
         const float4 lo = *reinterpret_cast<const float4*>(q);
         const float4 hi = *reinterpret_cast<const float4*>(q + 4);
-        const float x8 = q[8];
-        d[0] = lo.y; d[1] = lo.z; d[2] = lo.w;
-        d[3] = hi.x; d[4] = hi.y; d[5] = hi.z; d[6] = hi.w;
-        d[7] = x8;
+        v[u][0] = lo.y; v[u][1] = lo.z; v[u][2] = lo.w;
+        v[u][3] = hi.x; v[u][4] = hi.y; v[u][5] = hi.z; v[u][6] = hi.w;
+        v[u][7] = q[8];
       } else {
 #pragma unroll
-        for (int l = 0; l < SW_NL; ++l) d[l] = q[1 + l];
+        for (int l = 0; l < SW_NL; ++l) v[u][l] = q[1 + l];
       }
     }
+#pragma unroll
+    for (int u = 0; u < SW_G; ++u)
+      if (ok[u]) {
+        float* d = &slab[(lane + (g0 + u) * 64) * SW_CP];
+#pragma unroll
+        for (int l = 0; l < SW_NL; ++l) d[l] = v[u][l];
+      }
   }
   __syncthreads();
   if (!active) return;
@@ -469,7 +496,8 @@ void intp3d_H(float* u1l, float* v1l, float* w1l, float* xxl, float* yyl, float*
   C.kc = gc->nzt + AWP_ALIGN - 3;
   Fields9 L9 = {{u1l, v1l, w1l, xxl, yyl, zzl, xyl, xzl, yzl}};
   Fields9 H9 = {{u1h, v1h, w1h, xxh, yyh, zzh, xyh, xzh, yzh}};
-  dim3 grid(awp_cdiv(C.cj1 - C.cj0 + 1, 64), awp_cdiv(C.ci1 - C.ci0 + 1, 4));
+  // one lane per fine cell: 3*(coarse columns - 1) + 4 per direction, 9 fields
+  dim3 grid(awp_cdiv(3 * (C.cj1 - C.cj0) + 4, 64), awp_cdiv(3 * (C.ci1 - C.ci0) + 4, 4), 9);
   hipLaunchKernelGGL(intp3d_kernel, grid, dim3(64, 4), 0, St, L9, H9, C);
   awp_check_launch(W);
 }

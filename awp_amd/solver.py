@@ -184,11 +184,12 @@ class Solver(PmclStepOrder):
                                      Rzc, RzTc)
         _lib.check()
         # curvilinear topography replaces grid 0's velocity and stress updates
-        # (pmcl3d.c:1661-1694, 1843-1856)
+        # (pmcl3d.c:1661-1694, 1843-1856); the grids below it stay Cartesian
+        # and couple to it through the discontinuous-mesh kernels
         self.topo = None
         if topography is not None:
-            if G != 1 or plastic is not None:
-                raise ValueError("topography runs one grid without plasticity")
+            if plastic is not None:
+                raise ValueError("topography runs without plasticity")
             from .topography import TopoDevice
             self.topo = TopoDevice(self.lib, self.levels[0], topography, self.device,
                                    self.stream)
@@ -275,8 +276,10 @@ class Solver(PmclStepOrder):
            e_i, p, self.ngrids)
 
     def velocity_dm(self, p: int) -> None:
-        if self.dm_fused:
+        if self.dm_fused and not (p == 0 and self.topo is not None):
             return  # done in the velocity sweep (awp_dvelcx_dm)
+        # a curvilinear grid 0 has no Cartesian sweep to carry the level:
+        # dvelc2_H runs on its own, as in pmcl3d
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         self.lib.dvelc2_H(*(_ptr(f[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz",

@@ -184,7 +184,8 @@ NGSL_ORACLE = 8
 class OracleSolver(PmclStepOrder):
     """medium: one Medium or a per-grid list (grid 0 on top); sources and
     plastic: one entry (grid 0) or per-grid lists; receivers on grid 0;
-    topography: curvilinear grid 0 (one grid, no plasticity)."""
+    topography: curvilinear grid 0 (no plasticity), above any Cartesian
+    grids of a discontinuous mesh."""
 
     def __init__(self, medium, sources=None, receivers=None, nt_record: int = 0,
                  dtype=np.float32, variant: str = "dstrqc", NPC: int = 0, plastic=None,
@@ -211,8 +212,8 @@ class OracleSolver(PmclStepOrder):
         self.cur_step = 0
         self.topo = None
         if topography is not None:
-            if G != 1 or plastic is not None:
-                raise ValueError("topography runs one grid without plasticity")
+            if plastic is not None:
+                raise ValueError("topography runs without plasticity")
             L0 = self.levels[0]
             self.topo = TopoHost(self.lib, topography, L0.dims, L0.DH, L0.DT, self.dtype)
         self.kinematic = None

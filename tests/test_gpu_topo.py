@@ -125,6 +125,36 @@ def test_gpu_topography_run_bit_exact(gpu, oracle_lib, two_pass):
 
 
 @pytest.mark.gpu
+def test_gpu_topography_over_dm_bit_exact(gpu, oracle_lib):
+    """Curvilinear grid 0 (hills) over a 3x coarser Cartesian grid
+    (pmcl3d.c:1661-1694, 1843): the curvilinear updates on grid 0, dvelc2_H
+    and dstrqc2_H on its align+2 level, intp3d_H and swap_H between the two.
+    120 steps of NVE = 1 with the source in the coarse grid: surface traces
+    and both grids' wavefields bit-exact against the FP32 oracle."""
+    torch, _lib = gpu
+    from awp_amd import configs
+    from awp_amd.solver import FIELDS, Solver
+    from oracle.oracle import OracleSolver
+    nt = 120
+    w = configs.dm_plastic(nx=16, ny=16, nz=12, nz_fine=32, DH=300.0, DT=0.005, nt=nt,
+                           nsub=4, ND=4, M0=1.0e16, f0=5.0, src_depth=3, src_grid=1)
+    media = w.medium()
+    d0 = media[0].dims
+    topo = Topography.gaussian_hills(d0.nxt, d0.nyt, media[0].DH, n=3, seed=5, hmax=400.0,
+                                     wmin=600.0, wmax=1500.0)
+    g = Solver(media, w.sources, w.receivers, nt_record=nt, topography=topo)
+    g.run(nt, check_every=20)
+    o = OracleSolver(media, w.sources, w.receivers, nt_record=nt, dtype=np.float32,
+                     topography=topo)
+    o.run(nt)
+    assert np.abs(o.traces()).max() > 0
+    assert np.array_equal(g.traces(), o.traces())
+    for p in range(2):
+        for n in FIELDS:
+            assert np.array_equal(g.field(n, p), o.levels[p].f[n]), (p, n)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("storage", ["bf16", "fp16"])
 def test_gpu_topography_16bit_bit_exact(gpu, oracle_lib, storage):
     from awp_amd.solver import Solver16

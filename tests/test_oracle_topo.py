@@ -408,7 +408,8 @@ def _random_state(d, rng, dtype):
 def test_flat_topography_equals_cartesian(oracle_lib):
     """f = 1, g linear, h = 1, dt = 0.05/3 (tests/test_attenuation.cu:37-42):
     the curvilinear stress (with Q) and velocity updates equal dstrqc_new and
-    dvelcx on random fields in the interior, to single-precision rounding."""
+    dvelcx on random fields in the interior, to single-precision rounding;
+    the velocity increments agree where density does not vary along x."""
     import ctypes as C
     from awp_amd import configs
     from oracle.oracle import load
@@ -441,16 +442,129 @@ def test_flat_topography_equals_cartesian(oracle_lib):
         a, b = ft[n][box].astype(np.float64), fc[n][box].astype(np.float64)
         tot += np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
     assert tot < 1e-5, tot
-    # velocity
-    ft = {n: a.copy() for n, a in f0.items()}
-    fc = {n: a.copy() for n, a in f0.items()}
-    T.velocity(ft, m, DT)
-    lib.awp_oracle_dvelcx_f32(
-        *(p(fc[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")),
-        p(m["dcrjx"]), p(m["dcrjy"]), p(m["dcrjz"]), p(m["d1"]), d.nxt, d.nyt, d.nzt, DH, DT,
-        NGSL + 2, NGSL + 1 + d.nxt, 0, 1)
-    box = (slice(NGSL + 2, NGSL + 2 + d.nxt), slice(NGSL + 2, NGSL + 2 + d.nyt),
-           slice(ALIGN + 3, ALIGN + d.nzt - 6))
-    for n in ("u1", "v1", "w1"):
-        a, b = ft[n][box].astype(np.float64), fc[n][box].astype(np.float64)
-        assert np.abs(a - b).max() / np.abs(b).max() < 1e-5, n
+    # velocity: from zero velocities, so the compared values are the
+    # increments DT * div(sigma) / rho themselves
+    def vel_increments(m):
+        ft = {n: a.copy() for n, a in f0.items()}
+        fc = {n: a.copy() for n, a in f0.items()}
+        for f in (ft, fc):
+            for n in ("u1", "v1", "w1"):
+                f[n][...] = 0
+        T.velocity(ft, m, DT)
+        lib.awp_oracle_dvelcx_f32(
+            *(p(fc[n]) for n in ("u1", "v1", "w1", "xx", "yy", "zz", "xy", "xz", "yz")),
+            p(m["dcrjx"]), p(m["dcrjy"]), p(m["dcrjz"]), p(m["d1"]), d.nxt, d.nyt, d.nzt, DH,
+            DT, NGSL + 2, NGSL + 1 + d.nxt, 0, 1)
+        box = (slice(NGSL + 2, NGSL + 2 + d.nxt), slice(NGSL + 2, NGSL + 2 + d.nyt),
+               slice(ALIGN + 3, ALIGN + d.nzt - 6))
+        out = {}
+        for n in ("u1", "v1", "w1"):
+            a, b = ft[n][box].astype(np.float64), fc[n][box].astype(np.float64)
+            assert np.abs(b).max() > 0, n
+            out[n] = np.abs(a - b).max() / np.abs(b).max()
+        return out
+
+    # density varying along y and z only: the two updates agree
+    mx = dict(m)
+    mx["d1"] = np.ascontiguousarray(np.broadcast_to(m["d1"][m["d1"].shape[0] // 2],
+                                                    m["d1"].shape))
+    for n, e in vel_increments(mx).items():
+        assert e < 1e-5, (n, e)
+    # density varying along x: v and w differ, because the curvilinear update
+    # averages density toward i-1 (optimized_velocity.cu:747-751) where
+    # dvelcx averages toward i+1 (kernel.cu:212-216); u averages along j and
+    # k in both and still agrees
+    e = vel_increments(m)
+    assert e["u1"] < 1e-5, e
+    assert e["v1"] > 1e-4 and e["w1"] > 1e-4, e
+
+
+def _dm_pair_workload(nt):
+    from awp_amd import configs
+    return configs.dm_plastic(nx=16, ny=16, nz=12, nz_fine=32, DH=300.0, DT=0.005, nt=nt,
+                              nsub=4, ND=4, M0=1.0e16, f0=5.0, src_depth=3, src_grid=1)
+
+
+def _x_uniform_density(media):
+    """Density of each grid made constant along x (its value in the middle
+    plane).  The curvilinear velocity update averages density toward i-1 for
+    v and w (optimized_velocity.cu:747-751) where dvelcx averages toward i+1
+    (kernel.cu:212-216), so a flat curvilinear grid equals the Cartesian one
+    only where density does not vary along x."""
+    for m in media:
+        d1 = m.d1
+        d1[...] = d1[d1.shape[0] // 2][None, :, :]
+    return media
+
+
+def test_flat_topography_over_dm_equals_cartesian_dm(oracle_lib):
+    """A flat curvilinear grid 0 over a 3x coarser Cartesian grid
+    (pmcl3d.c:1661-1694, 1843: the curvilinear updates on grid 0, dvelc2,
+    dstrqc2, intp3d and swap3 as on a Cartesian stack) advances like the
+    Cartesian two-grid stack.  From one random state, three steps of the
+    pmcl3d order: the coarse grid and the lowest 12 levels of grid 0, away
+    from the lateral edges, agree to single-precision rounding.  A grid 0 that
+    skipped its align+2 level (dvelc2 or dstrqc2) or coupled on another level
+    differs there by O(1)."""
+    from oracle.oracle import OracleSolver
+    w = _dm_pair_workload(3)
+    media = _x_uniform_density(w.medium())
+    d0 = media[0].dims
+    flat = Topography.flat(d0.nxt, d0.nyt)
+    ot = OracleSolver(media, w.sources, None, dtype=np.float32, topography=flat)
+    oc = OracleSolver(media, w.sources, None, dtype=np.float32)
+    rng = np.random.default_rng(5)
+    for p in range(2):
+        for n in FIELDS:
+            a = rng.standard_normal(ot.levels[p].f[n].shape).astype(np.float32)
+            ot.levels[p].f[n][...] = a
+            oc.levels[p].f[n][...] = a
+    ot.run(3)
+    oc.run(3)
+    m0, m1 = 14, 5  # lateral margins (cells) on grid 0 and grid 1
+    boxes = [(slice(NGSL + 2 + m0, NGSL + 2 + d0.nxt - m0),
+              slice(NGSL + 2 + m0, NGSL + 2 + d0.nyt - m0), slice(ALIGN, ALIGN + 12))]
+    d1 = media[1].dims
+    boxes.append((slice(NGSL + 2 + m1, NGSL + 2 + d1.nxt - m1),
+                  slice(NGSL + 2 + m1, NGSL + 2 + d1.nyt - m1), slice(ALIGN, ALIGN + d1.nzt)))
+    worst = 0.0
+    for p, box in enumerate(boxes):
+        for n in FIELDS:
+            a = ot.levels[p].f[n][box].astype(np.float64)
+            b = oc.levels[p].f[n][box].astype(np.float64)
+            e = np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+            worst = max(worst, e)
+            assert e < 1e-5, (p, n, e)
+    print(f"flat topography over DM vs Cartesian DM: worst relative difference {worst:.2e}")
+
+
+def test_topography_over_dm_planted_mistake_is_caught(oracle_lib):
+    """The comparison above fails for a grid 0 that skips dvelc2 on its
+    align+2 level: the planted mistake (an oracle subclass whose velocity_dm
+    does nothing on grid 0) moves the lowest levels by O(1)."""
+    from oracle.oracle import OracleSolver
+
+    class NoDvelc2(OracleSolver):
+        def velocity_dm(self, p):
+            if p != 0:
+                super().velocity_dm(p)
+
+    w = _dm_pair_workload(3)
+    media = _x_uniform_density(w.medium())
+    d0 = media[0].dims
+    flat = Topography.flat(d0.nxt, d0.nyt)
+    ot = NoDvelc2(media, w.sources, None, dtype=np.float32, topography=flat)
+    oc = OracleSolver(media, w.sources, None, dtype=np.float32)
+    rng = np.random.default_rng(5)
+    for p in range(2):
+        for n in FIELDS:
+            a = rng.standard_normal(ot.levels[p].f[n].shape).astype(np.float32)
+            ot.levels[p].f[n][...] = a
+            oc.levels[p].f[n][...] = a
+    ot.run(3)
+    oc.run(3)
+    box = (slice(NGSL + 16, NGSL + 2 + d0.nxt - 14), slice(NGSL + 16, NGSL + 2 + d0.nyt - 14),
+           slice(ALIGN + 2, ALIGN + 3))
+    a = ot.levels[0].f["u1"][box].astype(np.float64)
+    b = oc.levels[0].f["u1"][box].astype(np.float64)
+    assert np.abs(a - b).max() / np.abs(b).max() > 1e-3
