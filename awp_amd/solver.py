@@ -144,8 +144,11 @@ class Solver(PmclStepOrder):
         self.levels = [_Level(media[p], srcs[p], pls[p], self.device) for p in range(G)]
         # relaxation texture of the stress update: the per-cell arrays
         # (dstrqc_H), or the 8-entry table with the ww array ("table") or with
-        # init_texture's parity rule ("rule") through awp_dstrqc_tex
-        if texture not in ("arrays", "table", "rule"):
+        # init_texture's parity rule ("rule") through awp_dstrqc_tex; "auto"
+        # picks per grid the rule where ww follows it, else the table, else
+        # the arrays (a coarse grid's top levels carry the fine grid's
+        # texture, pmcl3d.c:1049-1058, so it breaks the rule there)
+        if texture not in ("arrays", "table", "rule", "auto"):
             raise ValueError(f"unknown texture mode {texture!r}")
         if texture != "arrays" and variant != "dstrqc":
             raise ValueError("the texture table applies to the dstrqc variant")
@@ -160,10 +163,27 @@ class Solver(PmclStepOrder):
         if self.fused and (G != 1 or plastic is not None or topography is not None
                            or variant != "dstrqc" or NPC != 0):
             raise ValueError("the fused step runs one Cartesian grid, NVE = 1, dstrqc, NPC = 0")
+        for L in self.levels:
+            L.texmode = "arrays"
         if texture != "arrays" or self.fused:
             from .media import texture_table
             for L in self.levels:
-                L.tex = (C.c_float * 24)(*texture_table(L.medium, rule=texture == "rule").tolist())
+                if texture == "auto":
+                    for mode in ("rule", "table"):
+                        try:
+                            tab = texture_table(L.medium, rule=mode == "rule")
+                        except ValueError:
+                            continue
+                        L.texmode = mode
+                        L.tex = (C.c_float * 24)(*tab.tolist())
+                        break
+                else:
+                    tab = texture_table(L.medium, rule=texture == "rule")
+                    # the fused step always takes the table, with ww unless the rule
+                    L.texmode = "table" if texture == "arrays" else texture
+                    L.tex = (C.c_float * 24)(*tab.tolist())
+            if self.fused and self.levels[0].texmode == "arrays":
+                raise ValueError("the fused step needs vx1, vx2, wwo as functions of ww")
         if self.fused:
             L = self.levels[0]
             self.f_next = {n: torch.empty_like(L.f[n]) for n in STEP_FIELDS}
@@ -315,14 +335,14 @@ class Solver(PmclStepOrder):
         d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:
             return
-        if self.texture != "arrays":
+        if L.texmode != "arrays":
             self.lib.awp_dstrqc_tex(
                 *(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
                                        "r5", "r6", "u1", "v1", "w1")),
                 _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["coeff"]), _ptr(m["qs"]),
                 _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nyt, d.nzt, self.stream,
                 _ptr(m["lam_mu"]), C.addressof(L.tex),
-                _ptr(m["ww"]) if self.texture == "table" else None,
+                _ptr(m["ww"]) if L.texmode == "table" else None,
                 d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, p)
             return
         fn = self.lib.dstrqc_H if self.variant == "dstrqc" else self.lib.dstrqc_H_new
@@ -434,7 +454,7 @@ class Solver(PmclStepOrder):
         r6 = (C.c_void_p * 6)(*[_ptr(f[n]) for n in ("r1", "r2", "r3", "r4", "r5", "r6")])
         med = (C.c_void_p * 5)(*[_ptr(m[n]) for n in ("d1", "lam", "mu", "qp", "qs")])
         self.lib.awp_step(a9, b9, r6, med, _ptr(m["coeff"]), C.addressof(L.tex),
-                          _ptr(m["ww"]) if self.texture != "rule" else None,
+                          _ptr(m["ww"]) if L.texmode != "rule" else None,
                           _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]),
                           _ptr(m["lam_mu"]), d.nxt, d.nyt, d.nzt, self.stream)
         for n in STEP_FIELDS:

@@ -100,7 +100,8 @@ def parse():
     ap.add_argument("--texture", choices=("arrays", "table", "rule"), default="rule",
                     help="relaxation texture of the FP32 stress update: per-cell vx1/vx2/wwo "
                          "arrays (dstrqc_H), or awp_dstrqc_tex's 8-entry table with the ww "
-                         "array (-12 B/cell) or with init_texture's parity rule (-16 B/cell)")
+                         "array (-12 B/cell) or with init_texture's parity rule (-16 B/cell); "
+                         "the dm workload takes rule per grid where ww follows it, else the table")
     ap.add_argument("--fused", action="store_true",
                     help="anelastic workload: the whole step in one sweep (awp_step, state A -> "
                          "state B, texture table; 144 B/cell, 140 B with --texture rule)")
@@ -723,7 +724,11 @@ def bench_dm(args) -> None:
     pm = w.params["plastic_fn"](media)
     setup_s = time.perf_counter() - t0
     print(f"dm setup {setup_s:.1f} s", file=sys.stderr, flush=True)
-    sol = Solver(media, w.sources, w.receivers, nt_record=nt + 1, plastic=pm)
+    # the coarse grid's top levels carry the fine grid's texture
+    # (pmcl3d.c:1049-1058), so the parity rule holds on the fine grid only:
+    # "auto" takes the rule there and the ww table on the coarse grid
+    tex = "auto" if args.texture == "rule" else args.texture
+    sol = Solver(media, w.sources, w.receivers, nt_record=nt + 1, plastic=pm, texture=tex)
     cells = sum(m.dims.ncells for m in media)
     del media, pm
     sol.start()
@@ -761,7 +766,8 @@ def bench_dm(args) -> None:
         "config": {"workload": w.name, "fine": [d0.nxt, d0.nyt, d0.nzt], "coarse": [d1.nxt, d1.nyt, d1.nzt],
                    "h_m": [w.params["DH"][0], w.params["DH"][1]], "dt_s": w.DT,
                    "plasticity": "Drucker-Prager c=5 MPa, mu=0.75, Tv=0.01 s",
-                   "source": "200 double-couple subfaults in the coarse grid"},
+                   "source": "200 double-couple subfaults in the coarse grid",
+                   "texture": [L.texmode for L in sol.levels]},
         "step_events_avg_ms": step_ms, "cells_per_step": cells,
         "traces_finite": bool(np.isfinite(tr).all()), "setup_s": setup_s,
         "roofline": None, "cpu_baseline": None,

@@ -209,17 +209,23 @@ def test_dm_solver_anelastic_matches_oracle(gpu, oracle_lib):
     assert err < 1e-4
 
 
-def test_dm_solver_plastic_matches_oracle(gpu, oracle_lib):
+@pytest.mark.parametrize("texture", ["arrays", "auto"])
+def test_dm_solver_plastic_matches_oracle(gpu, oracle_lib, texture):
     """Two grids, NVE = 3 (region split on both grids, yield-factor strips,
     plastic correction), source in the fine grid: bit-exact against the FP32
-    oracle, with yielding on the fine grid."""
+    oracle, with yielding on the fine grid.  texture "auto": the fine grid's
+    stress update evaluates init_texture's parity rule; the coarse grid, whose
+    top levels carry the fine grid's texture (pmcl3d.c:1049-1058), reads ww
+    and the 8-entry table."""
     from awp_amd.solver import Solver
     from oracle.oracle import OracleSolver
     nt = 50
     w = _dm_workload(0, nt, 1.0e17)
     media = w.medium()
     pm = w.params["plastic_fn"](media)
-    g = Solver(media, w.sources, w.receivers, nt_record=nt, plastic=pm)
+    g = Solver(media, w.sources, w.receivers, nt_record=nt, plastic=pm, texture=texture)
+    if texture == "auto":
+        assert [L.texmode for L in g.levels] == ["rule", "table"]
     g.run(nt, check_every=10)
     o32 = OracleSolver(media, w.sources, w.receivers, nt_record=nt, dtype=np.float32,
                        plastic=pm)
