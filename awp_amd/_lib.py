@@ -75,6 +75,7 @@ SIGNATURES: dict[str, list] = {
     "topo_stress_left_H": [_fp],
     "topo_stress_right_H": [_fp],
     "awp_topo_stress_interior_rates_H": [_fp, _fp],
+    "awp_topo_stress_interior_tex_H": [_fp, _fp, _fp, _i],
     "awp_topo_velocity_s16": [_fp, _i, _fp],
     "awp_topo_stress_s16": [_fp, _i, _fp, _fp],
     "awp_topo_geom_custom": [_fp, _i, _i, _i, _f, _s],

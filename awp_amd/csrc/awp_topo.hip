@@ -382,12 +382,14 @@ __device__ __forceinline__ float topo_qweight(float q, const float* coeff, int w
 // The material averages, Q weights and the memory-variable recursion, shared
 // by the interior and the top kernel (stress_attenuation.cu:200-327, 462-467,
 // 534-540, 595-600, 651-656).
+// f_vx1, f_vx2, f_wwo, f_ww: the cell's relaxation texture.
 template <class ST>
 __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int pos, int sl, int yl,
                                                 float f_dcrj, float nvs1, float nvs2,
                                                 float nvs3, float xyv1, float xyv2,
                                                 float xzv1, float xzv2, float yzv1,
-                                                float yzv2) {
+                                                float yzv2, float f_vx1, float f_vx2,
+                                                float f_wwo, int f_ww) {
   const int pos_ip1 = pos + sl, pos_jm1 = pos - yl, pos_km1 = pos - 1;
   const int pos_jk1 = pos - yl - 1, pos_ik1 = pos + sl - 1, pos_ijk = pos + sl - yl,
             pos_ijk1 = pos + sl - yl - 1;
@@ -395,8 +397,6 @@ __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int po
   const float* mu = A.mu;
   const float* qp = A.qp;
   const float* qs = A.qs;
-  const float f_vx1 = A.vx1[pos], f_vx2 = A.vx2[pos], f_wwo = A.wwo[pos];
-  const int f_ww = A.ww[pos];
   float xl = 8.0f / (lam[pos] + lam[pos_ip1] + lam[pos_jm1] + lam[pos_ijk] + lam[pos_km1] +
                      lam[pos_ik1] + lam[pos_jk1] + lam[pos_ijk1]);
   float xm = 16.0f / (mu[pos] + mu[pos_ip1] + mu[pos_jm1] + mu[pos_ijk] + mu[pos_km1] +
@@ -748,21 +748,39 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         for (int c = 0; c < 9; ++c) R[c * rstride + pos] = r9[c];
       } else {
         topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2,
-                            yzv1, yzv2);
+                            yzv1, yzv2, A.vx1[pos], A.vx2[pos], A.wwo[pos], A.ww[pos]);
       }
     }
   }
 }
+
+// The relaxation texture of the Q pass: the per-cell arrays (TOPO_TEX_ARRAYS),
+// or vx1, vx2, wwo from the 8-entry table indexed by ww, with ww read
+// (TOPO_TEX_TABLE) or evaluated from init_texture's parity rule
+// (TOPO_TEX_RULE, mesh.c:1103-1137) over the stress region that starts at
+// (xls, yls, align) and has nj columns per x plane and nzt levels.
+enum { TOPO_TEX_ARRAYS = 0, TOPO_TEX_TABLE = 1, TOPO_TEX_RULE = 2 };
+struct TopoTex {
+  float t[24];  // vx1[8], vx2[8], wwo[8]
+  int xls, yls, nj, nzt;
+};
 
 // The material averages, Q weights and memory-variable recursion of the
 // stress update (topo_str_update) over the cells topo_str_kernel<ST, true>
 // wrote rates for: levels 3 <= k < nz of the stress bounds.  One lane per
 // level, 4 rows per workgroup, a march over a chunk of i; few registers, so
 // the per-cell loads of this streaming pass stay in flight.
-template <class ST>
+template <class ST, int TEX>
 __global__ __launch_bounds__(256) void topo_str_q_kernel(const TopoStrArgs<ST> A, const TGeom G,
                                                          const float* __restrict__ R,
-                                                         long long rstride, int ichunk) {
+                                                         long long rstride, int ichunk,
+                                                         const TopoTex TT) {
+  __shared__ float s_tex[24];
+  if (TEX != TOPO_TEX_ARRAYS) {
+    const int tid = (int)(threadIdx.y * 64 + threadIdx.x);
+    if (tid < 24) s_tex[tid] = TT.t[tid];
+    __syncthreads();
+  }
   const int k = 3 + (int)(blockIdx.x * 64 + threadIdx.x);  // topographic level
   const int j = G.bj + (int)(blockIdx.y * 4 + threadIdx.y);
   const int ib = G.bi + (int)blockIdx.z * ichunk;
@@ -775,9 +793,25 @@ __global__ __launch_bounds__(256) void topo_str_q_kernel(const TopoStrArgs<ST> A
   for (int i = ib; i < ie; ++i) {
     const int pos = yl * j + K + sl * i;
     const float f_dcrj = A.dcrjx[i] * dcy * dcz;
+    float f_vx1, f_vx2, f_wwo;
+    int f_ww;
+    if (TEX == TOPO_TEX_ARRAYS) {
+      f_vx1 = A.vx1[pos]; f_vx2 = A.vx2[pos]; f_wwo = A.wwo[pos];
+      f_ww = A.ww[pos];
+    } else {
+      if (TEX == TOPO_TEX_TABLE) {
+        f_ww = A.ww[pos];
+      } else {
+        const int ni = i - TT.xls, col = ni * TT.nj + (j - TT.yls);
+        f_ww = 1 + 4 * ((ni + 1) & 1) + 2 * ((col + 1) & 1) + ((TT.nzt + col * TT.nzt + k) & 1);
+      }
+      const int t = (f_ww - 1) & 7;
+      f_vx1 = s_tex[t]; f_vx2 = s_tex[8 + t]; f_wwo = s_tex[16 + t];
+    }
     topo_str_update<ST>(A, pos, sl, yl, f_dcrj, R[pos], R[rstride + pos], R[2 * rstride + pos],
                         R[3 * rstride + pos], R[4 * rstride + pos], R[5 * rstride + pos],
-                        R[6 * rstride + pos], R[7 * rstride + pos], R[8 * rstride + pos]);
+                        R[6 * rstride + pos], R[7 * rstride + pos], R[8 * rstride + pos],
+                        f_vx1, f_vx2, f_wwo, f_ww);
   }
 }
 
@@ -1010,8 +1044,12 @@ void topo_velocity_launch(topo_t* T, const WaveScales& W, const char* where) {
   awp_check_launch(where);
 }
 
+// tex, texmode: the Q pass's relaxation texture (TOPO_TEX_*; tex holds
+// vx1[8], vx2[8], wwo[8] unless texmode is TOPO_TEX_ARRAYS, which the
+// one-pass update always uses).
 template <class ST>
-void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float* rates) {
+void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float* rates,
+                        const float* tex = nullptr, int texmode = TOPO_TEX_ARRAYS) {
   TGeom G;
   if (!topo_geom(T, where, &G)) return;
   if (!T->use) return;
@@ -1056,13 +1094,21 @@ void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float
   if (!metrics_present(A.M, where)) return;
   const void* need[] = {T->xx, T->yy, T->zz, T->xy, T->xz, T->yz, T->r1, T->r2, T->r3,
                         T->r4, T->r5, T->r6, T->u1, T->v1, T->w1, T->lami, T->mui,
-                        T->qpi, T->coeff, T->qsi, T->dcrjx, T->dcrjy, T->dcrjz, T->vx1,
-                        T->vx2, T->ww, T->wwo};
+                        T->qpi, T->coeff, T->qsi, T->dcrjx, T->dcrjy, T->dcrjz};
   for (const void* x : need)
     if (!x) {
       awp_set_error(AWP_ERR_ARG, "%s: a field pointer is NULL", where);
       return;
     }
+  if (texmode != TOPO_TEX_ARRAYS && (!rates || !tex)) {
+    awp_set_error(AWP_ERR_ARG, "%s: the texture table applies to the two-pass update", where);
+    return;
+  }
+  if ((texmode == TOPO_TEX_ARRAYS && (!T->vx1 || !T->vx2 || !T->ww || !T->wwo)) ||
+      (texmode == TOPO_TEX_TABLE && !T->ww)) {
+    awp_set_error(AWP_ERR_ARG, "%s: a relaxation-texture pointer is NULL", where);
+    return;
+  }
   // stress bounds (topography.c:377-420) shifted by ngsl+2 into array
   // indices (stress_attenuation.cu:34, 76-79)
   const int half = AWP_NGSL / 2, shift = AWP_NGSL + 2;
@@ -1095,8 +1141,24 @@ void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float
                      rstride);
   const int qchunk = awp_tuning().topo_q_ichunk;
   dim3 gq(awp_cdiv(T->nz - 3, 64), awp_cdiv(G.ej - G.bj, 4), awp_cdiv(G.ei - G.bi, qchunk));
-  hipLaunchKernelGGL(topo_str_q_kernel<ST>, gq, dim3(64, 4), 0, st, A, G, rates, rstride,
-                     qchunk);
+  TopoTex TT = {};
+  if (texmode != TOPO_TEX_ARRAYS) {
+    for (int m = 0; m < 24; ++m) TT.t[m] = tex[m];
+    // the stress region init_texture walks (mesh.c:1103-1137)
+    TT.xls = 2 + AWP_NGSL;
+    TT.yls = 2 + AWP_NGSL;
+    TT.nj = T->ny;
+    TT.nzt = T->nz;
+  }
+  if (texmode == TOPO_TEX_RULE)
+    hipLaunchKernelGGL((topo_str_q_kernel<ST, TOPO_TEX_RULE>), gq, dim3(64, 4), 0, st, A, G,
+                       rates, rstride, qchunk, TT);
+  else if (texmode == TOPO_TEX_TABLE)
+    hipLaunchKernelGGL((topo_str_q_kernel<ST, TOPO_TEX_TABLE>), gq, dim3(64, 4), 0, st, A, G,
+                       rates, rstride, qchunk, TT);
+  else
+    hipLaunchKernelGGL((topo_str_q_kernel<ST, TOPO_TEX_ARRAYS>), gq, dim3(64, 4), 0, st, A, G,
+                       rates, rstride, qchunk, TT);
   awp_check_launch(where);
 }
 
@@ -1151,6 +1213,16 @@ void awp_topo_stress_interior_rates_H(topo_t* T, float* rates) {
     return;
   }
   topo_stress_launch<StF32>(T, WaveScales{}, "awp_topo_stress_interior_rates_H", rates);
+}
+
+void awp_topo_stress_interior_tex_H(topo_t* T, float* rates, const float* tex, int rule) {
+  const char* W = "awp_topo_stress_interior_tex_H";
+  if (!rates || !tex) {
+    awp_set_error(AWP_ERR_ARG, "%s: rates and tex must not be NULL", W);
+    return;
+  }
+  topo_stress_launch<StF32>(T, WaveScales{}, W, rates, tex,
+                            rule ? TOPO_TEX_RULE : TOPO_TEX_TABLE);
 }
 
 void topo_stress_left_H(topo_t* T) {

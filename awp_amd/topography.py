@@ -159,7 +159,10 @@ class TopoDevice:
     """Metric arrays in HBM, built by the library's setup kernels, and the
     topo_t of one grid-0 level (level.f / level.m tensors).  two_pass: the
     stress update runs in two passes over nine strain-rate arrays
-    (awp_topo_stress_interior_rates_H), else in one (topo_stress_interior_H)."""
+    (awp_topo_stress_interior_rates_H), else in one (topo_stress_interior_H).
+    The two-pass update takes the level's relaxation texture as the solver set
+    it up (level.texmode "rule" or "table" with the 8-entry level.tex:
+    awp_topo_stress_interior_tex_H); the one-pass update reads the arrays."""
 
     def __init__(self, lib, level, topo: Topography, device, stream: int,
                  two_pass: bool = True):
@@ -214,7 +217,12 @@ class TopoDevice:
 
     def stress(self, stream: int) -> None:
         self.T.stream_i = stream
-        if self.rates is not None:
+        mode = getattr(self.level, "texmode", "arrays")
+        if self.rates is not None and mode in ("rule", "table"):
+            self.lib.awp_topo_stress_interior_tex_H(C.byref(self.T), self.rates.data_ptr(),
+                                                    C.addressof(self.level.tex),
+                                                    int(mode == "rule"))
+        elif self.rates is not None:
             self.lib.awp_topo_stress_interior_rates_H(C.byref(self.T), self.rates.data_ptr())
         else:
             self.lib.topo_stress_interior_H(C.byref(self.T))

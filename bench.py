@@ -628,8 +628,11 @@ def bench_topo(args) -> None:
     med = w.medium()
     setup_s = time.perf_counter() - t0
     if args.storage == "f32":
-        sol = Solver(med, w.sources, w.receivers, nt_record=nt + 1, topography=topo)
-        b_vel, b_str = BYTES_VEL, BYTES_STR
+        # the texture applies to the two-pass stress update's Q pass
+        # (awp_topo_stress_interior_tex_H)
+        sol = Solver(med, w.sources, w.receivers, nt_record=nt + 1, topography=topo,
+                     texture=args.texture)
+        b_vel, b_str = BYTES_VEL, BYTES_STR - TEXTURE_SAVES[args.texture]
     else:
         sol = Solver16(med, w.sources, w.receivers, nt_record=nt + 1, storage=args.storage,
                        topography=topo)
@@ -689,8 +692,13 @@ def bench_topo(args) -> None:
                    "h_m": w.DH, "dt_s": w.DT,
                    "topography": "6 seeded Gaussian hills, height <= 1.5 km, widths 2-8 km",
                    "medium": "layered vs 1500-4500 m/s + 5% von Karman, Q",
-                   "step": "topo velocity + stress (Q) on grid 0, addsrc_H, fstr_H, receivers"},
-        "roofline": {"bound": "hbm", "kernel": "topo_stress_interior_H", "achieved": str_gbs,
+                   "step": "topo velocity + stress (Q) on grid 0, addsrc_H, fstr_H, receivers",
+                   "texture": getattr(sol.levels[0], "texmode", "arrays")},
+        "roofline": {"bound": "hbm",
+                     "kernel": ("awp_topo_stress_interior_tex_H"
+                                if getattr(sol.levels[0], "texmode", "arrays") != "arrays"
+                                else "awp_topo_stress_interior_rates_H"),
+                     "achieved": str_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_cell": b_str,
                      "cells_per_launch": cells, "avg_launch_ms": str_avg},

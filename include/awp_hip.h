@@ -561,6 +561,13 @@ void topo_stress_right_H(topo_t* T);
  * back): the strain rates first, then the streaming material and memory-
  * variable update; the same results bit for bit, faster. */
 void awp_topo_stress_interior_rates_H(topo_t* T, float* rates);
+/* no reference counterpart: awp_topo_stress_interior_rates_H with the
+ * relaxation texture of awp_dstrqc_tex: tex (host, 24 floats: vx1[8],
+ * vx2[8], wwo[8]) indexed by ww - 1, ww read from T->ww (rule = 0) or from
+ * init_texture's parity rule over grid 0 (rule = 1, mesh.c:1103-1137; the
+ * vx1, vx2, wwo and ww pointers of T are then not read).  The same results
+ * bit for bit when the per-cell arrays are those functions of ww. */
+void awp_topo_stress_interior_tex_H(topo_t* T, float* rates, const float* tex, int rule);
 /* no reference counterpart: the same updates with 16-bit wavefield storage
  * (the wavefield pointers of T address uint16_t; fmt and scale as for
  * awp_dvelcx_s16; rates NULL for one pass, or the two-pass scratch) */

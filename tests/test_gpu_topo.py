@@ -100,19 +100,25 @@ def test_gpu_launcher_one_call_bit_exact(gpu, oracle_lib, which):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("two_pass", [True, False], ids=["two_pass", "one_pass"])
-def test_gpu_topography_run_bit_exact(gpu, oracle_lib, two_pass):
+@pytest.mark.parametrize("two_pass,texture", [(True, "arrays"), (False, "arrays"),
+                                              (True, "rule"), (True, "table")],
+                         ids=["two_pass", "one_pass", "two_pass_rule", "two_pass_table"])
+def test_gpu_topography_run_bit_exact(gpu, oracle_lib, two_pass, texture):
     """40 steps of the pmcl3d order with the curvilinear updates on grid 0
     (source, free-surface pass, receivers): wavefield and traces, with the
     stress update in one kernel and in two passes (strain rates, then the
-    material and memory-variable update)."""
+    material and memory-variable update), the second pass reading the
+    per-cell relaxation texture or the 8-entry table with ww read or
+    evaluated from init_texture's parity rule
+    (awp_topo_stress_interior_tex_H)."""
     torch, _lib = gpu
     from awp_amd.solver import FIELDS, Solver
     from oracle.oracle import OracleSolver
     w = _workload()
     med = w.medium()
     topo = w.params["topography"]
-    g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo, texture=texture)
+    assert g.levels[0].texmode == texture
     if not two_pass:
         g.topo.rates = None
     g.run(w.nt, check_every=10)
