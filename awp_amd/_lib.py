@@ -43,6 +43,7 @@ SIGNATURES: dict[str, list] = {
     "awp_dvelcx_dm": [_fp] * 13 + [_i, _i, _s, _i, _i, _i, _i],
     "dvelc2_H": [_fp] * 13 + [_i, _i, _s, _i],
     "dstrqc2_H": [_fp] * 22 + [_i, _i, _s] + [_fp] * 5 + [_i] * 5,
+    "awp_dstrqc2_tex": [_fp] * 22 + [_i, _i, _s] + [_fp] * 3 + [_i] * 5,
     "intp3d_H": [_fp] * 18 + [_i, _i, _i, _s, _i],
     "swap_H": [_fp] * 18 + [_i, _i] + [_fp] * 4 + [_i, _s, _i],
     "awp_dvelcx_s16": [_fp] * 13 + [_i, _i, _s, _i, _i, _i, _i, _i, _fp],

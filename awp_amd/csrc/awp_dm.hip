@@ -71,6 +71,21 @@ __device__ __forceinline__ float dm_qweight(float ca, float cb, float wwo, float
   return w / wwo;
 }
 
+// Relaxation texture of dstrqc2 (as in dstrqc_H's table forms): the
+// per-cell vx1, vx2, ww, wwo arrays (DM2_TEX_ARRAYS), or vx1, vx2, wwo from
+// the 8-entry table indexed by ww - 1 with ww read (DM2_TEX_TABLE) or from
+// init_texture's parity rule (DM2_TEX_RULE, mesh.c:1103-1137): ni = i - xls,
+// col = ni*nj + j - yls, ww = 1 + 4((ni+1)&1) + 2((col+1)&1) + ((nzt +
+// col*nzt + 2)&1) at this kernel's level k - align = 2.  On one fine column
+// the level touches a line of each array it reads, so each per-cell array
+// dropped saves a 128-byte line per cell.
+enum { DM2_TEX_ARRAYS = 0, DM2_TEX_TABLE = 1, DM2_TEX_RULE = 2 };
+struct Dm2Tex {
+  float t[24];  // vx1[8], vx2[8], wwo[8]
+  int xls, yls, nj, nzt;
+};
+
+template <int TEX>
 __global__ __launch_bounds__(256) void dstrqc2_kernel(
     float* __restrict__ xx, float* __restrict__ yy, float* __restrict__ zz,
     float* __restrict__ xy, float* __restrict__ xz, float* __restrict__ yz,
@@ -82,7 +97,13 @@ __global__ __launch_bounds__(256) void dstrqc2_kernel(
     const float* __restrict__ dcrjy, const float* __restrict__ dcrjz,
     const float* __restrict__ coeff, const float* __restrict__ vx1p,
     const float* __restrict__ vx2p, const int* __restrict__ wwp,
-    const float* __restrict__ wwop, const PlaneLaunch L) {
+    const float* __restrict__ wwop, const PlaneLaunch L, const Dm2Tex TT) {
+  __shared__ float s_tex[24];
+  if (TEX != DM2_TEX_ARRAYS) {
+    const int tid = (int)(threadIdx.y * 64 + threadIdx.x);
+    if (tid < 24) s_tex[tid] = TT.t[tid];
+    __syncthreads();
+  }
   const int j = L.j0 + (int)(blockIdx.x * 64 + threadIdx.x);
   const int i = L.i0 + (int)(blockIdx.y * 4 + threadIdx.y);
   if (j > L.j1 || i > L.i1) return;
@@ -93,8 +114,21 @@ __global__ __launch_bounds__(256) void dstrqc2_kernel(
                   jk1 = p - Y - 1, ijk1 = p + S - Y - 1;
   const float DT = L.DT, dth = L.dth, dh1 = L.dh1;
   const float f_dcrj = dcrjz[k] * dcrjx[i] * dcrjy[j];
-  const float f_vx1 = vx1p[p], f_vx2 = vx2p[p], f_wwo = wwop[p];
-  const int f_ww = wwp[p];
+  float f_vx1, f_vx2, f_wwo;
+  int f_ww;
+  if (TEX == DM2_TEX_ARRAYS) {
+    f_vx1 = vx1p[p]; f_vx2 = vx2p[p]; f_wwo = wwop[p];
+    f_ww = wwp[p];
+  } else {
+    if (TEX == DM2_TEX_TABLE) {
+      f_ww = wwp[p];
+    } else {
+      const int ni = i - TT.xls, col = ni * TT.nj + (j - TT.yls);
+      f_ww = 1 + 4 * ((ni + 1) & 1) + 2 * ((col + 1) & 1) + ((TT.nzt + col * TT.nzt + 2) & 1);
+    }
+    const int t = (f_ww - 1) & 7;
+    f_vx1 = s_tex[t]; f_vx2 = s_tex[8 + t]; f_wwo = s_tex[16 + t];
+  }
   float xl = awp_quot(8.0f, lam[p] + lam[ip1] + lam[jm1] + lam[ijk] + lam[km1] + lam[ik1] +
                                lam[jk1] + lam[ijk1]);
   float xm = awp_quot(16.0f, mu[p] + mu[ip1] + mu[jm1] + mu[ijk] + mu[km1] + mu[ik1] + mu[jk1] +
@@ -437,13 +471,16 @@ void dvelc2_H(float* u1, float* v1, float* w1, float* xx, float* yy, float* zz, 
   awp_check_launch(W);
 }
 
-void dstrqc2_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
-               float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
-               float* w1, float* lam, float* mu, float* qp, float* qs, float* dcrjx, float* dcrjy,
-               float* dcrjz, int nxt, int nyt, hipStream_t St, float* coeff, float* d_vx1,
-               float* d_vx2, int* d_ww, float* d_wwo, int s_i, int e_i, int s_j, int e_j,
-               int d_i) {
-  const char* W = "dstrqc2_H";
+}  // extern "C"
+
+namespace {
+
+void dstrqc2_common(const char* W, float* xx, float* yy, float* zz, float* xy, float* xz,
+                    float* yz, float* r1, float* r2, float* r3, float* r4, float* r5, float* r6,
+                    float* u1, float* v1, float* w1, float* lam, float* mu, float* qp, float* qs,
+                    float* dcrjx, float* dcrjy, float* dcrjz, int nxt, int nyt, hipStream_t St,
+                    float* coeff, float* d_vx1, float* d_vx2, int* d_ww, float* d_wwo, int s_i,
+                    int e_i, int s_j, int e_j, int d_i, int tex_mode, const float* tex) {
   if (!awp_grid_ok(d_i, W)) return;
   const AwpGrid& g = awp_state().grid[d_i];
   if (nxt != g.nxt || nyt != g.nyt) {
@@ -467,10 +504,64 @@ void dstrqc2_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
   L.yline = g.yline; L.slice = g.slice;
   L.dth = g.dth; L.dh1 = g.dh1; L.DT = g.DT;
   dim3 grid(awp_cdiv(e_j - s_j + 1, 64), awp_cdiv(e_i - s_i + 1, 4));
-  hipLaunchKernelGGL(dstrqc2_kernel, grid, dim3(64, 4), 0, St, xx, yy, zz, xy, xz, yz, r1, r2,
-                     r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, qs, dcrjx, dcrjy, dcrjz, coeff,
-                     d_vx1, d_vx2, d_ww, d_wwo, L);
+  Dm2Tex TT = {};
+  if (tex_mode != DM2_TEX_ARRAYS) {
+    for (int m = 0; m < 24; ++m) TT.t[m] = tex[m];
+    // init_texture's region on one rank: the interior columns, all levels
+    TT.xls = 2 + AWP_NGSL;
+    TT.yls = 2 + AWP_NGSL;
+    TT.nj = g.nyt;
+    TT.nzt = g.nzt;
+  }
+  if (tex_mode == DM2_TEX_RULE)
+    hipLaunchKernelGGL(dstrqc2_kernel<DM2_TEX_RULE>, grid, dim3(64, 4), 0, St, xx, yy, zz, xy,
+                       xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, qs, dcrjx, dcrjy,
+                       dcrjz, coeff, nullptr, nullptr, nullptr, nullptr, L, TT);
+  else if (tex_mode == DM2_TEX_TABLE)
+    hipLaunchKernelGGL(dstrqc2_kernel<DM2_TEX_TABLE>, grid, dim3(64, 4), 0, St, xx, yy, zz, xy,
+                       xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, qs, dcrjx, dcrjy,
+                       dcrjz, coeff, nullptr, nullptr, d_ww, nullptr, L, TT);
+  else
+    hipLaunchKernelGGL(dstrqc2_kernel<DM2_TEX_ARRAYS>, grid, dim3(64, 4), 0, St, xx, yy, zz, xy,
+                       xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, qs, dcrjx, dcrjy,
+                       dcrjz, coeff, d_vx1, d_vx2, d_ww, d_wwo, L, TT);
   awp_check_launch(W);
+}
+
+}  // namespace
+
+extern "C" {
+
+void dstrqc2_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz, float* r1,
+               float* r2, float* r3, float* r4, float* r5, float* r6, float* u1, float* v1,
+               float* w1, float* lam, float* mu, float* qp, float* qs, float* dcrjx, float* dcrjy,
+               float* dcrjz, int nxt, int nyt, hipStream_t St, float* coeff, float* d_vx1,
+               float* d_vx2, int* d_ww, float* d_wwo, int s_i, int e_i, int s_j, int e_j,
+               int d_i) {
+  const char* W = "dstrqc2_H";
+  if (!d_vx1 || !d_vx2 || !d_ww || !d_wwo) {
+    awp_set_error(AWP_ERR_ARG, "%s: a relaxation-texture pointer is NULL", W);
+    return;
+  }
+  dstrqc2_common(W, xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, qs,
+                 dcrjx, dcrjy, dcrjz, nxt, nyt, St, coeff, d_vx1, d_vx2, d_ww, d_wwo, s_i, e_i,
+                 s_j, e_j, d_i, DM2_TEX_ARRAYS, nullptr);
+}
+
+void awp_dstrqc2_tex(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                     float* r1, float* r2, float* r3, float* r4, float* r5, float* r6, float* u1,
+                     float* v1, float* w1, float* lam, float* mu, float* qp, float* qs,
+                     float* dcrjx, float* dcrjy, float* dcrjz, int nxt, int nyt, hipStream_t St,
+                     float* coeff, const float* tex, int* ww, int s_i, int e_i, int s_j, int e_j,
+                     int d_i) {
+  const char* W = "awp_dstrqc2_tex";
+  if (!tex) {
+    awp_set_error(AWP_ERR_ARG, "%s: the texture table is NULL", W);
+    return;
+  }
+  dstrqc2_common(W, xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1, lam, mu, qp, qs,
+                 dcrjx, dcrjy, dcrjz, nxt, nyt, St, coeff, nullptr, nullptr, ww, nullptr, s_i,
+                 e_i, s_j, e_j, d_i, ww ? DM2_TEX_TABLE : DM2_TEX_RULE, tex);
 }
 
 void intp3d_H(float* u1l, float* v1l, float* w1l, float* xxl, float* yyl, float* zzl, float* xyl,

@@ -110,9 +110,14 @@ struct VelTiles {
   int ichunk; // planes per march
 };
 
-template <class ST>
+// COMP: 0 updates u1, u2 and u3; 1, 2, 3 update only u1, u2 or u3 (the
+// other components' loads, LDS rows and arithmetic drop out), so three
+// launches with fewer live registers each give the same results.
+template <class ST, int COMP = 0>
 __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<ST> A,
                                                               const TGeom G, const VelTiles V) {
+  constexpr bool DU = COMP == 0 || COMP == 1, DV = COMP == 0 || COMP == 2,
+                 DW = COMP == 0 || COMP == 3;
   __shared__ float lds[2][VT_BJ][VT_NL][64];
   const int lane = (int)threadIdx.x;
   const int w = (int)threadIdx.y;
@@ -203,15 +208,21 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
     const float I13 = P4F(1) * s13x[1] + P4F(0) * s13x[0] + P4F(2) * s13x[2] + P4F(3) * s13x[3];
     const float I23 = P4F(2) * s23o + P4F(0) * s23m2 + P4F(1) * s23m1 + P4F(3) * s23p1;
     float(*L)[64] = lds[par][w];
-    L[0][lane] = I11;
-    L[1][lane] = I12;
-    L[2][lane] = I12x;
-    L[3][lane] = I22;
-    L[4][lane] = I13;
-    L[5][lane] = I23;
-    L[6][lane] = s13o;
-    L[7][lane] = s23o;
-    L[8][lane] = s33o;
+    if (DU) {
+      L[0][lane] = I11;
+      L[1][lane] = I12;
+      L[6][lane] = s13o;
+    }
+    if (DV) {
+      L[2][lane] = I12x;
+      L[3][lane] = I22;
+      L[7][lane] = s23o;
+    }
+    if (DW) {
+      L[4][lane] = I13;
+      L[5][lane] = I23;
+      L[8][lane] = s33o;
+    }
     // density at plane i
     const float r_k1 = A.rho[p - 1], r_k = A.rho[p], r_j = A.rho[p - yl],
                 r_jk1 = A.rho[p - yl - 1];
@@ -275,6 +286,7 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
       }
       float s;
       // u1
+      if (DU) {
       s = D4F(2) * M.f_c[q] * g3c * s11w[2];
       s = s + D4F(0) * M.f_c[q - 2 * fr] * g3c * s11w[0];
       s = s + D4F(1) * M.f_c[q - fr] * g3c * s11w[1];
@@ -296,7 +308,10 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
       s = s - M.f1_1[q] * z11;
       s = s - M.f2_1[q] * z12;
       const float v1 = (A.a * ST::ld(A.u1, p, A.su1) + Ai1 * s) * f_dcrj;
+      ST::st(A.u1, p, v1, A.iu1);
+      }
       // u2
+      if (DV) {
       if (!bnd) {
         s = D4F(2) * s23o;
         s = s + D4F(0) * L[7][lane - 2];
@@ -319,7 +334,10 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
       s = s - M.f1_2[q] * z12x;
       s = s - M.f2_2[q] * z22;
       const float v2 = (A.a * ST::ld(A.u2, p, A.su2) + Ai2 * s) * f_dcrj;
+      ST::st(A.u2, p, v2, A.iu2);
+      }
       // u3
+      if (DW) {
       s = D4F(2) * M.f_2[q] * g3v * s23o;
       s = s + D4F(0) * M.f_2[q - 2] * g3v * s23m2;
       s = s + D4F(1) * M.f_2[q - 1] * g3v * s23m1;
@@ -341,9 +359,8 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
       s = s - M.f1_c[q] * z13;
       s = s - M.f2_c[q] * z23;
       const float v3 = (A.a * ST::ld(A.u3, p, A.su3) + Ai3 * s) * f_dcrj;
-      ST::st(A.u1, p, v1, A.iu1);
-      ST::st(A.u2, p, v2, A.iu2);
       ST::st(A.u3, p, v3, A.iu3);
+      }
     }
     rm_k1 = r_k1;
     rm_k = r_k;
@@ -386,10 +403,9 @@ __device__ __forceinline__ float topo_qweight(float q, const float* coeff, int w
 template <class ST>
 __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int pos, int sl, int yl,
                                                 float f_dcrj, float nvs1, float nvs2,
-                                                float nvs3, float xyv1, float xyv2,
-                                                float xzv1, float xzv2, float yzv1,
-                                                float yzv2, float f_vx1, float f_vx2,
-                                                float f_wwo, int f_ww) {
+                                                float nvs3, float xyv, float xzv, float yzv,
+                                                float f_vx1, float f_vx2, float f_wwo,
+                                                int f_ww) {
   const int pos_ip1 = pos + sl, pos_jm1 = pos - yl, pos_km1 = pos - 1;
   const int pos_jk1 = pos - yl - 1, pos_ik1 = pos + sl - 1, pos_ijk = pos + sl - yl,
             pos_ijk1 = pos + sl - yl - 1;
@@ -468,22 +484,22 @@ __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int po
   ST::st(A.zz, pos, (sv + DT * f_rtmp) * f_dcrj, W.inv[AWP_SZZ]);
 
   f_r = ST::ld(A.r4, pos, W.s[AWP_SR4]);
-  f_rtmp = h1 * (xyv1 + xyv2);
-  sv = ST::ld(A.xy, pos, W.s[AWP_SXY]) + xmu1 * (xyv1 + xyv2) + vx1 * f_r;
+  f_rtmp = h1 * xyv;
+  sv = ST::ld(A.xy, pos, W.s[AWP_SXY]) + xmu1 * xyv + vx1 * f_r;
   ST::st(A.r4, pos, f_vx2 * f_r + f_wwo * f_rtmp, W.inv[AWP_SR4]);
   f_rtmp = f_rtmp * (f_wwo - 1) + f_vx2 * f_r * (1 - f_vx1);
   ST::st(A.xy, pos, (sv + DT * f_rtmp) * f_dcrj, W.inv[AWP_SXY]);
 
   f_r = ST::ld(A.r5, pos, W.s[AWP_SR5]);
-  f_rtmp = h2 * (xzv1 + xzv2);
-  sv = ST::ld(A.xz, pos, W.s[AWP_SXZ]) + xmu2 * (xzv1 + xzv2) + vx1 * f_r;
+  f_rtmp = h2 * xzv;
+  sv = ST::ld(A.xz, pos, W.s[AWP_SXZ]) + xmu2 * xzv + vx1 * f_r;
   ST::st(A.r5, pos, f_vx2 * f_r + f_wwo * f_rtmp, W.inv[AWP_SR5]);
   f_rtmp = f_rtmp * (f_wwo - 1.0f) + f_vx2 * f_r * (1.0f - f_vx1);
   ST::st(A.xz, pos, (sv + DT * f_rtmp) * f_dcrj, W.inv[AWP_SXZ]);
 
   f_r = ST::ld(A.r6, pos, W.s[AWP_SR6]);
-  f_rtmp = h3 * (yzv1 + yzv2);
-  sv = ST::ld(A.yz, pos, W.s[AWP_SYZ]) + xmu3 * (yzv1 + yzv2) + vx1 * f_r;
+  f_rtmp = h3 * yzv;
+  sv = ST::ld(A.yz, pos, W.s[AWP_SYZ]) + xmu3 * yzv + vx1 * f_r;
   ST::st(A.r6, pos, f_vx2 * f_r + f_wwo * f_rtmp, W.inv[AWP_SR6]);
   f_rtmp = f_rtmp * (f_wwo - 1.0f) + f_vx2 * f_r * (1.0f - f_vx1);
   ST::st(A.yz, pos, (sv + DT * f_rtmp) * f_dcrj, W.inv[AWP_SYZ]);
@@ -501,14 +517,15 @@ __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int po
 // read velocity columns that each plane stages in LDS (15 columns per wave).
 #define ST_BJ 4
 #define ST_NC 15
+#define TOPO_NRATES AWP_TOPO_NRATES
 enum {
   C_UN, C_UO, C_UM1, C_UP1, C_UP2,   // u1 at (i+2, j), (i, j), (i, j-1), (i, j+1), (i, j+2)
   C_VN, C_VO, C_VM2, C_VM1, C_VP1,   // v1 at (i+1, j), (i, j), (i, j-2), (i, j-1), (i, j+1)
   C_WN, C_WO, C_WM1, C_WP1, C_WP2    // w1 at (i+1, j), (i, j), (i, j-1), (i, j+1), (i, j+2)
 };
 
-// RATES: store the nine strain-rate terms of every output cell into R
-// (nine arrays of the field's size, rstride floats apart) for
+// RATES: store the six strain-rate terms of every output cell into R
+// (TOPO_NRATES arrays of the field's size, rstride floats apart) for
 // topo_str_q_kernel, instead of applying the material and memory-variable
 // update here.
 template <class ST, bool RATES>
@@ -742,13 +759,16 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
       acc = acc + P4F(3) * M.f2_c[q + 2] * zw_p2;
       const float yzv2 = D4F(1) * wo + D4F(0) * wm1 + D4F(2) * wp1 + D4F(3) * wp2 -
                          J23i * g_k * acc;
+      // the update uses the shear terms only as the sums xyv1 + xyv2,
+      // xzv1 + xzv2 and yzv1 + yzv2 (stress_attenuation.cu:306-327)
+      const float xyv = xyv1 + xyv2, xzv = xzv1 + xzv2, yzv = yzv1 + yzv2;
       if (RATES) {
-        const float r9[9] = {vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2, yzv1, yzv2};
+        const float r6[TOPO_NRATES] = {vs1, vs2, vs3, xyv, xzv, yzv};
 #pragma unroll
-        for (int c = 0; c < 9; ++c) R[c * rstride + pos] = r9[c];
+        for (int c = 0; c < TOPO_NRATES; ++c) R[c * rstride + pos] = r6[c];
       } else {
-        topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv1, xyv2, xzv1, xzv2,
-                            yzv1, yzv2, A.vx1[pos], A.vx2[pos], A.wwo[pos], A.ww[pos]);
+        topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv, xzv, yzv, A.vx1[pos],
+                            A.vx2[pos], A.wwo[pos], A.ww[pos]);
       }
     }
   }
@@ -810,7 +830,6 @@ __global__ __launch_bounds__(256) void topo_str_q_kernel(const TopoStrArgs<ST> A
     }
     topo_str_update<ST>(A, pos, sl, yl, f_dcrj, R[pos], R[rstride + pos], R[2 * rstride + pos],
                         R[3 * rstride + pos], R[4 * rstride + pos], R[5 * rstride + pos],
-                        R[6 * rstride + pos], R[7 * rstride + pos], R[8 * rstride + pos],
                         f_vx1, f_vx2, f_wwo, f_ww);
   }
 }
@@ -1040,7 +1059,13 @@ void topo_velocity_launch(topo_t* T, const WaveScales& W, const char* where) {
   V.ichunk = awp_tuning().topo_vel_ichunk;
   dim3 block(64, VT_BJ, 1);
   dim3 grid(V.nint + 1, awp_cdiv(T->ny, VT_BJ), awp_cdiv(T->nx, V.ichunk));
-  hipLaunchKernelGGL(topo_vel_kernel<ST>, grid, block, 0, st, A, G, V);
+  if (awp_tuning().topo_vel_split) {
+    hipLaunchKernelGGL((topo_vel_kernel<ST, 1>), grid, block, 0, st, A, G, V);
+    hipLaunchKernelGGL((topo_vel_kernel<ST, 2>), grid, block, 0, st, A, G, V);
+    hipLaunchKernelGGL((topo_vel_kernel<ST, 3>), grid, block, 0, st, A, G, V);
+  } else {
+    hipLaunchKernelGGL((topo_vel_kernel<ST, 0>), grid, block, 0, st, A, G, V);
+  }
   awp_check_launch(where);
 }
 

@@ -18,6 +18,7 @@ struct AwpTuning {
   int vel_jfast;        // 1: j tiles fastest in the block order (velocity)
   int str_jfast;        // same, stress
   int step_jfast;       // same, fused step
+  int topo_vel_split;   // 1: the curvilinear velocity update as one launch per component
 };
 
 AwpTuning& awp_tuning();

@@ -356,6 +356,15 @@ class Solver(PmclStepOrder):
     def stress_dm(self, p: int, s_i: int, e_i: int, s_j: int, e_j: int) -> None:
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
+        if L.texmode != "arrays":
+            self.lib.awp_dstrqc2_tex(
+                *(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                       "r5", "r6", "u1", "v1", "w1")),
+                _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["qs"]), _ptr(m["dcrjx"]),
+                _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nxt, d.nyt, self.stream, _ptr(m["coeff"]),
+                C.addressof(L.tex), _ptr(m["ww"]) if L.texmode == "table" else None,
+                s_i, e_i, s_j, e_j, p)
+            return
         self.lib.dstrqc2_H(*(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2",
                                                   "r3", "r4", "r5", "r6", "u1", "v1", "w1")),
                            _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["qs"]),

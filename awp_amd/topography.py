@@ -21,6 +21,8 @@ from .grid import ALIGN, NGSL
 from .refstructs import FGrid, GGrid, Topo
 
 # Metric array names (f_grid_t / g_grid_t members without their d_ prefix).
+# strain-rate arrays of the two-pass stress update (AWP_TOPO_NRATES)
+TOPO_NRATES = 6
 F_NAMES = ("f", "f_1", "f_2", "f_c", "f1_1", "f1_2", "f1_c", "f2_1", "f2_2", "f2_c")
 G_NAMES = ("g", "g3", "g3_c", "g_c")
 
@@ -158,7 +160,7 @@ _FIELDS = ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4", "r5", "r6
 class TopoDevice:
     """Metric arrays in HBM, built by the library's setup kernels, and the
     topo_t of one grid-0 level (level.f / level.m tensors).  two_pass: the
-    stress update runs in two passes over nine strain-rate arrays
+    stress update runs in two passes over six strain-rate arrays
     (awp_topo_stress_interior_rates_H), else in one (topo_stress_interior_H).
     The two-pass update takes the level's relaxation texture as the solver set
     it up (level.texmode "rule" or "table" with the 8-entry level.tex:
@@ -171,8 +173,10 @@ class TopoDevice:
         topo.check(d.nxt, d.nyt)
         self.lib = lib
         self.level = level
-        # scratch of the two-pass stress update: nine strain-rate arrays
-        self.rates = (torch.empty((9,) + tuple(d.shape), dtype=torch.float32, device=device)
+        # scratch of the two-pass stress update: AWP_TOPO_NRATES strain-rate
+        # arrays
+        self.rates = (torch.empty((TOPO_NRATES,) + tuple(d.shape), dtype=torch.float32,
+                                  device=device)
                       if two_pass else None)
         f32 = dict(dtype=torch.float32, device=device)
         self.mf = {n: torch.zeros(f_shape(d.nxt, d.nyt), **f32) for n in F_NAMES}

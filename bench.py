@@ -53,6 +53,9 @@ BYTES16_STR = 86.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch-shape knob for awp_set_tuning (repeatable); results do not "
+                         "depend on these")
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nx", type=int, default=None,
@@ -237,6 +240,11 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    for kv in args.tune:
+        key, _, val = kv.partition("=")
+        if _lib.load().awp_set_tuning(key.encode(), int(val)) != 0:
+            print(f"unknown tuning key or value: {kv}", file=sys.stderr)
+            sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return

@@ -198,6 +198,21 @@ void dstrqc2_H(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
                float* d_vx2, int* d_ww, float* d_wwo, int s_i, int e_i, int s_j, int e_j,
                int d_i);
 
+/* dstrqc2_H (reference src/awp/kernel.cu:2564-2784) with the relaxation
+ * texture as the table of awp_dstrqc_tex: tex (host, 24 floats: vx1[8],
+ * vx2[8], wwo[8]) indexed by ww - 1, ww read from the ww array or, with ww
+ * NULL, from init_texture's parity rule (src/awp/mesh.c:1103-1137) at this
+ * level.  Each fine column's level costs a memory line per array read, so
+ * the table saves three (ww given) or four lines per cell.  Results equal
+ * dstrqc2_H's bit for bit whenever the arrays dstrqc2_H would read hold
+ * tex[ww - 1]. */
+void awp_dstrqc2_tex(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
+                     float* r1, float* r2, float* r3, float* r4, float* r5, float* r6, float* u1,
+                     float* v1, float* w1, float* lam, float* mu, float* qp, float* qs,
+                     float* dcrjx, float* dcrjy, float* dcrjz, int nxt, int nyt, hipStream_t St,
+                     float* coeff, const float* tex, int* ww, int s_i, int e_i, int s_j, int e_j,
+                     int d_i);
+
 /* Replaces intp3d_H, include/awp/kernel_launch.h:105-109
  * (src/awp/kernel.cu:2359-2561, 2786-2804).  d_i is the coarse grid ("l"
  * arrays); the fine grid d_i-1 ("h" arrays) receives, at k = align+1, the
@@ -557,9 +572,10 @@ void topo_stress_interior_H(topo_t* T);
 void topo_stress_left_H(topo_t* T);
 void topo_stress_right_H(topo_t* T);
 /* no reference counterpart: topo_stress_interior_H in two passes over
- * caller-owned scratch `rates` (9 arrays of the 3-D field size, back to
- * back): the strain rates first, then the streaming material and memory-
- * variable update; the same results bit for bit, faster. */
+ * caller-owned scratch `rates` (AWP_TOPO_NRATES arrays of the 3-D field
+ * size, back to back): the strain rates first, then the streaming material
+ * and memory-variable update; the same results bit for bit, faster. */
+#define AWP_TOPO_NRATES 6
 void awp_topo_stress_interior_rates_H(topo_t* T, float* rates);
 /* no reference counterpart: awp_topo_stress_interior_rates_H with the
  * relaxation texture of awp_dstrqc_tex: tex (host, 24 floats: vx1[8],

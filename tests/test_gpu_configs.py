@@ -140,14 +140,18 @@ def test_anelastic_512_random_state(torch_gpu, oracle_lib, many_threads):
 
 def test_dm_plastic_full_size_random_state(torch_gpu, oracle_lib, many_threads):
     """C5 at 768x768x96 over 256x256x192 with Drucker-Prager plasticity:
-    three NVE = 3 steps from random wavefields on both grids."""
+    three NVE = 3 steps from random wavefields on both grids, with the
+    relaxation texture as the benchmark takes it (texture "auto": the parity
+    rule on the fine grid, dstrqc2 included, the ww table on the coarse
+    grid)."""
     from awp_amd.solver import Solver
     from oracle.oracle import OracleSolver
     w = configs.dm_plastic(nt=4)
     _say(f"case {w.name}: random state, 3 steps")
     media = w.medium()
     pm = w.params["plastic_fn"](media)
-    g = Solver(media, w.sources, w.receivers, nt_record=4, plastic=pm)
+    g = Solver(media, w.sources, w.receivers, nt_record=4, plastic=pm, texture="auto")
+    assert [L.texmode for L in g.levels] == ["rule", "table"]
     o = OracleSolver(media, w.sources, w.receivers, nt_record=4, dtype=np.float32, plastic=pm)
     del media, pm
     _seed_state(g, o, seed=768)

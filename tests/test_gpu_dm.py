@@ -209,14 +209,15 @@ def test_dm_solver_anelastic_matches_oracle(gpu, oracle_lib):
     assert err < 1e-4
 
 
-@pytest.mark.parametrize("texture", ["arrays", "auto"])
+@pytest.mark.parametrize("texture", ["arrays", "auto", "table"])
 def test_dm_solver_plastic_matches_oracle(gpu, oracle_lib, texture):
     """Two grids, NVE = 3 (region split on both grids, yield-factor strips,
     plastic correction), source in the fine grid: bit-exact against the FP32
     oracle, with yielding on the fine grid.  texture "auto": the fine grid's
-    stress update evaluates init_texture's parity rule; the coarse grid, whose
-    top levels carry the fine grid's texture (pmcl3d.c:1049-1058), reads ww
-    and the 8-entry table."""
+    stress updates (dstrqc_H's and dstrqc2_H's) evaluate init_texture's parity
+    rule; the coarse grid, whose top levels carry the fine grid's texture
+    (pmcl3d.c:1049-1058), reads ww and the 8-entry table.  "table": ww and the
+    table on both grids (awp_dstrqc_tex, awp_dstrqc2_tex)."""
     from awp_amd.solver import Solver
     from oracle.oracle import OracleSolver
     nt = 50
@@ -226,6 +227,8 @@ def test_dm_solver_plastic_matches_oracle(gpu, oracle_lib, texture):
     g = Solver(media, w.sources, w.receivers, nt_record=nt, plastic=pm, texture=texture)
     if texture == "auto":
         assert [L.texmode for L in g.levels] == ["rule", "table"]
+    if texture == "table":
+        assert [L.texmode for L in g.levels] == ["table", "table"]
     g.run(nt, check_every=10)
     o32 = OracleSolver(media, w.sources, w.receivers, nt_record=nt, dtype=np.float32,
                        plastic=pm)

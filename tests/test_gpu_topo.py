@@ -100,6 +100,34 @@ def test_gpu_launcher_one_call_bit_exact(gpu, oracle_lib, which):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("split", [1, 0])
+def test_gpu_topography_velocity_forms_bit_exact(gpu, oracle_lib, split):
+    """The curvilinear velocity update as one launch per component (the
+    default, awp_set_tuning("topo_vel_split", 1)) and as one launch for all
+    three: 40 steps bit-exact against the oracle."""
+    torch, _lib = gpu
+    from awp_amd.solver import FIELDS, Solver
+    from oracle.oracle import OracleSolver
+    lib = _lib.load()
+    w = _workload()
+    med = w.medium()
+    topo = w.params["topography"]
+    assert lib.awp_set_tuning(b"topo_vel_split", split) == 0
+    try:
+        g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+        g.run(w.nt, check_every=10)
+        torch.cuda.synchronize()
+    finally:
+        assert lib.awp_set_tuning(b"topo_vel_split", 1) == 0
+    o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    o.run(w.nt)
+    assert np.abs(o.traces()).max() > 0
+    assert np.array_equal(g.traces(), o.traces())
+    for n in FIELDS:
+        assert np.array_equal(g.field(n), o.f[n]), n
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("two_pass,texture", [(True, "arrays"), (False, "arrays"),
                                               (True, "rule"), (True, "table")],
                          ids=["two_pass", "one_pass", "two_pass_rule", "two_pass_table"])
