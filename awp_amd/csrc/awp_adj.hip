@@ -18,32 +18,73 @@
 #include <mutex>
 
 #include "awp_common.h"
+#include "awp_tuning.h"
 #include "../../include/awp_hip.h"
 
 namespace {
 
 struct AdjGrid {
   int nx, ny, nz, top, yline, lam_mu_ld;
+  int jgroup;  // block order: 0 plane by plane, else groups of jgroup j tiles over all i
   long long slice;
   float c1, c2, dth, dh1, DT;
 };
 
 // Block (x, y, z) of a (k tile, j tile, i plane) grid after the XCD-aware
-// renumbering (awp_xcd_remap): consecutive logical blocks, the k and j tiles
-// of one plane and then of the next, share one XCD's L2, so the j and i
-// neighbours the gathers read come from L2.
+// renumbering (awp_xcd_remap): consecutive logical blocks share one XCD's
+// L2, so the j and i neighbours the gathers read come from L2.  Logical
+// order: with G.jgroup = 0 the k and j tiles of one plane, then of the next;
+// with G.jgroup = JG > 0 the k tiles and JG j tiles of one plane, then of
+// the next plane, through all planes, then the next JG j tiles (the last
+// group takes the remaining tiles), so the blocks an XCD holds at once span
+// a few planes of a narrow j band.
 struct Blk3 {
   int x, y, z;
 };
-__device__ __forceinline__ Blk3 adj_block() {
-  const int nx = (int)gridDim.x, ny = (int)gridDim.y;
-  const int nb = nx * ny * (int)gridDim.z;
-  const int b = awp_xcd_remap((int)(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z)), nb);
-  return Blk3{b % nx, (b / nx) % ny, b / (nx * ny)};
+__device__ __forceinline__ Blk3 adj_block(const AdjGrid& G) {
+  const int nx = (int)gridDim.x, ny = (int)gridDim.y, nz = (int)gridDim.z;
+  const int nb = nx * ny * nz;
+  int b = awp_xcd_remap((int)(blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z)), nb);
+  if (G.jgroup <= 0 || G.jgroup >= ny) return Blk3{b % nx, (b / nx) % ny, b / (nx * ny)};
+  const int x = b % nx;
+  b /= nx;  // (j group, i, j tile in group), j tile fastest
+  const int JG = G.jgroup, full = ny / JG, per = JG * nz;
+  int y0, jg;
+  if (b < full * per) {
+    y0 = (b / per) * JG;
+    jg = JG;
+    b %= per;
+  } else {
+    y0 = full * JG;
+    jg = ny - y0;
+    b -= full * per;
+  }
+  return Blk3{x, y0 + b % jg, b / jg};
 }
 
 __device__ __forceinline__ long long aidx(const AdjGrid& G, int i, int j, int k) {
   return (long long)i * G.slice + (long long)j * G.yline + k;
+}
+
+// Cell of a launch over the free-surface band, levels [k0, k1] of the
+// interior columns: the band is a few levels deep, so the 256 lanes of a
+// block take nk = k1-k0+1 consecutive levels of 256/nk consecutive columns
+// (k fastest, then j) of one i plane, instead of one column per wave with
+// nk of its 64 lanes active.  Grid: (cdiv(nk * ny, 256), 1, nx).
+struct BandCell {
+  int i, j, k;
+  bool ok;
+};
+__device__ __forceinline__ BandCell band_cell(const AdjGrid& G, int k0, int k1) {
+  const Blk3 B = adj_block(G);
+  const int nk = k1 - k0 + 1;
+  const int t = B.x * 256 + (int)(threadIdx.y * 64 + threadIdx.x);
+  BandCell c;
+  c.j = 2 + AWP_NGSL + t / nk;
+  c.k = k0 + t % nk;
+  c.i = 2 + AWP_NGSL + B.z;
+  c.ok = c.j < G.ny + 2 + AWP_NGSL;
+  return c;
 }
 
 // Production Q weight (as the forward dstrqc).
@@ -129,7 +170,7 @@ __device__ __forceinline__ void vel_gather6(const CPtr6& Aw, long long q, const 
 __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es, CPtr3 V,
                                                                Medium M, Grad8 Gr,
                                                                const AdjGrid G, CPtr6 Awp) {
-  const Blk3 B = adj_block();
+  const Blk3 B = adj_block(G);
   const int k = AWP_ALIGN + 3 + (int)(B.x * 64 + threadIdx.x);
   const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
   const int i = 2 + AWP_NGSL + B.z;
@@ -261,13 +302,10 @@ __global__ __launch_bounds__(256) void adj_strain_gather_kernel(float* __restric
                                                                 float* __restrict__ wb,
                                                                 CPtr6 E, const AdjGrid G,
                                                                 int k0, int k1) {
-  const Blk3 B = adj_block();
-  const int k = k0 + (int)(B.x * 64 + threadIdx.x);
-  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
-  const int i = 2 + AWP_NGSL + B.z;
-  if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
+  const BandCell b = band_cell(G, k0, k1);
+  if (!b.ok) return;
   const long long S = G.slice, Y = G.yline;
-  const long long c = aidx(G, i, j, k);
+  const long long c = aidx(G, b.i, b.j, b.k);
   const Vec3 g = strain_gather(E, c, S, Y, G.c1, G.c2);
   ub[c] += g.u;
   vb[c] += g.v;
@@ -356,7 +394,8 @@ __global__ __launch_bounds__(256) void adj_ghost2_kernel(float* __restrict__ ub,
 // gradients with the stress differences of the state before the step.
 // GATHER: the strain -> velocity gather of the same cell first (levels that
 // the free-surface ghosts do not touch), so the velocity adjoints make one
-// trip through HBM instead of two.
+// trip through HBM instead of two; one column's levels per wave.  Without
+// GATHER the launch covers the free-surface band (band_cell's lane map).
 template <bool GATHER>
 __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ ub,
                                                             float* __restrict__ vb,
@@ -364,11 +403,20 @@ __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ 
                                                             CPtr6 Sg, Medium M, Grad8 Gr,
                                                             const AdjGrid G, CPtr6 E, int k0,
                                                             int k1) {
-  const Blk3 B = adj_block();
-  const int k = k0 + (int)(B.x * 64 + threadIdx.x);
-  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
-  const int i = 2 + AWP_NGSL + B.z;
-  if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
+  int i, j, k;
+  if constexpr (GATHER) {
+    const Blk3 B = adj_block(G);
+    k = k0 + (int)(B.x * 64 + threadIdx.x);
+    j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
+    i = 2 + AWP_NGSL + B.z;
+    if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
+  } else {
+    const BandCell b = band_cell(G, k0, k1);
+    if (!b.ok) return;
+    i = b.i;
+    j = b.j;
+    k = b.k;
+  }
   const long long S = G.slice, Y = G.yline;
   const long long p = aidx(G, i, j, k);
   const float* d1 = M.d1;
@@ -415,12 +463,9 @@ __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ 
 // band, which fstr^T touches, is gathered at once.
 __global__ __launch_bounds__(256) void adj_vel_gather_kernel(Adj15 Ab, CPtr6 Aw,
                                                              const AdjGrid G, int k0, int k1) {
-  const Blk3 B = adj_block();
-  const int k = k0 + (int)(B.x * 64 + threadIdx.x);
-  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
-  const int i = 2 + AWP_NGSL + B.z;
-  if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
-  const long long q = aidx(G, i, j, k);
+  const BandCell b = band_cell(G, k0, k1);
+  if (!b.ok) return;
+  const long long q = aidx(G, b.i, b.j, b.k);
   float g[6];
   vel_gather6(Aw, q, G, g);
 #pragma unroll
@@ -430,7 +475,7 @@ __global__ __launch_bounds__(256) void adj_vel_gather_kernel(Adj15 Ab, CPtr6 Aw,
 // The pending velocity^T gather after the last step (levels align+3 ..
 // top-3).
 __global__ __launch_bounds__(256) void adj_settle_kernel(Adj15 Ab, CPtr6 Aw, const AdjGrid G) {
-  const Blk3 B = adj_block();
+  const Blk3 B = adj_block(G);
   const int k = AWP_ALIGN + 3 + (int)(B.x * 64 + threadIdx.x);
   const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
   const int i = 2 + AWP_NGSL + B.z;
@@ -615,6 +660,7 @@ AdjGrid adj_grid(int d_i) {
   G.yline = g.yline; G.slice = g.slice;
   G.lam_mu_ld = g.nyt + 4 + AWP_NGSL2;
   G.c1 = g.c1; G.c2 = g.c2; G.dth = g.dth; G.dh1 = g.dh1; G.DT = g.DT;
+  G.jgroup = awp_tuning().adj_jgroup;
   return G;
 }
 
@@ -668,19 +714,20 @@ void awp_adjoint_step(float* const* adj, const float* const* fwd_v, const float*
   hipLaunchKernelGGL((adj_vel_local_kernel<true>),
                      dim3(awp_cdiv(nkf, 64), awp_cdiv(G.ny, 4), G.nx), dim3(64, 4), 0, St,
                      A.f[12], A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec, AWP_ALIGN, kf1);
-  // levels top-1 .. top+1: gather, ghosts^T, then velocity^T
-  hipLaunchKernelGGL(adj_strain_gather_kernel, dim3(1, awp_cdiv(G.ny, 4), G.nx), dim3(64, 4), 0,
-                     St, A.f[12], A.f[13], A.f[14], Ec, G, G.top - 1, G.top + 1);
+  // levels top-1 .. top+1: gather, ghosts^T, then velocity^T (band launches:
+  // (cdiv(nk * ny, 256), 1, nx) blocks for nk levels)
+  auto band = [&](int nk) { return dim3(awp_cdiv(nk * G.ny, 256), 1, G.nx); };
+  hipLaunchKernelGGL(adj_strain_gather_kernel, band(3), dim3(64, 4), 0, St, A.f[12], A.f[13],
+                     A.f[14], Ec, G, G.top - 1, G.top + 1);
   hipLaunchKernelGGL(adj_ghost1_kernel, g2, b2, 0, St, A.f[12], A.f[13], A.f[14], V, M.lam_mu,
                      col, grad_lam_mu, G);
   hipLaunchKernelGGL(adj_ghost2_kernel, g2, b2, 0, St, A.f[12], A.f[13], A.f[14], col, G);
-  hipLaunchKernelGGL((adj_vel_local_kernel<false>), dim3(1, awp_cdiv(G.ny, 4), G.nx),
-                     dim3(64, 4), 0, St, A.f[12], A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec,
-                     G.top - 1, G.top);
+  hipLaunchKernelGGL((adj_vel_local_kernel<false>), band(2), dim3(64, 4), 0, St, A.f[12],
+                     A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec, G.top - 1, G.top);
   // the free-surface band now; the levels below stay pending for the next
   // step (or awp_adjoint_settle)
-  hipLaunchKernelGGL(adj_vel_gather_kernel, dim3(1, awp_cdiv(G.ny, 4), G.nx), dim3(64, 4), 0,
-                     St, A, Awc, G, G.top - 2, G.top + 2);
+  hipLaunchKernelGGL(adj_vel_gather_kernel, band(5), dim3(64, 4), 0, St, A, Awc, G, G.top - 2,
+                     G.top + 2);
   awp_check_launch(W);
 }
 

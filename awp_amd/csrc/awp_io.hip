@@ -11,8 +11,9 @@
 // Defaults from sweeps on MI355X at 512x512x256: velocity 4 rows x 2 i-chunks
 // (all workgroups resident), stress 16 rows x ~32-plane i-chunks; the
 // curvilinear velocity update as three launches (0.985 against 1.043 ms for
-// one launch at 384x384x192).
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1};
+// one launch at 384x384x192); the adjoint kernels' blocks in groups of 8 j
+// tiles over all planes (adj_stress_local 551 -> 533 us at 256x256x128).
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -208,6 +209,9 @@ int awp_set_tuning(const char* key, int value) {
     t.step_jfast = value ? 1 : 0;
   } else if (!strcmp(key, "topo_vel_split")) {
     t.topo_vel_split = value ? 1 : 0;
+  } else if (!strcmp(key, "adj_jgroup")) {
+    if (value < 0) return -1;
+    t.adj_jgroup = value;
   } else if (!strcmp(key, "step_target_wgs")) {
     if (value < 1) return -1;
     t.step_target_wgs = value;

@@ -19,6 +19,7 @@ struct AwpTuning {
   int str_jfast;        // same, stress
   int step_jfast;       // same, fused step
   int topo_vel_split;   // 1: the curvilinear velocity update as one launch per component
+  int adj_jgroup;       // adjoint kernels' block order: 0 plane by plane, else j tiles per group
 };
 
 AwpTuning& awp_tuning();

@@ -207,3 +207,30 @@ def test_gpu_resident_tail_gives_the_same_gradient(gpu, case):
         assert np.array_equal(tr, out[0][1])
         for a, b in zip(g, out[0][2]):
             assert torch.equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("jgroup", [0, 3, 64])
+def test_gpu_block_order_gives_the_same_gradient(gpu, case, jgroup):
+    """The adjoint kernels' block order (awp_set_tuning("adj_jgroup"): plane
+    by plane, groups of 3 j tiles over all planes with a partial last group,
+    one group of every tile; the default is groups of 8) only changes which
+    blocks run together, not what any cell computes: misfit and gradients bit
+    for bit as the default."""
+    from awp_amd import _lib
+    lib = _lib.load()
+    bg = case[0]
+    out = []
+    for g in (None, jgroup):
+        if g is not None:
+            assert lib.awp_set_tuning(b"adj_jgroup", g) == 0
+        try:
+            P = _gpu_problem(case, 10)
+            P.set_medium(*_arrays(bg))
+            J = P.forward()
+            out.append((J, [x.cpu() for x in P.backward()]))
+        finally:
+            lib.awp_set_tuning(b"adj_jgroup", 8)
+    assert out[1][0] == out[0][0]
+    for a, b in zip(out[1][1], out[0][1]):
+        assert torch.equal(a, b)
