@@ -523,17 +523,41 @@ enum {
   C_VN, C_VO, C_VM2, C_VM1, C_VP1,   // v1 at (i+1, j), (i, j), (i, j-2), (i, j-1), (i, j+1)
   C_WN, C_WO, C_WM1, C_WP1, C_WP2    // w1 at (i+1, j), (i, j), (i, j-1), (i, j+1), (i, j+2)
 };
+// Rate groups of the two-pass update: 0 all six rates (and the one-pass
+// update), 1 the normal rates vs1, vs2, vs3, 2 the shear sums xy, xz, yz.
+// A group stages only the columns its z sums read, in LDS slots 0.. .
+constexpr unsigned st_group_cols(int group) {
+  return group == 1   ? (1u << C_UN) | (1u << C_VO) | (1u << C_VM2) | (1u << C_VM1) |
+                            (1u << C_VP1) | (1u << C_WO)
+         : group == 2 ? (1u << C_UO) | (1u << C_UM1) | (1u << C_UP1) | (1u << C_UP2) |
+                            (1u << C_VN) | (1u << C_VO) | (1u << C_WN) | (1u << C_WM1) |
+                            (1u << C_WP1) | (1u << C_WP2)
+                      : (1u << ST_NC) - 1u;
+}
+constexpr int st_group_ncols(int group) { return __builtin_popcount(st_group_cols(group)); }
+// LDS slot of column c in a group (the number of staged columns before it)
+constexpr int st_slot(int group, int c) {
+  return __builtin_popcount(st_group_cols(group) & ((1u << c) - 1u));
+}
+constexpr bool st_staged(int group, int c) { return (st_group_cols(group) >> c) & 1u; }
 
 // RATES: store the six strain-rate terms of every output cell into R
 // (TOPO_NRATES arrays of the field's size, rstride floats apart) for
 // topo_str_q_kernel, instead of applying the material and memory-variable
 // update here.
-template <class ST, bool RATES>
+// GROUP (RATES only): the rate group this launch computes and stores
+// (st_group_cols); two launches of groups 1 and 2 store what one of group 0
+// does, each with fewer live registers and LDS slots.
+template <class ST, bool RATES, int GROUP = 0>
 __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<ST> A,
                                                               const TGeom G, const VelTiles V,
                                                               float* __restrict__ R,
                                                               long long rstride) {
-  __shared__ float lds[2][ST_BJ][ST_NC][64];
+  static_assert(GROUP == 0 || RATES, "rate groups apply to the two-pass update");
+  constexpr bool GN = GROUP != 2, GS = GROUP != 1;  // normal rates, shear sums
+  constexpr int NC = st_group_ncols(GROUP);
+#define SC_(c) st_slot(GROUP, c)
+  __shared__ float lds[2][ST_BJ][NC][64];
   const int lane = (int)threadIdx.x;
   const int w = (int)threadIdx.y;
   const int t = (int)blockIdx.x;
@@ -604,13 +628,17 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     ux[q + 1] = A.u1[colj + sl * iu];
     vx[q + 1] = A.v1[colj + sl * iv];
     wx[q + 1] = A.w1[colj + sl * iv];
-    L[C_UN][lane] = ux[q + 1];
-    L[C_VN][lane] = vx[q + 1];
-    L[C_WN][lane] = wx[q + 1];
+    if constexpr (GN) L[SC_(C_UN)][lane] = ux[q + 1];
+    if constexpr (GS) {
+      L[SC_(C_VN)][lane] = vx[q + 1];
+      L[SC_(C_WN)][lane] = wx[q + 1];
+    }
     __syncthreads();
-    Zu[q + 1] = zu_x(L[C_UN]);
-    Zv[q + 1] = zv_x(L[C_VN]);
-    Zw[q + 1] = zw(L[C_WN]);
+    if constexpr (GN) Zu[q + 1] = zu_x(L[SC_(C_UN)]);
+    if constexpr (GS) {
+      Zv[q + 1] = zv_x(L[SC_(C_VN)]);
+      Zw[q + 1] = zw(L[SC_(C_WN)]);
+    }
   }
 
   const float g_ck = M.g_c[K], g_k = M.g[K], g3c_k = M.g3_c[K], g3_k = M.g3[K];
@@ -634,144 +662,177 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     ux[3] = A.u1[pos + 2 * sl];
     vx[3] = A.v1[pos + sl];
     wx[3] = A.w1[pos + sl];
-    const float um1 = A.u1[pos - yl], up1 = A.u1[pos + yl], up2 = A.u1[pos + 2 * yl];
-    const float vm2 = A.v1[pos - 2 * yl], vm1 = A.v1[pos - yl], vp1 = A.v1[pos + yl];
-    const float wm1 = A.w1[pos - yl], wp1 = A.w1[pos + yl], wp2 = A.w1[pos + 2 * yl];
+    float um1 = 0.f, up1 = 0.f, up2 = 0.f, vm2 = 0.f, vm1 = 0.f, vp1 = 0.f, wm1 = 0.f,
+          wp1 = 0.f, wp2 = 0.f;
+    if constexpr (GS) {
+      um1 = A.u1[pos - yl];
+      up1 = A.u1[pos + yl];
+      up2 = A.u1[pos + 2 * yl];
+      wm1 = A.w1[pos - yl];
+      wp1 = A.w1[pos + yl];
+      wp2 = A.w1[pos + 2 * yl];
+    }
+    if constexpr (GN) {
+      vm2 = A.v1[pos - 2 * yl];
+      vm1 = A.v1[pos - yl];
+      vp1 = A.v1[pos + yl];
+    }
     const float uo = ux[1], vo = vx[2], wo = wx[2];
-    L[C_UN][lane] = ux[3];
-    L[C_UO][lane] = uo;
-    L[C_UM1][lane] = um1;
-    L[C_UP1][lane] = up1;
-    L[C_UP2][lane] = up2;
-    L[C_VN][lane] = vx[3];
-    L[C_VO][lane] = vo;
-    L[C_VM2][lane] = vm2;
-    L[C_VM1][lane] = vm1;
-    L[C_VP1][lane] = vp1;
-    L[C_WN][lane] = wx[3];
-    L[C_WO][lane] = wo;
-    L[C_WM1][lane] = wm1;
-    L[C_WP1][lane] = wp1;
-    L[C_WP2][lane] = wp2;
+    auto stage = [&](int c, float x) {
+      // c is a literal column at every call, so the test folds away
+      if (st_staged(GROUP, c)) L[st_slot(GROUP, c)][lane] = x;
+    };
+    stage(C_UN, ux[3]);
+    stage(C_UO, uo);
+    stage(C_UM1, um1);
+    stage(C_UP1, up1);
+    stage(C_UP2, up2);
+    stage(C_VN, vx[3]);
+    stage(C_VO, vo);
+    stage(C_VM2, vm2);
+    stage(C_VM1, vm1);
+    stage(C_VP1, vp1);
+    stage(C_WN, wx[3]);
+    stage(C_WO, wo);
+    stage(C_WM1, wm1);
+    stage(C_WP1, wp1);
+    stage(C_WP2, wp2);
     __syncthreads();
-    Zu[3] = zu_x(L[C_UN]);
-    Zv[3] = zv_x(L[C_VN]);
-    Zw[3] = zw(L[C_WN]);
+    if constexpr (GN) Zu[3] = zu_x(L[SC_(C_UN)]);
+    if constexpr (GS) {
+      Zv[3] = zv_x(L[SC_(C_VN)]);
+      Zw[3] = zw(L[SC_(C_WN)]);
+    }
     if (out) {
       const int q = j + AWP_ALIGN + i * fr;
       const float f_dcrj = A.dcrjx[i] * dcy * dcz;
-      float acc;
-      // xx, yy, zz
-      float Jii = M.f_c[q] * g3c_k;
-      Jii = awp_quot(1.0f, Jii);
-      if (!bnd) {
-        acc = P4F(0) * M.f1_1[q - fr] * Zu[0];
-        acc = acc + P4F(1) * M.f1_1[q] * Zu[1];
-      } else {
-        acc = P4F(1) * M.f1_1[q] * Zu[1];
-        acc = acc + P4F(0) * M.f1_1[q - fr] * Zu[0];
+      float vs1 = 0.f, vs2 = 0.f, vs3 = 0.f, xyv = 0.f, xzv = 0.f, yzv = 0.f;
+      if constexpr (GN) {
+        float acc;
+        // xx, yy, zz
+        float Jii = M.f_c[q] * g3c_k;
+        Jii = awp_quot(1.0f, Jii);
+        if (!bnd) {
+          acc = P4F(0) * M.f1_1[q - fr] * Zu[0];
+          acc = acc + P4F(1) * M.f1_1[q] * Zu[1];
+        } else {
+          acc = P4F(1) * M.f1_1[q] * Zu[1];
+          acc = acc + P4F(0) * M.f1_1[q - fr] * Zu[0];
+        }
+        acc = acc + P4F(2) * M.f1_1[q + fr] * Zu[2];
+        acc = acc + P4F(3) * M.f1_1[q + 2 * fr] * Zu[3];
+        vs1 = D4F(1) * ux[1] + D4F(0) * ux[0] + D4F(2) * ux[2] + D4F(3) * ux[3] -
+                          Jii * g_ck * acc;
+        const float zv_o = bnd ? zrow(c_str_phdz4r, L[SC_(C_VO)]) : zasc(L[SC_(C_VO)]);
+        const float zv_m2 = bnd ? zrow(c_str_phdz4r, L[SC_(C_VM2)]) : zasc(L[SC_(C_VM2)]);
+        const float zv_m1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_VM1)]) : zasc(L[SC_(C_VM1)]);
+        const float zv_p1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_VP1)]) : zasc(L[SC_(C_VP1)]);
+        acc = P4F(2) * M.f2_2[q] * zv_o;
+        acc = acc + P4F(0) * M.f2_2[q - 2] * zv_m2;
+        acc = acc + P4F(1) * M.f2_2[q - 1] * zv_m1;
+        acc = acc + P4F(3) * M.f2_2[q + 1] * zv_p1;
+        vs2 = D4F(2) * vo + D4F(0) * vm2 + D4F(1) * vm1 + D4F(3) * vp1 -
+                          Jii * g_ck * acc;
+        if (!bnd) {
+          const float* c = L[SC_(C_WO)];
+          vs3 = Jii * (D4F(2) * c[lane] + D4F(0) * c[lane - 2] + D4F(1) * c[lane - 1] +
+                       D4F(3) * c[lane + 1]);
+        } else {
+          const float* c = L[SC_(C_WO)];
+          float zs = c_str_dhz4r.v[kb][7] * c[lb - 7];
+  #pragma unroll
+          for (int m = 6; m >= 0; --m) zs = zs + c_str_dhz4r.v[kb][m] * c[lb - m];
+          vs3 = Jii * zs;
+        }
       }
-      acc = acc + P4F(2) * M.f1_1[q + fr] * Zu[2];
-      acc = acc + P4F(3) * M.f1_1[q + 2 * fr] * Zu[3];
-      const float vs1 = D4F(1) * ux[1] + D4F(0) * ux[0] + D4F(2) * ux[2] + D4F(3) * ux[3] -
-                        Jii * g_ck * acc;
-      const float zv_o = bnd ? zrow(c_str_phdz4r, L[C_VO]) : zasc(L[C_VO]);
-      const float zv_m2 = bnd ? zrow(c_str_phdz4r, L[C_VM2]) : zasc(L[C_VM2]);
-      const float zv_m1 = bnd ? zrow(c_str_phdz4r, L[C_VM1]) : zasc(L[C_VM1]);
-      const float zv_p1 = bnd ? zrow(c_str_phdz4r, L[C_VP1]) : zasc(L[C_VP1]);
-      acc = P4F(2) * M.f2_2[q] * zv_o;
-      acc = acc + P4F(0) * M.f2_2[q - 2] * zv_m2;
-      acc = acc + P4F(1) * M.f2_2[q - 1] * zv_m1;
-      acc = acc + P4F(3) * M.f2_2[q + 1] * zv_p1;
-      const float vs2 = D4F(2) * vo + D4F(0) * vm2 + D4F(1) * vm1 + D4F(3) * vp1 -
-                        Jii * g_ck * acc;
-      float vs3;
-      if (!bnd) {
-        const float* c = L[C_WO];
-        vs3 = Jii * (D4F(2) * c[lane] + D4F(0) * c[lane - 2] + D4F(1) * c[lane - 1] +
-                     D4F(3) * c[lane + 1]);
-      } else {
-        const float* c = L[C_WO];
-        float zs = c_str_dhz4r.v[kb][7] * c[lb - 7];
-#pragma unroll
-        for (int m = 6; m >= 0; --m) zs = zs + c_str_dhz4r.v[kb][m] * c[lb - m];
-        vs3 = Jii * zs;
+      if constexpr (GS) {
+        float acc;
+        // xy
+        float J12i = M.f[q] * g3c_k6;
+        J12i = awp_quot(1.0f, J12i);
+        const float zu_o = bnd ? zrow(c_str_phdz4r, L[SC_(C_UO)]) : zmid(phdz4, L[SC_(C_UO)]);
+        const float zu_m1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_UM1)]) : zmid(phdz4, L[SC_(C_UM1)]);
+        const float zu_p1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_UP1)]) : zmid(phdz4, L[SC_(C_UP1)]);
+        const float zu_p2 = bnd ? zrow(c_str_phdz4r, L[SC_(C_UP2)]) : zmid(phdz4, L[SC_(C_UP2)]);
+        acc = P4F(1) * M.f2_1[q] * zu_o;
+        acc = acc + P4F(0) * M.f2_1[q - 1] * zu_m1;
+        acc = acc + P4F(2) * M.f2_1[q + 1] * zu_p1;
+        acc = acc + P4F(3) * M.f2_1[q + 2] * zu_p2;
+        const float xyv1 = D4F(1) * uo + D4F(0) * um1 + D4F(2) * up1 + D4F(3) * up2 -
+                           J12i * g_ck * acc;
+        acc = P4F(2) * M.f1_2[q] * Zv[2];
+        acc = acc + P4F(0) * M.f1_2[q - 2 * fr] * Zv[0];
+        acc = acc + P4F(1) * M.f1_2[q - fr] * Zv[1];
+        acc = acc + P4F(3) * M.f1_2[q + fr] * Zv[3];
+        const float xyv2 = D4F(2) * vx[2] + D4F(0) * vx[0] + D4F(1) * vx[1] + D4F(3) * vx[3] -
+                           J12i * g_ck * acc;
+        // xz
+        float J13i = M.f_1[q] * g3_k;
+        J13i = awp_quot(1.0f, J13i);
+        float xzv1;
+        if (!bnd) {
+          const float* c = L[SC_(C_UO)];
+          xzv1 = J13i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
+                         D4F(3) * c[lane + 2]);
+        } else {
+          const float* c = L[SC_(C_UO)];
+          float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
+  #pragma unroll
+          for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
+          xzv1 = J13i * zs;
+        }
+        acc = P4F(2) * M.f1_c[q] * Zw[2];
+        acc = acc + P4F(0) * M.f1_c[q - 2 * fr] * Zw[0];
+        acc = acc + P4F(1) * M.f1_c[q - fr] * Zw[1];
+        acc = acc + P4F(3) * M.f1_c[q + fr] * Zw[3];
+        const float xzv2 = D4F(2) * wx[2] + D4F(0) * wx[0] + D4F(1) * wx[1] + D4F(3) * wx[3] -
+                           J13i * g_k * acc;
+        // yz
+        float J23i = M.f_2[q] * g3_k;
+        J23i = awp_quot(1.0f, J23i);
+        float yzv1;
+        if (!bnd) {
+          const float* c = L[SC_(C_VO)];
+          yzv1 = J23i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
+                         D4F(3) * c[lane + 2]);
+        } else {
+          const float* c = L[SC_(C_VO)];
+          float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
+  #pragma unroll
+          for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
+          yzv1 = J23i * zs;
+        }
+        const float zw_m1 = zw(L[SC_(C_WM1)]), zw_p1 = zw(L[SC_(C_WP1)]), zw_p2 = zw(L[SC_(C_WP2)]);
+        acc = P4F(1) * M.f2_c[q] * Zw[2];
+        acc = acc + P4F(0) * M.f2_c[q - 1] * zw_m1;
+        acc = acc + P4F(2) * M.f2_c[q + 1] * zw_p1;
+        acc = acc + P4F(3) * M.f2_c[q + 2] * zw_p2;
+        const float yzv2 = D4F(1) * wo + D4F(0) * wm1 + D4F(2) * wp1 + D4F(3) * wp2 -
+                           J23i * g_k * acc;
+        // the update uses the shear terms only as the sums xyv1 + xyv2,
+        // xzv1 + xzv2 and yzv1 + yzv2 (stress_attenuation.cu:306-327)
+        xyv = xyv1 + xyv2;
+        xzv = xzv1 + xzv2;
+        yzv = yzv1 + yzv2;
       }
-      // xy
-      float J12i = M.f[q] * g3c_k6;
-      J12i = awp_quot(1.0f, J12i);
-      const float zu_o = bnd ? zrow(c_str_phdz4r, L[C_UO]) : zmid(phdz4, L[C_UO]);
-      const float zu_m1 = bnd ? zrow(c_str_phdz4r, L[C_UM1]) : zmid(phdz4, L[C_UM1]);
-      const float zu_p1 = bnd ? zrow(c_str_phdz4r, L[C_UP1]) : zmid(phdz4, L[C_UP1]);
-      const float zu_p2 = bnd ? zrow(c_str_phdz4r, L[C_UP2]) : zmid(phdz4, L[C_UP2]);
-      acc = P4F(1) * M.f2_1[q] * zu_o;
-      acc = acc + P4F(0) * M.f2_1[q - 1] * zu_m1;
-      acc = acc + P4F(2) * M.f2_1[q + 1] * zu_p1;
-      acc = acc + P4F(3) * M.f2_1[q + 2] * zu_p2;
-      const float xyv1 = D4F(1) * uo + D4F(0) * um1 + D4F(2) * up1 + D4F(3) * up2 -
-                         J12i * g_ck * acc;
-      acc = P4F(2) * M.f1_2[q] * Zv[2];
-      acc = acc + P4F(0) * M.f1_2[q - 2 * fr] * Zv[0];
-      acc = acc + P4F(1) * M.f1_2[q - fr] * Zv[1];
-      acc = acc + P4F(3) * M.f1_2[q + fr] * Zv[3];
-      const float xyv2 = D4F(2) * vx[2] + D4F(0) * vx[0] + D4F(1) * vx[1] + D4F(3) * vx[3] -
-                         J12i * g_ck * acc;
-      // xz
-      float J13i = M.f_1[q] * g3_k;
-      J13i = awp_quot(1.0f, J13i);
-      float xzv1;
-      if (!bnd) {
-        const float* c = L[C_UO];
-        xzv1 = J13i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
-                       D4F(3) * c[lane + 2]);
-      } else {
-        const float* c = L[C_UO];
-        float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
-#pragma unroll
-        for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
-        xzv1 = J13i * zs;
-      }
-      acc = P4F(2) * M.f1_c[q] * Zw[2];
-      acc = acc + P4F(0) * M.f1_c[q - 2 * fr] * Zw[0];
-      acc = acc + P4F(1) * M.f1_c[q - fr] * Zw[1];
-      acc = acc + P4F(3) * M.f1_c[q + fr] * Zw[3];
-      const float xzv2 = D4F(2) * wx[2] + D4F(0) * wx[0] + D4F(1) * wx[1] + D4F(3) * wx[3] -
-                         J13i * g_k * acc;
-      // yz
-      float J23i = M.f_2[q] * g3_k;
-      J23i = awp_quot(1.0f, J23i);
-      float yzv1;
-      if (!bnd) {
-        const float* c = L[C_VO];
-        yzv1 = J23i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
-                       D4F(3) * c[lane + 2]);
-      } else {
-        const float* c = L[C_VO];
-        float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
-#pragma unroll
-        for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
-        yzv1 = J23i * zs;
-      }
-      const float zw_m1 = zw(L[C_WM1]), zw_p1 = zw(L[C_WP1]), zw_p2 = zw(L[C_WP2]);
-      acc = P4F(1) * M.f2_c[q] * Zw[2];
-      acc = acc + P4F(0) * M.f2_c[q - 1] * zw_m1;
-      acc = acc + P4F(2) * M.f2_c[q + 1] * zw_p1;
-      acc = acc + P4F(3) * M.f2_c[q + 2] * zw_p2;
-      const float yzv2 = D4F(1) * wo + D4F(0) * wm1 + D4F(2) * wp1 + D4F(3) * wp2 -
-                         J23i * g_k * acc;
-      // the update uses the shear terms only as the sums xyv1 + xyv2,
-      // xzv1 + xzv2 and yzv1 + yzv2 (stress_attenuation.cu:306-327)
-      const float xyv = xyv1 + xyv2, xzv = xzv1 + xzv2, yzv = yzv1 + yzv2;
       if (RATES) {
-        const float r6[TOPO_NRATES] = {vs1, vs2, vs3, xyv, xzv, yzv};
-#pragma unroll
-        for (int c = 0; c < TOPO_NRATES; ++c) R[c * rstride + pos] = r6[c];
+        if constexpr (GN) {
+          R[pos] = vs1;
+          R[rstride + pos] = vs2;
+          R[2 * rstride + pos] = vs3;
+        }
+        if constexpr (GS) {
+          R[3 * rstride + pos] = xyv;
+          R[4 * rstride + pos] = xzv;
+          R[5 * rstride + pos] = yzv;
+        }
       } else {
         topo_str_update<ST>(A, pos, sl, yl, f_dcrj, vs1, vs2, vs3, xyv, xzv, yzv, A.vx1[pos],
                             A.vx2[pos], A.wwo[pos], A.ww[pos]);
       }
     }
   }
+#undef SC_
 }
 
 // The relaxation texture of the Q pass: the per-cell arrays (TOPO_TEX_ARRAYS),
@@ -1162,8 +1223,15 @@ void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float
   // two passes: strain rates into rates, then the material and memory-
   // variable update streaming them back
   const long long rstride = (long long)G.sl * (T->nx + 4 + AWP_NGSL2);
-  hipLaunchKernelGGL((topo_str_kernel<ST, true>), grid, block, 0, st, A, G, V, rates,
-                     rstride);
+  if (awp_tuning().topo_str_split) {
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 1>), grid, block, 0, st, A, G, V, rates,
+                       rstride);
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 2>), grid, block, 0, st, A, G, V, rates,
+                       rstride);
+  } else {
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 0>), grid, block, 0, st, A, G, V, rates,
+                       rstride);
+  }
   const int qchunk = awp_tuning().topo_q_ichunk;
   dim3 gq(awp_cdiv(T->nz - 3, 64), awp_cdiv(G.ej - G.bj, 4), awp_cdiv(G.ei - G.bi, qchunk));
   TopoTex TT = {};

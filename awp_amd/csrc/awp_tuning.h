@@ -20,6 +20,7 @@ struct AwpTuning {
   int step_jfast;       // same, fused step
   int topo_vel_split;   // 1: the curvilinear velocity update as one launch per component
   int adj_jgroup;       // adjoint kernels' block order: 0 plane by plane, else j tiles per group
+  int topo_str_split;   // 1: the strain-rate pass as two launches (normal rates, shear sums)
 };
 
 AwpTuning& awp_tuning();

@@ -128,6 +128,47 @@ def test_gpu_topography_velocity_forms_bit_exact(gpu, oracle_lib, split):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("split", [1, 0])
+def test_gpu_topography_rate_groups_bit_exact(gpu, oracle_lib, split):
+    """The strain-rate pass of the two-pass stress update as two launches
+    (normal rates, then shear sums: awp_set_tuning("topo_str_split", 1), the
+    default) and as one: 40 steps bit-exact against the oracle, FP32 and BF16
+    storage."""
+    torch, _lib = gpu
+    from awp_amd.solver import FIELDS, Solver, Solver16
+    from awp_amd.storage import FIELDS16
+    from oracle.oracle import OracleSolver, OracleSolver16
+    lib = _lib.load()
+    w = _workload()
+    med = w.medium()
+    topo = w.params["topography"]
+    default = 1
+    assert lib.awp_set_tuning(b"topo_str_split", split) == 0
+    try:
+        g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo, texture="rule")
+        g.run(w.nt, check_every=10)
+        g16 = Solver16(med, w.sources, w.receivers, nt_record=w.nt, storage="bf16",
+                       topography=topo)
+        g16.run(w.nt, check_every=10)
+        torch.cuda.synchronize()
+    finally:
+        assert lib.awp_set_tuning(b"topo_str_split", default) == 0
+    o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    o.run(w.nt)
+    assert np.abs(o.traces()).max() > 0
+    assert np.array_equal(g.traces(), o.traces())
+    for n in FIELDS:
+        assert np.array_equal(g.field(n), o.f[n]), n
+    o16 = OracleSolver16(med, w.sources, w.receivers, nt_record=w.nt, storage="bf16",
+                         topography=topo)
+    o16.run(w.nt)
+    assert g16.scales == o16.scales and g16.rescales == o16.rescales
+    assert np.array_equal(g16.traces(), o16.traces())
+    for n in FIELDS16:
+        assert np.array_equal(g16.raw(n), o16.levels[0].q[n]), n
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("two_pass,texture", [(True, "arrays"), (False, "arrays"),
                                               (True, "rule"), (True, "table")],
                          ids=["two_pass", "one_pass", "two_pass_rule", "two_pass_table"])
