@@ -11,10 +11,11 @@
 // Defaults from sweeps on MI355X at 512x512x256: velocity 4 rows x 2 i-chunks
 // (all workgroups resident), stress 16 rows x ~32-plane i-chunks; the
 // curvilinear velocity update as three launches (0.985 against 1.043 ms for
-// one launch at 384x384x192), its strain-rate pass as two (2.056 -> 2.006 ms
-// per stress update); the adjoint kernels' blocks in groups of 8 j tiles over
+// one launch at 384x384x192), its strain-rate pass as three (normal rates,
+// xy, xz and yz: 2.056 -> 2.006 -> 1.948 ms per stress update for one, two,
+// three launches); the adjoint kernels' blocks in groups of 8 j tiles over
 // all planes (adj_stress_local 551 -> 533 us at 256x256x128).
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 1};
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 2};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -211,7 +212,8 @@ int awp_set_tuning(const char* key, int value) {
   } else if (!strcmp(key, "topo_vel_split")) {
     t.topo_vel_split = value ? 1 : 0;
   } else if (!strcmp(key, "topo_str_split")) {
-    t.topo_str_split = value ? 1 : 0;
+    if (value < 0 || value > 2) return -1;
+    t.topo_str_split = value;
   } else if (!strcmp(key, "adj_jgroup")) {
     if (value < 0) return -1;
     t.adj_jgroup = value;

@@ -524,14 +524,19 @@ enum {
   C_WN, C_WO, C_WM1, C_WP1, C_WP2    // w1 at (i+1, j), (i, j), (i, j-1), (i, j+1), (i, j+2)
 };
 // Rate groups of the two-pass update: 0 all six rates (and the one-pass
-// update), 1 the normal rates vs1, vs2, vs3, 2 the shear sums xy, xz, yz.
-// A group stages only the columns its z sums read, in LDS slots 0.. .
+// update), 1 the normal rates vs1, vs2, vs3, 2 the shear sums xy, xz, yz,
+// 3 the xy sum, 4 the xz and yz sums.  A group stages only the columns its
+// z sums read, in LDS slots 0.. .
+constexpr unsigned ST_COLS_XY =
+    (1u << C_UO) | (1u << C_UM1) | (1u << C_UP1) | (1u << C_UP2) | (1u << C_VN);
+constexpr unsigned ST_COLS_XZYZ = (1u << C_UO) | (1u << C_VO) | (1u << C_WN) | (1u << C_WM1) |
+                                  (1u << C_WP1) | (1u << C_WP2);
 constexpr unsigned st_group_cols(int group) {
   return group == 1   ? (1u << C_UN) | (1u << C_VO) | (1u << C_VM2) | (1u << C_VM1) |
                             (1u << C_VP1) | (1u << C_WO)
-         : group == 2 ? (1u << C_UO) | (1u << C_UM1) | (1u << C_UP1) | (1u << C_UP2) |
-                            (1u << C_VN) | (1u << C_VO) | (1u << C_WN) | (1u << C_WM1) |
-                            (1u << C_WP1) | (1u << C_WP2)
+         : group == 2 ? ST_COLS_XY | ST_COLS_XZYZ
+         : group == 3 ? ST_COLS_XY
+         : group == 4 ? ST_COLS_XZYZ
                       : (1u << ST_NC) - 1u;
 }
 constexpr int st_group_ncols(int group) { return __builtin_popcount(st_group_cols(group)); }
@@ -546,15 +551,17 @@ constexpr bool st_staged(int group, int c) { return (st_group_cols(group) >> c) 
 // topo_str_q_kernel, instead of applying the material and memory-variable
 // update here.
 // GROUP (RATES only): the rate group this launch computes and stores
-// (st_group_cols); two launches of groups 1 and 2 store what one of group 0
-// does, each with fewer live registers and LDS slots.
+// (st_group_cols); launches of groups 1 and 2, or 1, 3 and 4, store what one
+// of group 0 does, each with fewer live registers and LDS slots.
 template <class ST, bool RATES, int GROUP = 0>
 __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<ST> A,
                                                               const TGeom G, const VelTiles V,
                                                               float* __restrict__ R,
                                                               long long rstride) {
   static_assert(GROUP == 0 || RATES, "rate groups apply to the two-pass update");
-  constexpr bool GN = GROUP != 2, GS = GROUP != 1;  // normal rates, shear sums
+  // normal rates, the xy sum, the xz and yz sums
+  constexpr bool GN = GROUP <= 1, GXY = GROUP != 1 && GROUP != 4,
+                 GZ = GROUP != 1 && GROUP != 3;
   constexpr int NC = st_group_ncols(GROUP);
 #define SC_(c) st_slot(GROUP, c)
   __shared__ float lds[2][ST_BJ][NC][64];
@@ -629,16 +636,12 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     vx[q + 1] = A.v1[colj + sl * iv];
     wx[q + 1] = A.w1[colj + sl * iv];
     if constexpr (GN) L[SC_(C_UN)][lane] = ux[q + 1];
-    if constexpr (GS) {
-      L[SC_(C_VN)][lane] = vx[q + 1];
-      L[SC_(C_WN)][lane] = wx[q + 1];
-    }
+    if constexpr (GXY) L[SC_(C_VN)][lane] = vx[q + 1];
+    if constexpr (GZ) L[SC_(C_WN)][lane] = wx[q + 1];
     __syncthreads();
     if constexpr (GN) Zu[q + 1] = zu_x(L[SC_(C_UN)]);
-    if constexpr (GS) {
-      Zv[q + 1] = zv_x(L[SC_(C_VN)]);
-      Zw[q + 1] = zw(L[SC_(C_WN)]);
-    }
+    if constexpr (GXY) Zv[q + 1] = zv_x(L[SC_(C_VN)]);
+    if constexpr (GZ) Zw[q + 1] = zw(L[SC_(C_WN)]);
   }
 
   const float g_ck = M.g_c[K], g_k = M.g[K], g3c_k = M.g3_c[K], g3_k = M.g3[K];
@@ -664,10 +667,12 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     wx[3] = A.w1[pos + sl];
     float um1 = 0.f, up1 = 0.f, up2 = 0.f, vm2 = 0.f, vm1 = 0.f, vp1 = 0.f, wm1 = 0.f,
           wp1 = 0.f, wp2 = 0.f;
-    if constexpr (GS) {
+    if constexpr (GXY) {
       um1 = A.u1[pos - yl];
       up1 = A.u1[pos + yl];
       up2 = A.u1[pos + 2 * yl];
+    }
+    if constexpr (GZ) {
       wm1 = A.w1[pos - yl];
       wp1 = A.w1[pos + yl];
       wp2 = A.w1[pos + 2 * yl];
@@ -699,10 +704,8 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     stage(C_WP2, wp2);
     __syncthreads();
     if constexpr (GN) Zu[3] = zu_x(L[SC_(C_UN)]);
-    if constexpr (GS) {
-      Zv[3] = zv_x(L[SC_(C_VN)]);
-      Zw[3] = zw(L[SC_(C_WN)]);
-    }
+    if constexpr (GXY) Zv[3] = zv_x(L[SC_(C_VN)]);
+    if constexpr (GZ) Zw[3] = zw(L[SC_(C_WN)]);
     if (out) {
       const int q = j + AWP_ALIGN + i * fr;
       const float f_dcrj = A.dcrjx[i] * dcy * dcz;
@@ -745,7 +748,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
           vs3 = Jii * zs;
         }
       }
-      if constexpr (GS) {
+      if constexpr (GXY) {
         float acc;
         // xy
         float J12i = M.f[q] * g3c_k6;
@@ -766,6 +769,12 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         acc = acc + P4F(3) * M.f1_2[q + fr] * Zv[3];
         const float xyv2 = D4F(2) * vx[2] + D4F(0) * vx[0] + D4F(1) * vx[1] + D4F(3) * vx[3] -
                            J12i * g_ck * acc;
+        // the update uses the shear terms only as the sums xyv1 + xyv2,
+        // xzv1 + xzv2 and yzv1 + yzv2 (stress_attenuation.cu:306-327)
+        xyv = xyv1 + xyv2;
+      }
+      if constexpr (GZ) {
+        float acc;
         // xz
         float J13i = M.f_1[q] * g3_k;
         J13i = awp_quot(1.0f, J13i);
@@ -809,9 +818,6 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         acc = acc + P4F(3) * M.f2_c[q + 2] * zw_p2;
         const float yzv2 = D4F(1) * wo + D4F(0) * wm1 + D4F(2) * wp1 + D4F(3) * wp2 -
                            J23i * g_k * acc;
-        // the update uses the shear terms only as the sums xyv1 + xyv2,
-        // xzv1 + xzv2 and yzv1 + yzv2 (stress_attenuation.cu:306-327)
-        xyv = xyv1 + xyv2;
         xzv = xzv1 + xzv2;
         yzv = yzv1 + yzv2;
       }
@@ -821,8 +827,8 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
           R[rstride + pos] = vs2;
           R[2 * rstride + pos] = vs3;
         }
-        if constexpr (GS) {
-          R[3 * rstride + pos] = xyv;
+        if constexpr (GXY) R[3 * rstride + pos] = xyv;
+        if constexpr (GZ) {
           R[4 * rstride + pos] = xzv;
           R[5 * rstride + pos] = yzv;
         }
@@ -1223,7 +1229,14 @@ void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float
   // two passes: strain rates into rates, then the material and memory-
   // variable update streaming them back
   const long long rstride = (long long)G.sl * (T->nx + 4 + AWP_NGSL2);
-  if (awp_tuning().topo_str_split) {
+  if (awp_tuning().topo_str_split == 2) {
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 1>), grid, block, 0, st, A, G, V, rates,
+                       rstride);
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 3>), grid, block, 0, st, A, G, V, rates,
+                       rstride);
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 4>), grid, block, 0, st, A, G, V, rates,
+                       rstride);
+  } else if (awp_tuning().topo_str_split) {
     hipLaunchKernelGGL((topo_str_kernel<ST, true, 1>), grid, block, 0, st, A, G, V, rates,
                        rstride);
     hipLaunchKernelGGL((topo_str_kernel<ST, true, 2>), grid, block, 0, st, A, G, V, rates,

@@ -20,7 +20,7 @@ struct AwpTuning {
   int step_jfast;       // same, fused step
   int topo_vel_split;   // 1: the curvilinear velocity update as one launch per component
   int adj_jgroup;       // adjoint kernels' block order: 0 plane by plane, else j tiles per group
-  int topo_str_split;   // 1: the strain-rate pass as two launches (normal rates, shear sums)
+  int topo_str_split;   // strain-rate pass: 0 one launch, 1 two (normal, shear), 2 three (normal, xy, xz+yz)
 };
 
 AwpTuning& awp_tuning();

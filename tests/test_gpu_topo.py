@@ -128,12 +128,12 @@ def test_gpu_topography_velocity_forms_bit_exact(gpu, oracle_lib, split):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("split", [1, 0])
+@pytest.mark.parametrize("split", [1, 2, 0])
 def test_gpu_topography_rate_groups_bit_exact(gpu, oracle_lib, split):
     """The strain-rate pass of the two-pass stress update as two launches
-    (normal rates, then shear sums: awp_set_tuning("topo_str_split", 1), the
-    default) and as one: 40 steps bit-exact against the oracle, FP32 and BF16
-    storage."""
+    (normal rates, then shear sums: awp_set_tuning("topo_str_split", 1)), as
+    three (normal rates, the xy sum, the xz and yz sums: 2, the default) and
+    as one: 40 steps bit-exact against the oracle, FP32 and BF16 storage."""
     torch, _lib = gpu
     from awp_amd.solver import FIELDS, Solver, Solver16
     from awp_amd.storage import FIELDS16
@@ -142,7 +142,7 @@ def test_gpu_topography_rate_groups_bit_exact(gpu, oracle_lib, split):
     w = _workload()
     med = w.medium()
     topo = w.params["topography"]
-    default = 1
+    default = 2
     assert lib.awp_set_tuning(b"topo_str_split", split) == 0
     try:
         g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo, texture="rule")
