@@ -170,6 +170,11 @@ __device__ __forceinline__ void vel_gather6(const CPtr6& Aw, long long q, const 
 __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es, CPtr3 V,
                                                                Medium M, Grad8 Gr,
                                                                const AdjGrid G, CPtr6 Awp) {
+  // the Q coefficients by mechanism, from LDS (the lookup by ww would
+  // otherwise be a second dependent round trip to memory)
+  __shared__ float s_coeff[16];
+  if (threadIdx.y == 0 && threadIdx.x < 16) s_coeff[threadIdx.x] = M.coeff[threadIdx.x];
+  __syncthreads();
   const Blk3 B = adj_block(G);
   const int k = AWP_ALIGN + 3 + (int)(B.x * 64 + threadIdx.x);
   const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
@@ -179,28 +184,93 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
   const long long p = aidx(G, i, j, k);
   const long long ip1 = p + S, jm1 = p - Y, ijk = p + S - Y, km1 = p - 1, ik1 = p + S - 1,
                   jk1 = p - Y - 1, ijk1 = p + S - Y - 1;
+  const bool has_pend = k <= G.top - 3;
+
+  // Every load whose address is known up front, issued together before any
+  // arithmetic (the scheduling barrier keeps the compiler from sinking them
+  // to their uses), so a wave waits for one round trip instead of three.
   const float f_vx1 = M.vx1[p], f_vx2 = M.vx2[p], f_wwo = M.wwo[p];
   const int f_ww = M.ww[p];
-  const float ca = M.coeff[(f_ww * 2 - 2) & 15], cb = M.coeff[(f_ww * 2 - 1) & 15];
-  const float dcrj = M.dcrjx[i] * M.dcrjy[j] * M.dcrjz[k];
-  const float sl8 = M.lam[p] + M.lam[ip1] + M.lam[jm1] + M.lam[ijk] + M.lam[km1] + M.lam[ik1] +
-                    M.lam[jk1] + M.lam[ijk1];
-  const float sm8 = M.mu[p] + M.mu[ip1] + M.mu[jm1] + M.mu[ijk] + M.mu[km1] + M.mu[ik1] +
-                    M.mu[jk1] + M.mu[ijk1];
-  const float sp8 = M.qp[p] + M.qp[ip1] + M.qp[jm1] + M.qp[ijk] + M.qp[km1] + M.qp[ik1] +
-                    M.qp[jk1] + M.qp[ijk1];
-  const float ss8 = M.qs[p] + M.qs[ip1] + M.qs[jm1] + M.qs[ijk] + M.qs[km1] + M.qs[ik1] +
-                    M.qs[jk1] + M.qs[ijk1];
+  const float l0 = M.lam[p], l1 = M.lam[ip1], l2 = M.lam[jm1], l3 = M.lam[ijk], l4 = M.lam[km1],
+              l5 = M.lam[ik1], l6 = M.lam[jk1], l7 = M.lam[ijk1];
+  const float m0 = M.mu[p], m1 = M.mu[ip1], m2 = M.mu[jm1], m3 = M.mu[ijk], m4 = M.mu[km1],
+              m5 = M.mu[ik1], m6 = M.mu[jk1], m7 = M.mu[ijk1];
+  const float p0 = M.qp[p], p1 = M.qp[ip1], p2 = M.qp[jm1], p3 = M.qp[ijk], p4 = M.qp[km1],
+              p5 = M.qp[ik1], p6 = M.qp[jk1], p7 = M.qp[ijk1];
+  const float q0 = M.qs[p], q1 = M.qs[ip1], q2 = M.qs[jm1], q3 = M.qs[ijk], q4 = M.qs[km1],
+              q5 = M.qs[ik1], q6 = M.qs[jk1], q7 = M.qs[ijk1];
+  const float dcx = M.dcrjx[i], dcy = M.dcrjy[j], dcz = M.dcrjz[k];
+  const float* u = V.f[0];
+  const float* v = V.f[1];
+  const float* w = V.f[2];
+  const float u_0 = u[p], u_is = u[p + S], u_i2 = u[p + 2 * S], u_im = u[p - S];
+  const float u_jp = u[p + Y], u_j2 = u[p + 2 * Y], u_jm = u[p - Y];
+  const float u_kp = u[p + 1], u_k2 = u[p + 2], u_km = u[p - 1];
+  const float v_0 = v[p], v_jm = v[p - Y], v_jp = v[p + Y], v_j2m = v[p - 2 * Y];
+  const float v_im = v[p - S], v_ip = v[p + S], v_i2m = v[p - 2 * S];
+  const float v_kp = v[p + 1], v_k2 = v[p + 2], v_km = v[p - 1];
+  const float w_0 = w[p], w_km = w[p - 1], w_kp = w[p + 1], w_k2m = w[p - 2];
+  const float w_im = w[p - S], w_ip = w[p + S], w_i2m = w[p - 2 * S];
+  const float w_jp = w[p + Y], w_j2 = w[p + 2 * Y], w_jm = w[p - Y];
+  float sbl[6], rbl[6], pend[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    sbl[c] = A.f[c][p];
+    rbl[c] = A.f[6 + c][p];
+  }
+  // the pending velocity^T gather (vel_gather6's operands)
+  float au0 = 0.f, au_is = 0.f, au_im = 0.f, au_i2 = 0.f, au_jp = 0.f, au_jm = 0.f,
+        au_j2 = 0.f, au_kp = 0.f, au_km = 0.f, au_k2 = 0.f;
+  float av0 = 0.f, av_jm = 0.f, av_j2m = 0.f, av_jp = 0.f, av_im = 0.f, av_i2m = 0.f,
+        av_ip = 0.f, av_kp = 0.f, av_km = 0.f, av_k2 = 0.f;
+  float aw0 = 0.f, aw_km = 0.f, aw_k2m = 0.f, aw_kp = 0.f, aw_im = 0.f, aw_i2m = 0.f,
+        aw_ip = 0.f, aw_jp = 0.f, aw_jm = 0.f, aw_j2 = 0.f;
+  if (has_pend) {
+    const float *Au = Awp.f[0], *Av = Awp.f[1], *Aw_ = Awp.f[2];
+    au0 = Au[p]; au_is = Au[p + S]; au_im = Au[p - S]; au_i2 = Au[p + 2 * S];
+    au_jp = Au[p + Y]; au_jm = Au[p - Y]; au_j2 = Au[p + 2 * Y];
+    au_kp = Au[p + 1]; au_km = Au[p - 1]; au_k2 = Au[p + 2];
+    av0 = Av[p]; av_jm = Av[p - Y]; av_j2m = Av[p - 2 * Y]; av_jp = Av[p + Y];
+    av_im = Av[p - S]; av_i2m = Av[p - 2 * S]; av_ip = Av[p + S];
+    av_kp = Av[p + 1]; av_km = Av[p - 1]; av_k2 = Av[p + 2];
+    aw0 = Aw_[p]; aw_km = Aw_[p - 1]; aw_k2m = Aw_[p - 2]; aw_kp = Aw_[p + 1];
+    aw_im = Aw_[p - S]; aw_i2m = Aw_[p - 2 * S]; aw_ip = Aw_[p + S];
+    aw_jp = Aw_[p + Y]; aw_jm = Aw_[p - Y]; aw_j2 = Aw_[p + 2 * Y];
+  }
+  float gold[5];
+#pragma unroll
+  for (int c = 0; c < 5; ++c) gold[c] = Gr.f[3 + c][p];
+  __builtin_amdgcn_sched_barrier(0);
+
+  const float c1 = G.c1, c2 = G.c2;
+  if (has_pend) {
+    // vel_gather6's sums, term for term
+    pend[0] = c1 * (au0 - au_is) + c2 * (au_im - au_i2);
+    pend[1] = c1 * (av_jm - av0) + c2 * (av_j2m - av_jp);
+    pend[2] = c1 * (aw_km - aw0) + c2 * (aw_k2m - aw_kp);
+    pend[3] = c1 * (au0 - au_jp) + c2 * (au_jm - au_j2) + c1 * (av_im - av0) +
+              c2 * (av_i2m - av_ip);
+    pend[4] = c1 * (au0 - au_kp) + c2 * (au_km - au_k2) + c1 * (aw_im - aw0) +
+              c2 * (aw_i2m - aw_ip);
+    pend[5] = c1 * (av0 - av_kp) + c2 * (av_km - av_k2) + c1 * (aw0 - aw_jp) +
+              c2 * (aw_jm - aw_j2);
+  }
+  const float ca = s_coeff[(f_ww * 2 - 2) & 15], cb = s_coeff[(f_ww * 2 - 1) & 15];
+  const float dcrj = dcx * dcy * dcz;
+  const float sl8 = l0 + l1 + l2 + l3 + l4 + l5 + l6 + l7;
+  const float sm8 = m0 + m1 + m2 + m3 + m4 + m5 + m6 + m7;
+  const float sp8 = p0 + p1 + p2 + p3 + p4 + p5 + p6 + p7;
+  const float ss8 = q0 + q1 + q2 + q3 + q4 + q5 + q6 + q7;
   const float xm = awp_quot(16.0f, sm8);
   const float xl = awp_quot(8.0f, sl8) + xm;
-  const float xmu1 = awp_quot(2.0f, M.mu[p] + M.mu[km1]);
-  const float xmu2 = awp_quot(2.0f, M.mu[p] + M.mu[jm1]);
-  const float xmu3 = awp_quot(2.0f, M.mu[p] + M.mu[ip1]);
+  const float xmu1 = awp_quot(2.0f, m0 + m4);
+  const float xmu2 = awp_quot(2.0f, m0 + m2);
+  const float xmu3 = awp_quot(2.0f, m0 + m1);
   const float qpaw = adj_qw(0.0625f * sp8, ca, cb, f_wwo);
   const float hw = adj_qw(0.0625f * ss8, ca, cb, f_wwo);
-  const float h1w = adj_qw(0.250f * (M.qs[p] + M.qs[km1]), ca, cb, f_wwo);
-  const float h2w = adj_qw(0.250f * (M.qs[p] + M.qs[jm1]), ca, cb, f_wwo);
-  const float h3w = adj_qw(0.250f * (M.qs[p] + M.qs[ip1]), ca, cb, f_wwo);
+  const float h1w = adj_qw(0.250f * (q0 + q4), ca, cb, f_wwo);
+  const float h2w = adj_qw(0.250f * (q0 + q2), ca, cb, f_wwo);
+  const float h3w = adj_qw(0.250f * (q0 + q1), ca, cb, f_wwo);
   const float dth = G.dth, dh1 = G.dh1, DT = G.DT;
   // coefficients per unit of xl, xm, xmu_k (the step is linear in them)
   const float qf = -qpaw * dh1 * f_vx1;            // qpa = xl * qf
@@ -217,34 +287,24 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
   const float keep = f_vx2 * (1 - f_vx1);
 
   // forward strains after the step (for the coefficient gradients)
-  const float c1 = G.c1, c2 = G.c2;
-  const float* u = V.f[0];
-  const float* v = V.f[1];
-  const float* w = V.f[2];
-  const float N1 = c1 * (u[p + S] - u[p]) + c2 * (u[p + 2 * S] - u[p - S]);
-  const float N2 = c1 * (v[p] - v[p - Y]) + c2 * (v[p + Y] - v[p - 2 * Y]);
-  const float N3 = c1 * (w[p] - w[p - 1]) + c2 * (w[p + 1] - w[p - 2]);
+  const float N1 = c1 * (u_is - u_0) + c2 * (u_i2 - u_im);
+  const float N2 = c1 * (v_0 - v_jm) + c2 * (v_jp - v_j2m);
+  const float N3 = c1 * (w_0 - w_km) + c2 * (w_kp - w_k2m);
   const float Esh[3] = {
-      c1 * (u[p + Y] - u[p]) + c2 * (u[p + 2 * Y] - u[p - Y]) + c1 * (v[p] - v[p - S]) +
-          c2 * (v[p + S] - v[p - 2 * S]),
-      c1 * (u[p + 1] - u[p]) + c2 * (u[p + 2] - u[p - 1]) + c1 * (w[p] - w[p - S]) +
-          c2 * (w[p + S] - w[p - 2 * S]),
-      c1 * (v[p + 1] - v[p]) + c2 * (v[p + 2] - v[p - 1]) + c1 * (w[p + Y] - w[p]) +
-          c2 * (w[p + 2 * Y] - w[p - Y])};
+      c1 * (u_jp - u_0) + c2 * (u_j2 - u_jm) + c1 * (v_0 - v_im) + c2 * (v_ip - v_i2m),
+      c1 * (u_kp - u_0) + c2 * (u_k2 - u_km) + c1 * (w_0 - w_im) + c2 * (w_ip - w_i2m),
+      c1 * (v_kp - v_0) + c2 * (v_k2 - v_km) + c1 * (w_jp - w_0) + c2 * (w_j2 - w_jm)};
   const float Ssum = N1 + N2 + N3;
   const float O[3] = {N2 + N3, N1 + N3, N1 + N2};
 
   // normal components: sigma' = P (sigma + (XL + DT QPA) S - XM O + vx1p r)
   //                              + D DT ((wwo-1) frt + keep r),  P = D (xx), D^2 (yy, zz)
   //                     r' = vx2 r + wwo frt,  frt = -H O + QPA S
-  float pend[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const bool has_pend = k <= G.top - 3;
-  if (has_pend) vel_gather6(Awp, p, G, pend);
   float Sbar = 0.0f, Obar[3], gxl = 0.0f, gxm = 0.0f;
   for (int c = 0; c < 3; ++c) {
     const float P = (c == 0) ? dcrj : dcrj * dcrj;
-    const float sb = has_pend ? A.f[c][p] + pend[c] : A.f[c][p];
-    const float rb = A.f[6 + c][p];
+    const float sb = has_pend ? sbl[c] + pend[c] : sbl[c];
+    const float rb = rbl[c];
     const float frtb = dcrj * DT * (f_wwo - 1) * sb + f_wwo * rb;
     A.f[c][p] = P * sb;
     A.f[6 + c][p] = P * vx1p * sb + dcrj * DT * keep * sb + f_vx2 * rb;
@@ -255,8 +315,8 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
   }
   float gmu[3];
   for (int c = 0; c < 3; ++c) {
-    const float sb = has_pend ? A.f[3 + c][p] + pend[3 + c] : A.f[3 + c][p];
-    const float rb = A.f[9 + c][p];
+    const float sb = has_pend ? sbl[3 + c] + pend[3 + c] : sbl[3 + c];
+    const float rb = rbl[3 + c];
     const float frtb = dcrj * DT * (f_wwo - 1) * sb + f_wwo * rb;
     A.f[3 + c][p] = dcrj * sb;
     A.f[9 + c][p] = dcrj * vx1p * sb + dcrj * DT * keep * sb + f_vx2 * rb;
@@ -266,11 +326,11 @@ __global__ __launch_bounds__(256) void adj_stress_local_kernel(Adj15 A, Ptr6 Es,
   Es.f[0][p] = Sbar + Obar[1] + Obar[2];
   Es.f[1][p] = Sbar + Obar[0] + Obar[2];
   Es.f[2][p] = Sbar + Obar[0] + Obar[1];
-  Gr.f[3][p] += gxl;
-  Gr.f[4][p] += gxm;
-  Gr.f[5][p] += gmu[0];
-  Gr.f[6][p] += gmu[1];
-  Gr.f[7][p] += gmu[2];
+  Gr.f[3][p] = gold[0] + gxl;
+  Gr.f[4][p] = gold[1] + gxm;
+  Gr.f[5][p] = gold[2] + gmu[0];
+  Gr.f[6][p] = gold[3] + gmu[1];
+  Gr.f[7][p] = gold[4] + gmu[2];
 }
 
 // stress^T, gather part: velocity adjoints from the strain adjoints of the
