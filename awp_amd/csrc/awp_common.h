@@ -63,6 +63,17 @@ __device__ __forceinline__ int awp_xcd_remap(int b, int nblocks) {
 
 static inline int awp_cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// k-run width of the (k, j)-tiled kernels: a wave covers 64 consecutive k of
+// one j row, or 32 consecutive k of each of two j rows when that leaves at
+// least 16 fewer lanes idle per column of n levels (n = 96 fills 2 x 64
+// lanes to 75 % but 3 x 32 to 100 %).  forced: 32 or 64 from the tuning,
+// else 0 for this rule.
+static inline int awp_pick_kw(int n, int forced) {
+  if (forced == 32 || forced == 64) return forced;
+  const int idle64 = awp_cdiv(n, 64) * 64 - n, idle32 = awp_cdiv(n, 32) * 32 - n;
+  return idle64 - idle32 >= 16 ? 32 : 64;
+}
+
 // The reference's (float)(c / (double)x) for binary32 c and x.  Rounding the
 // binary64 quotient to binary32 gives the correctly rounded binary32 quotient:
 // double rounding is innocuous when the wider format has at least 2p+2

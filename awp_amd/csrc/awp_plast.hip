@@ -24,6 +24,7 @@ namespace {
 
 struct PlastLaunch {
   int k0, k1, j0, j1, i0, ni, ichunk, nkt, njt, nblocks, remap, srfpos, yline;
+  int kw;  // k-run width (awp_pick_kw)
   long long slice;
   float DH, DT, Tv_fixed;
   float fmajor, fminor;
@@ -69,7 +70,7 @@ __device__ __forceinline__ float pfluid_at(int srfpos, int k, float DH) {
   return (float)((double)DH / 2. * 9.81e3);
 }
 
-template <int BJ>
+template <int BJ, int KW = 64>
 __global__ __launch_bounds__(64 * BJ) void drprecpc_calc_kernel(
     const float* __restrict__ xx, const float* __restrict__ yy, const float* __restrict__ zz,
     const float* __restrict__ xy, const float* __restrict__ xz, const float* __restrict__ yz,
@@ -81,8 +82,10 @@ __global__ __launch_bounds__(64 * BJ) void drprecpc_calc_kernel(
   const int kt = b % L.nkt;
   const int jt = (b / L.nkt) % L.njt;
   const int ic = b / (L.nkt * L.njt);
-  const int k = L.k0 + kt * 64 + (int)threadIdx.x;
-  const int j = L.j0 + jt * BJ + (int)threadIdx.y;
+  // KW = 32: a wave covers 32 levels of each of two j rows
+  const int lane = (int)threadIdx.x;
+  const int k = L.k0 + kt * KW + (KW == 64 ? lane : lane % KW);
+  const int j = L.j0 + (jt * BJ + (int)threadIdx.y) * (64 / KW) + (KW == 64 ? 0 : lane / KW);
   if (k > L.k1 || j > L.j1) return;
   const int ia = L.i0 + ic * L.ichunk;
   int nsteps = L.ni - ic * L.ichunk;
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(64 * BJ) void drprecpc_calc_kernel(
   }
 }
 
-template <int BJ>
+template <int BJ, int KW = 64>
 __global__ __launch_bounds__(64 * BJ) void drprecpc_app_kernel(
     float* __restrict__ xx, float* __restrict__ yy, float* __restrict__ zz,
     float* __restrict__ xy, float* __restrict__ xz, float* __restrict__ yz,
@@ -175,8 +178,10 @@ __global__ __launch_bounds__(64 * BJ) void drprecpc_app_kernel(
   const int kt = b % L.nkt;
   const int jt = (b / L.nkt) % L.njt;
   const int ic = b / (L.nkt * L.njt);
-  const int k = L.k0 + kt * 64 + (int)threadIdx.x;
-  const int j = L.j0 + jt * BJ + (int)threadIdx.y;
+  // KW = 32: a wave covers 32 levels of each of two j rows
+  const int lane = (int)threadIdx.x;
+  const int k = L.k0 + kt * KW + (KW == 64 ? lane : lane % KW);
+  const int j = L.j0 + (jt * BJ + (int)threadIdx.y) * (64 / KW) + (KW == 64 ? 0 : lane / KW);
   if (k > L.k1 || j > L.j1) return;
   const int ia = L.i0 + ic * L.ichunk;
   int nsteps = L.ni - ic * L.ichunk;
@@ -277,8 +282,9 @@ bool plast_setup(const char* W, int nzt, int xls, int xre, int yls, int yre, int
     L.RzT[q] = pc.RzT[q];
   }
   L.remap = awp_tuning().xcd_remap;
-  L.nkt = awp_cdiv(L.k1 - L.k0 + 1, 64);
-  L.njt = awp_cdiv(L.j1 - L.j0 + 1, 4);
+  L.kw = awp_pick_kw(L.k1 - L.k0 + 1, awp_tuning().kw);
+  L.nkt = awp_cdiv(L.k1 - L.k0 + 1, L.kw);
+  L.njt = awp_cdiv(L.j1 - L.j0 + 1, 4 * (64 / L.kw));
   int nic, ichunk;
   awp_split_i(L.ni, L.nkt * L.njt, 0, app ? 4096 : 2048, &nic, &ichunk);
   L.ichunk = ichunk;
@@ -292,8 +298,12 @@ void calc_common(const char* W, float* xx, float* yy, float* zz, float* xy, floa
                  hipStream_t St, int d_i, float Tv) {
   PlastLaunch L;
   if (!plast_setup(W, nzt, xls, xre, yls, yre, d_i, Tv, false, L)) return;
-  hipLaunchKernelGGL(drprecpc_calc_kernel<4>, dim3(L.nblocks), dim3(64, 4), 0, St, xx, yy, zz,
-                     xy, xz, yz, mu, d1, sigma2, yldfac, cohes, phi, neta, L);
+  if (L.kw == 32)
+    hipLaunchKernelGGL((drprecpc_calc_kernel<4, 32>), dim3(L.nblocks), dim3(64, 4), 0, St, xx, yy,
+                       zz, xy, xz, yz, mu, d1, sigma2, yldfac, cohes, phi, neta, L);
+  else
+    hipLaunchKernelGGL((drprecpc_calc_kernel<4, 64>), dim3(L.nblocks), dim3(64, 4), 0, St, xx, yy,
+                       zz, xy, xz, yz, mu, d1, sigma2, yldfac, cohes, phi, neta, L);
   awp_check_launch(W);
 }
 
@@ -328,8 +338,12 @@ void drprecpc_app_H(float* xx, float* yy, float* zz, float* xy, float* xz, float
   const char* W = "drprecpc_app_H";
   PlastLaunch L;
   if (!plast_setup(W, nzt, xls, xre, yls, yre, d_i, 0.0f, true, L)) return;
-  hipLaunchKernelGGL(drprecpc_app_kernel<4>, dim3(L.nblocks), dim3(64, 4), 0, St, xx, yy, zz, xy,
-                     xz, yz, sigma2, yldfac, L);
+  if (L.kw == 32)
+    hipLaunchKernelGGL((drprecpc_app_kernel<4, 32>), dim3(L.nblocks), dim3(64, 4), 0, St, xx, yy,
+                       zz, xy, xz, yz, sigma2, yldfac, L);
+  else
+    hipLaunchKernelGGL((drprecpc_app_kernel<4, 64>), dim3(L.nblocks), dim3(64, 4), 0, St, xx, yy,
+                       zz, xy, xz, yz, sigma2, yldfac, L);
   awp_check_launch(W);
 }
 

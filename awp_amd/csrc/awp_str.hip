@@ -76,7 +76,8 @@ struct TexTable {
   int xls, yls, nj, nzt;         // TEX_RULE only
 };
 
-template <int BJ, bool NEWLIT, bool OOP, class ST, int TEX = TEX_ARRAYS>
+// KW: k-run width (awp_pick_kw); with KW = 32 a wave covers two j rows.
+template <int BJ, bool NEWLIT, bool OOP, class ST, int TEX = TEX_ARRAYS, int KW = 64>
 __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
     typename ST::T* __restrict__ xx, typename ST::T* __restrict__ yy,
     typename ST::T* __restrict__ zz, typename ST::T* __restrict__ xy,
@@ -103,8 +104,10 @@ __global__ __launch_bounds__(64 * BJ) void dstrqc_kernel(
   const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
   int kt, jt, ic;
   awp_tile3(b, L.nkt, L.njt, L.jfast, kt, jt, ic);
-  const int k = L.kb + kt * 64 + (int)threadIdx.x;
-  const int j = L.j0 + jt * BJ + (int)threadIdx.y;
+  constexpr int RW = 64 / KW;  // j rows per wave
+  const int lane = (int)threadIdx.x;
+  const int k = L.kb + kt * KW + (KW == 64 ? lane : lane % KW);
+  const int j = L.j0 + (jt * BJ + (int)threadIdx.y) * RW + (KW == 64 ? 0 : lane / KW);
   if (k < L.k0 || k > L.k1 || j > L.j1) return;
   const int ia = L.i0 + ic * L.ichunk;
   int nsteps = L.ni - ic * L.ichunk;
@@ -414,8 +417,9 @@ void dstrqc_common(const char* W, void* const* f15, const CStress6& S0, float* l
   L.jfast = awp_tuning().str_jfast;
   if (L.k1 < L.k0) return;
   const int bj = awp_tuning().str_bj;
-  L.nkt = awp_cdiv(L.k1 - L.kb + 1, 64);
-  L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
+  const int kw = awp_pick_kw(L.k1 - L.kb + 1, awp_tuning().kw);
+  L.nkt = awp_cdiv(L.k1 - L.kb + 1, kw);
+  L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj * (64 / kw));
   int nic, ichunk;
   awp_split_i(L.ni, L.nkt * L.njt, awp_tuning().str_ichunk, awp_tuning().str_target_wgs, &nic, &ichunk);
   L.ichunk = ichunk;
@@ -423,15 +427,25 @@ void dstrqc_common(const char* W, void* const* f15, const CStress6& S0, float* l
 #define AWP_STR_ARGS                                                                          \
   A[0], A[1], A[2], A[3], A[4], A[5], S0, A[6], A[7], A[8], A[9], A[10], A[11], A[12], A[13], \
       A[14], lam, mu, qp, coeff, qs, dcrjx, dcrjy, dcrjz, vx1, vx2, ww, wwo, L, SC, TT
-#define AWP_STR_LAUNCH(B)                                                                \
-  hipLaunchKernelGGL((dstrqc_kernel<B, NEWLIT, OOP, ST, TEX>), dim3(L.nblocks), dim3(64, B), 0, \
-                     St, AWP_STR_ARGS)
-  switch (bj) {
-    case 1: AWP_STR_LAUNCH(1); break;
-    case 2: AWP_STR_LAUNCH(2); break;
-    case 4: AWP_STR_LAUNCH(4); break;
-    case 8: AWP_STR_LAUNCH(8); break;
-    default: AWP_STR_LAUNCH(16); break;
+#define AWP_STR_LAUNCH(B, KW)                                                               \
+  hipLaunchKernelGGL((dstrqc_kernel<B, NEWLIT, OOP, ST, TEX, KW>), dim3(L.nblocks), dim3(64, B), \
+                     0, St, AWP_STR_ARGS)
+  if (kw == 32) {
+    switch (bj) {
+      case 1: AWP_STR_LAUNCH(1, 32); break;
+      case 2: AWP_STR_LAUNCH(2, 32); break;
+      case 4: AWP_STR_LAUNCH(4, 32); break;
+      case 8: AWP_STR_LAUNCH(8, 32); break;
+      default: AWP_STR_LAUNCH(16, 32); break;
+    }
+  } else {
+    switch (bj) {
+      case 1: AWP_STR_LAUNCH(1, 64); break;
+      case 2: AWP_STR_LAUNCH(2, 64); break;
+      case 4: AWP_STR_LAUNCH(4, 64); break;
+      case 8: AWP_STR_LAUNCH(8, 64); break;
+      default: AWP_STR_LAUNCH(16, 64); break;
+    }
   }
 #undef AWP_STR_LAUNCH
 #undef AWP_STR_ARGS

@@ -21,6 +21,7 @@ struct AwpTuning {
   int topo_vel_split;   // 1: the curvilinear velocity update as one launch per component
   int adj_jgroup;       // adjoint kernels' block order: 0 plane by plane, else j tiles per group
   int topo_str_split;   // strain-rate pass: 0 one launch, 1 two (normal, shear), 2 three (normal, xy, xz+yz)
+  int kw;               // k-run width of the Cartesian kernels: 0 by the idle-lane rule, 32, 64
 };
 
 AwpTuning& awp_tuning();

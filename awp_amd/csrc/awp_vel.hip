@@ -32,7 +32,9 @@ struct VelLaunch {
 // the 4th-order update skips there) applies dvelc2's 2nd-order update
 // (kernel.cu:2289-2356) from the operands it has loaded anyway, so one
 // launch does dvelcx_H_opt + dvelc2_H.
-template <int BJ, bool OOP, class ST, bool DM2 = false>
+// KW: k-run width (awp_pick_kw); with KW = 32 a wave covers two j rows, and
+// the second row's offset rides in the per-lane byte offsets.
+template <int BJ, bool OOP, class ST, bool DM2 = false, int KW = 64>
 __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
     typename ST::T* __restrict__ u1, typename ST::T* __restrict__ v1,
     typename ST::T* __restrict__ w1, const typename ST::T* __restrict__ u0,
@@ -46,9 +48,14 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
   const int b = L.remap ? awp_xcd_remap(blockIdx.x, L.nblocks) : (int)blockIdx.x;
   int kt, jt, ic;
   awp_tile3(b, L.nkt, L.njt, L.jfast, kt, jt, ic);
-  // one wave per j row: j is wave-uniform, k is the lane
-  const int j = L.j0 + jt * BJ + __builtin_amdgcn_readfirstlane((int)threadIdx.y);
-  const int k = L.kb + kt * 64 + (int)threadIdx.x;
+  // one wave per j row (KW = 64) or per two rows (KW = 32): the wave's first
+  // row jw is wave-uniform, k is the lane (mod KW)
+  constexpr int RW = 64 / KW;
+  const int lane = (int)threadIdx.x;
+  const int lr = KW == 64 ? 0 : lane / KW;  // row of this lane in the wave
+  const int jw = L.j0 + (jt * BJ + __builtin_amdgcn_readfirstlane((int)threadIdx.y)) * RW;
+  const int j = jw + lr;
+  const int k = L.kb + kt * KW + (KW == 64 ? lane : lane % KW);
   if (j > L.j1) return;
   const bool dm2 = DM2 && k == AWP_ALIGN + 2;
   if ((k < L.k0 && !dm2) || k > L.k1) return;
@@ -58,15 +65,16 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
 
   const long long Y = L.yline;
   const long long S = L.slice;
-  const long long J = (long long)j * Y;
-  const unsigned kb = (unsigned)k * 4u;                      // medium (binary32)
-  const unsigned kw = (unsigned)k * (unsigned)sizeof(typename ST::T);  // wavefield
+  const long long J = (long long)jw * Y;
+  const unsigned ko = (unsigned)(lr * Y + k);                // lane offset from row jw
+  const unsigned kb = ko * 4u;                               // medium (binary32)
+  const unsigned kw = ko * (unsigned)sizeof(typename ST::T);  // wavefield
   const float sxx = SC.s[AWP_SXX], syy = SC.s[AWP_SYY], szz = SC.s[AWP_SZZ];
   const float sxy = SC.s[AWP_SXY], sxz = SC.s[AWP_SXZ], syz = SC.s[AWP_SYZ];
   const float c1 = L.c1, c2 = L.c2, dth = L.dth;
   const float dy = dcrjy[j];
   const float dz = dcrjz[k];
-// row pointer of field A at plane ii, row j+dj, shifted by dk in k (uniform)
+// row pointer of field A at plane ii, row jw+dj, shifted by dk in k (uniform)
 #define RP(A, ii, dj, dk) ((A) + (long long)(ii) * S + J + (long long)(dj) * Y + (dk))
 
   // Prime the x pipelines at i = ia.
@@ -160,7 +168,7 @@ __global__ __launch_bounds__(64 * BJ) void dvelcx_kernel(
 #undef RP
 }
 
-template <int BJ, class ST, bool DM2 = false>
+template <int BJ, class ST, bool DM2 = false, int KW = 64>
 void launch_vel(void* const* f9, void* const* v0, float* dcrjx, float* dcrjy, float* dcrjz,
                 float* d_1, hipStream_t St, VelLaunch L, const WaveScales& SC) {
   typedef typename ST::T T;
@@ -170,18 +178,18 @@ void launch_vel(void* const* f9, void* const* v0, float* dcrjx, float* dcrjy, fl
   L.ichunk = ichunk;
   L.nblocks = nkj * nic;
   if (DM2)
-    hipLaunchKernelGGL((dvelcx_kernel<BJ, false, ST, true>), dim3(L.nblocks), dim3(64, BJ), 0,
+    hipLaunchKernelGGL((dvelcx_kernel<BJ, false, ST, true, KW>), dim3(L.nblocks), dim3(64, BJ), 0,
                        St, (T*)f9[0], (T*)f9[1], (T*)f9[2], nullptr, nullptr, nullptr,
                        (const T*)f9[3], (const T*)f9[4], (const T*)f9[5], (const T*)f9[6],
                        (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz, d_1, L, SC);
   else if (v0)
-    hipLaunchKernelGGL((dvelcx_kernel<BJ, true, ST>), dim3(L.nblocks), dim3(64, BJ), 0, St,
+    hipLaunchKernelGGL((dvelcx_kernel<BJ, true, ST, false, KW>), dim3(L.nblocks), dim3(64, BJ), 0, St,
                        (T*)f9[0], (T*)f9[1], (T*)f9[2], (const T*)v0[0], (const T*)v0[1],
                        (const T*)v0[2], (const T*)f9[3], (const T*)f9[4], (const T*)f9[5],
                        (const T*)f9[6], (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz,
                        d_1, L, SC);
   else
-    hipLaunchKernelGGL((dvelcx_kernel<BJ, false, ST>), dim3(L.nblocks), dim3(64, BJ), 0, St,
+    hipLaunchKernelGGL((dvelcx_kernel<BJ, false, ST, false, KW>), dim3(L.nblocks), dim3(64, BJ), 0, St,
                        (T*)f9[0], (T*)f9[1], (T*)f9[2], nullptr, nullptr, nullptr,
                        (const T*)f9[3], (const T*)f9[4], (const T*)f9[5], (const T*)f9[6],
                        (const T*)f9[7], (const T*)f9[8], dcrjx, dcrjy, dcrjz, d_1, L, SC);
@@ -225,16 +233,30 @@ void dvelcx_common(const char* W, void* const* f9, void* const* v0, float* dcrjx
   L.jfast = awp_tuning().vel_jfast;
   L.kb = awp_tuning().kalign ? (L.k0 & ~31) : L.k0;
   const int bj = awp_tuning().vel_bj;
-  L.nkt = awp_cdiv(L.k1 - L.kb + 1, 64);
-  L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj);
+  const int kw = awp_pick_kw(L.k1 - L.kb + 1, awp_tuning().kw);
+  L.nkt = awp_cdiv(L.k1 - L.kb + 1, kw);
+  L.njt = awp_cdiv(L.j1 - L.j0 + 1, bj * (64 / kw));
   if (L.k1 < L.k0) return;
-  switch (bj) {
-    case 1: launch_vel<1, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 2: launch_vel<2, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 8: launch_vel<8, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    case 16: launch_vel<16, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
-    default: launch_vel<4, ST, DM2>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC); break;
+#define AWP_VEL_LAUNCH(B, KW) \
+  launch_vel<B, ST, DM2, KW>(f9, v0, dcrjx, dcrjy, dcrjz, d_1, St, L, SC)
+  if (kw == 32) {
+    switch (bj) {
+      case 1: AWP_VEL_LAUNCH(1, 32); break;
+      case 2: AWP_VEL_LAUNCH(2, 32); break;
+      case 8: AWP_VEL_LAUNCH(8, 32); break;
+      case 16: AWP_VEL_LAUNCH(16, 32); break;
+      default: AWP_VEL_LAUNCH(4, 32); break;
+    }
+  } else {
+    switch (bj) {
+      case 1: AWP_VEL_LAUNCH(1, 64); break;
+      case 2: AWP_VEL_LAUNCH(2, 64); break;
+      case 8: AWP_VEL_LAUNCH(8, 64); break;
+      case 16: AWP_VEL_LAUNCH(16, 64); break;
+      default: AWP_VEL_LAUNCH(4, 64); break;
+    }
   }
+#undef AWP_VEL_LAUNCH
   awp_check_launch(W);
 }
 

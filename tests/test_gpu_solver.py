@@ -142,3 +142,34 @@ def test_fused_step_matches_separate_launchers(torch_gpu, oracle_lib, texture, s
     assert np.array_equal(g.traces(), o.traces())
     for n in STATE15:
         assert np.array_equal(g.field(n), o.f[n]), n
+
+
+@pytest.mark.parametrize("kw", [32, 64])
+def test_k_run_width_matches_oracle(torch_gpu, oracle_lib, kw):
+    """The Cartesian kernels with a k run of 64 levels of one j row per wave
+    and of 32 levels of each of two rows (awp_set_tuning("kw"), which the
+    idle-lane rule otherwise picks by the k extent): an NVE = 3 run on a grid
+    of 70 levels, odd in i and j, with the texture rule and with the per-cell
+    texture arrays, bit-exact against the FP32 oracle either way."""
+    from awp_amd import _lib
+    from awp_amd.solver import Solver
+    from oracle.oracle import OracleSolver
+    lib = _lib.load()
+    w = configs.plastic(nx=37, ny=29, nz=70, nt=40, nsub=12, M0=1.0e18, f0=5.0)
+    med = w.medium()
+    pm = w.params["plastic_fn"](med)
+    o32 = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, dtype=np.float32, plastic=pm)
+    o32.run(w.nt)
+    assert int((o32.f["yldfac"] < 1).sum()) > 0
+    assert lib.awp_set_tuning(b"kw", kw) == 0
+    try:
+        for texture in ("rule", "arrays"):
+            g = Solver(med, w.sources, w.receivers, nt_record=w.nt, plastic=pm, texture=texture)
+            g.run(w.nt, check_every=10)
+            torch_gpu.cuda.synchronize()
+            assert np.array_equal(g.traces(), o32.traces()), texture
+            for n in ("u1", "v1", "w1", "xx", "yy", "xz", "yz", "r1", "r4", "yldfac", "neta"):
+                assert np.array_equal(g.field(n), o32.f[n]), (texture, n)
+            del g
+    finally:
+        assert lib.awp_set_tuning(b"kw", 0) == 0
