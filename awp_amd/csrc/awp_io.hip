@@ -15,7 +15,7 @@
 // xy, xz and yz: 2.056 -> 2.006 -> 1.948 ms per stress update for one, two,
 // three launches); the adjoint kernels' blocks in groups of 8 j tiles over
 // all planes (adj_stress_local 551 -> 533 us at 256x256x128).
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 2, 0};
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 2, 0, 8};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -214,6 +214,9 @@ int awp_set_tuning(const char* key, int value) {
   } else if (!strcmp(key, "topo_str_split")) {
     if (value < 0 || value > 2) return -1;
     t.topo_str_split = value;
+  } else if (!strcmp(key, "topo_nseg")) {
+    if (value < 1 || value > 8) return -1;
+    t.topo_nseg = value;
   } else if (!strcmp(key, "kw")) {
     if (value != 0 && value != 32 && value != 64) return -1;
     t.kw = value;

@@ -102,12 +102,20 @@ __device__ __forceinline__ float dbl_quot(float num, float den) { return awp_quo
 // rows (table row kk = nz-1-k), as a divergent branch of its wave.
 #define VT_BJ 4      // waves (j columns) per workgroup
 #define VT_NL 9      // LDS arrays per wave
+#define VT_MAXW 8    // waves per workgroup of the chunked stress kernel at most
 
+// The k tiling of a launch.  The velocity kernel takes one-wave segments
+// (the first four fields, 58 outputs each); the stress kernel takes chunks
+// of nseg waves that share one LDS column (all fields, topo_tiles).
 struct VelTiles {
-  int nint;   // interior segments (58 outputs each, from level 3)
-  int ktop0;  // first level of the top segment (nz - 64, may be negative)
-  int wtop0;  // first level the top segment writes
+  int nint;   // interior segments / chunks (58 or ch - 6 outputs each, from level 3)
+  int ktop0;  // first level of the top one (nz - 64 or nz - ch, may be negative)
+  int wtop0;  // first level the top one writes
   int ichunk; // planes per march
+  int nseg;   // waves per chunk (64 levels each)
+  int ch;     // levels per chunk, 64 * nseg
+  int step;   // ch - 6
+  int rj;     // j rows per workgroup (nseg * rj waves)
 };
 
 // COMP: 0 updates u1, u2 and u3; 1, 2, 3 update only u1, u2 or u3 (the
@@ -507,15 +515,16 @@ __device__ __forceinline__ void topo_str_update(const TopoStrArgs<ST>& A, int po
 
 // Stress: dtopo_str_111 (stress_attenuation.cu:74-668) on levels
 // 3 <= k < nz-6 and dtopo_str_112 (:670-1410) on the top six, in one kernel
-// with the velocity kernel's lane map (wave = (j, 64-level segment), march
-// along i, 58 output levels per segment, the top rows in the last segment).
+// with the velocity kernel's lane map (a chunk of nseg waves = (j, 64*nseg
+// levels), march along i, ch-6 output levels per chunk, the top rows in the
+// last chunk).
 //
 // The metric terms need 7-point (top rows: 9-point table) z sums of
 // velocity columns at x neighbours (i-2 .. i+2) and y neighbours (j-2 .. j+2).
 // The x-neighbour sums ride in register windows: one new column's sum per
 // field and plane.  The y-neighbour sums and the own column's z differences
-// read velocity columns that each plane stages in LDS (15 columns per wave).
-#define ST_BJ 4
+// read velocity columns that each plane stages in LDS (15 columns per chunk
+// and row).
 #define ST_NC 15
 #define TOPO_NRATES AWP_TOPO_NRATES
 enum {
@@ -554,7 +563,7 @@ constexpr bool st_staged(int group, int c) { return (st_group_cols(group) >> c) 
 // (st_group_cols); launches of groups 1 and 2, or 1, 3 and 4, store what one
 // of group 0 does, each with fewer live registers and LDS slots.
 template <class ST, bool RATES, int GROUP = 0>
-__global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<ST> A,
+__global__ __launch_bounds__(64 * VT_MAXW) void topo_str_kernel(const TopoStrArgs<ST> A,
                                                               const TGeom G, const VelTiles V,
                                                               float* __restrict__ R,
                                                               long long rstride) {
@@ -564,19 +573,24 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
                  GZ = GROUP != 1 && GROUP != 3;
   constexpr int NC = st_group_ncols(GROUP);
 #define SC_(c) st_slot(GROUP, c)
-  __shared__ float lds[2][ST_BJ][NC][64];
+  // [parity][row][slot][level of the chunk]: 2 * rj * NC * ch floats; the
+  // column of array c at parity base L
+#define LCOL(c) (L + SC_(c) * V.ch)
+  extern __shared__ float st_lds[];
   const int lane = (int)threadIdx.x;
   const int w = (int)threadIdx.y;
+  const int seg = w % V.nseg, row = w / V.nseg;
   const int t = (int)blockIdx.x;
   const bool top = t == V.nint;
-  const int k0 = top ? V.ktop0 : 58 * t;
-  const int k = k0 + lane;  // topographic level
+  const int k0 = top ? V.ktop0 : V.step * t;
+  const int kl = seg * 64 + lane;  // level within the chunk: the LDS index
+  const int k = k0 + kl;  // topographic level
   const int nz = G.nz;
-  const int w0 = top ? V.wtop0 : 58 * t + 3;
-  const int w1 = top ? nz : (58 * t + 61 < V.wtop0 ? 58 * t + 61 : V.wtop0);
+  const int w0 = top ? V.wtop0 : V.step * t + 3;
+  const int w1 = top ? nz : (V.step * t + V.ch - 3 < V.wtop0 ? V.step * t + V.ch - 3 : V.wtop0);
   const bool bnd = k >= nz - 6;
   const int kb = nz - 1 - k;  // table row of a top level
-  const int jw = G.bj + (int)(blockIdx.y * ST_BJ) + w;
+  const int jw = G.bj + (int)blockIdx.y * V.rj + row;
   const bool jok = jw < G.ej;
   const int j = __builtin_amdgcn_readfirstlane(jok ? jw : G.ej - 1);
   const int ib = G.bi + (int)blockIdx.z * V.ichunk;
@@ -587,7 +601,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
   const int kr = k < 0 ? 0 : k;          // read level (see topo_vel_kernel)
   const int K = kr + AWP_ALIGN;          // array level
   const TMetrics& M = A.M;
-  const int lb = nz - 1 - k0;            // lane of level nz-1 (top segment)
+  const int lb = nz - 1 - k0;            // LDS level of nz-1 (top chunk)
 
   constexpr float phdz4[7] = {(float)TS_PHDZ4[0], (float)TS_PHDZ4[1], (float)TS_PHDZ4[2],
                               (float)TS_PHDZ4[3], (float)TS_PHDZ4[4], (float)TS_PHDZ4[5],
@@ -597,19 +611,19 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
                               (float)TS_PDHZ4[6]};
   // z sums of an LDS column at this lane's level
   auto zasc = [&](const float* c) {  // phdz4, ascending (normal stresses)
-    float acc = phdz4[0] * c[lane - 3];
+    float acc = phdz4[0] * c[kl - 3];
 #pragma unroll
-    for (int m = 1; m < 7; ++m) acc = acc + phdz4[m] * c[lane + m - 3];
+    for (int m = 1; m < 7; ++m) acc = acc + phdz4[m] * c[kl + m - 3];
     return acc;
   };
   auto zmid = [&](const float(&cf)[7], const float* c) {  // order 3, 0, 1, 2, 4, 5, 6
-    float acc = cf[3] * c[lane];
-    acc = acc + cf[0] * c[lane - 3];
-    acc = acc + cf[1] * c[lane - 2];
-    acc = acc + cf[2] * c[lane - 1];
-    acc = acc + cf[4] * c[lane + 1];
-    acc = acc + cf[5] * c[lane + 2];
-    acc = acc + cf[6] * c[lane + 3];
+    float acc = cf[3] * c[kl];
+    acc = acc + cf[0] * c[kl - 3];
+    acc = acc + cf[1] * c[kl - 2];
+    acc = acc + cf[2] * c[kl - 1];
+    acc = acc + cf[4] * c[kl + 1];
+    acc = acc + cf[5] * c[kl + 2];
+    acc = acc + cf[6] * c[kl + 3];
     return acc;
   };
   auto zrow = [&](const FTab<6, 9>& tab, const float* c) {  // top rows, m = 8 .. 0
@@ -631,17 +645,17 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
   // prologue: stage the columns the first plane's windows need
   for (int q = 0; q < 3; ++q) {
     const int iu = ib - 1 + q, iv = ib - 2 + q;
-    float(*L)[64] = lds[q & 1][w];
+    float* L = st_lds + ((q & 1) * V.rj + row) * NC * V.ch;
     ux[q + 1] = A.u1[colj + sl * iu];
     vx[q + 1] = A.v1[colj + sl * iv];
     wx[q + 1] = A.w1[colj + sl * iv];
-    if constexpr (GN) L[SC_(C_UN)][lane] = ux[q + 1];
-    if constexpr (GXY) L[SC_(C_VN)][lane] = vx[q + 1];
-    if constexpr (GZ) L[SC_(C_WN)][lane] = wx[q + 1];
+    if constexpr (GN) LCOL(C_UN)[kl] = ux[q + 1];
+    if constexpr (GXY) LCOL(C_VN)[kl] = vx[q + 1];
+    if constexpr (GZ) LCOL(C_WN)[kl] = wx[q + 1];
     __syncthreads();
-    if constexpr (GN) Zu[q + 1] = zu_x(L[SC_(C_UN)]);
-    if constexpr (GXY) Zv[q + 1] = zv_x(L[SC_(C_VN)]);
-    if constexpr (GZ) Zw[q + 1] = zw(L[SC_(C_WN)]);
+    if constexpr (GN) Zu[q + 1] = zu_x(LCOL(C_UN));
+    if constexpr (GXY) Zv[q + 1] = zv_x(LCOL(C_VN));
+    if constexpr (GZ) Zw[q + 1] = zw(LCOL(C_WN));
   }
 
   const float g_ck = M.g_c[K], g_k = M.g[K], g3c_k = M.g3_c[K], g3_k = M.g3[K];
@@ -651,7 +665,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
 
   for (int i = ib; i < ie; ++i) {
     const int par = (i + 1) & 1;  // the prologue used parities 0, 1, 0
-    float(*L)[64] = lds[par][w];
+    float* L = st_lds + (par * V.rj + row) * NC * V.ch;
     const int pos = colj + sl * i;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -685,7 +699,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     const float uo = ux[1], vo = vx[2], wo = wx[2];
     auto stage = [&](int c, float x) {
       // c is a literal column at every call, so the test folds away
-      if (st_staged(GROUP, c)) L[st_slot(GROUP, c)][lane] = x;
+      if (st_staged(GROUP, c)) L[st_slot(GROUP, c) * V.ch + kl] = x;
     };
     stage(C_UN, ux[3]);
     stage(C_UO, uo);
@@ -703,9 +717,9 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     stage(C_WP1, wp1);
     stage(C_WP2, wp2);
     __syncthreads();
-    if constexpr (GN) Zu[3] = zu_x(L[SC_(C_UN)]);
-    if constexpr (GXY) Zv[3] = zv_x(L[SC_(C_VN)]);
-    if constexpr (GZ) Zw[3] = zw(L[SC_(C_WN)]);
+    if constexpr (GN) Zu[3] = zu_x(LCOL(C_UN));
+    if constexpr (GXY) Zv[3] = zv_x(LCOL(C_VN));
+    if constexpr (GZ) Zw[3] = zw(LCOL(C_WN));
     if (out) {
       const int q = j + AWP_ALIGN + i * fr;
       const float f_dcrj = A.dcrjx[i] * dcy * dcz;
@@ -726,10 +740,10 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         acc = acc + P4F(3) * M.f1_1[q + 2 * fr] * Zu[3];
         vs1 = D4F(1) * ux[1] + D4F(0) * ux[0] + D4F(2) * ux[2] + D4F(3) * ux[3] -
                           Jii * g_ck * acc;
-        const float zv_o = bnd ? zrow(c_str_phdz4r, L[SC_(C_VO)]) : zasc(L[SC_(C_VO)]);
-        const float zv_m2 = bnd ? zrow(c_str_phdz4r, L[SC_(C_VM2)]) : zasc(L[SC_(C_VM2)]);
-        const float zv_m1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_VM1)]) : zasc(L[SC_(C_VM1)]);
-        const float zv_p1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_VP1)]) : zasc(L[SC_(C_VP1)]);
+        const float zv_o = bnd ? zrow(c_str_phdz4r, LCOL(C_VO)) : zasc(LCOL(C_VO));
+        const float zv_m2 = bnd ? zrow(c_str_phdz4r, LCOL(C_VM2)) : zasc(LCOL(C_VM2));
+        const float zv_m1 = bnd ? zrow(c_str_phdz4r, LCOL(C_VM1)) : zasc(LCOL(C_VM1));
+        const float zv_p1 = bnd ? zrow(c_str_phdz4r, LCOL(C_VP1)) : zasc(LCOL(C_VP1));
         acc = P4F(2) * M.f2_2[q] * zv_o;
         acc = acc + P4F(0) * M.f2_2[q - 2] * zv_m2;
         acc = acc + P4F(1) * M.f2_2[q - 1] * zv_m1;
@@ -737,11 +751,11 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         vs2 = D4F(2) * vo + D4F(0) * vm2 + D4F(1) * vm1 + D4F(3) * vp1 -
                           Jii * g_ck * acc;
         if (!bnd) {
-          const float* c = L[SC_(C_WO)];
-          vs3 = Jii * (D4F(2) * c[lane] + D4F(0) * c[lane - 2] + D4F(1) * c[lane - 1] +
-                       D4F(3) * c[lane + 1]);
+          const float* c = LCOL(C_WO);
+          vs3 = Jii * (D4F(2) * c[kl] + D4F(0) * c[kl - 2] + D4F(1) * c[kl - 1] +
+                       D4F(3) * c[kl + 1]);
         } else {
-          const float* c = L[SC_(C_WO)];
+          const float* c = LCOL(C_WO);
           float zs = c_str_dhz4r.v[kb][7] * c[lb - 7];
   #pragma unroll
           for (int m = 6; m >= 0; --m) zs = zs + c_str_dhz4r.v[kb][m] * c[lb - m];
@@ -753,10 +767,10 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         // xy
         float J12i = M.f[q] * g3c_k6;
         J12i = awp_quot(1.0f, J12i);
-        const float zu_o = bnd ? zrow(c_str_phdz4r, L[SC_(C_UO)]) : zmid(phdz4, L[SC_(C_UO)]);
-        const float zu_m1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_UM1)]) : zmid(phdz4, L[SC_(C_UM1)]);
-        const float zu_p1 = bnd ? zrow(c_str_phdz4r, L[SC_(C_UP1)]) : zmid(phdz4, L[SC_(C_UP1)]);
-        const float zu_p2 = bnd ? zrow(c_str_phdz4r, L[SC_(C_UP2)]) : zmid(phdz4, L[SC_(C_UP2)]);
+        const float zu_o = bnd ? zrow(c_str_phdz4r, LCOL(C_UO)) : zmid(phdz4, LCOL(C_UO));
+        const float zu_m1 = bnd ? zrow(c_str_phdz4r, LCOL(C_UM1)) : zmid(phdz4, LCOL(C_UM1));
+        const float zu_p1 = bnd ? zrow(c_str_phdz4r, LCOL(C_UP1)) : zmid(phdz4, LCOL(C_UP1));
+        const float zu_p2 = bnd ? zrow(c_str_phdz4r, LCOL(C_UP2)) : zmid(phdz4, LCOL(C_UP2));
         acc = P4F(1) * M.f2_1[q] * zu_o;
         acc = acc + P4F(0) * M.f2_1[q - 1] * zu_m1;
         acc = acc + P4F(2) * M.f2_1[q + 1] * zu_p1;
@@ -780,11 +794,11 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         J13i = awp_quot(1.0f, J13i);
         float xzv1;
         if (!bnd) {
-          const float* c = L[SC_(C_UO)];
-          xzv1 = J13i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
-                         D4F(3) * c[lane + 2]);
+          const float* c = LCOL(C_UO);
+          xzv1 = J13i * (D4F(1) * c[kl] + D4F(0) * c[kl - 1] + D4F(2) * c[kl + 1] +
+                         D4F(3) * c[kl + 2]);
         } else {
-          const float* c = L[SC_(C_UO)];
+          const float* c = LCOL(C_UO);
           float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
   #pragma unroll
           for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
@@ -801,17 +815,17 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
         J23i = awp_quot(1.0f, J23i);
         float yzv1;
         if (!bnd) {
-          const float* c = L[SC_(C_VO)];
-          yzv1 = J23i * (D4F(1) * c[lane] + D4F(0) * c[lane - 1] + D4F(2) * c[lane + 1] +
-                         D4F(3) * c[lane + 2]);
+          const float* c = LCOL(C_VO);
+          yzv1 = J23i * (D4F(1) * c[kl] + D4F(0) * c[kl - 1] + D4F(2) * c[kl + 1] +
+                         D4F(3) * c[kl + 2]);
         } else {
-          const float* c = L[SC_(C_VO)];
+          const float* c = LCOL(C_VO);
           float zs = c_str_dz4r.v[kb][6] * c[lb - 6];
   #pragma unroll
           for (int m = 5; m >= 0; --m) zs = zs + c_str_dz4r.v[kb][m] * c[lb - m];
           yzv1 = J23i * zs;
         }
-        const float zw_m1 = zw(L[SC_(C_WM1)]), zw_p1 = zw(L[SC_(C_WP1)]), zw_p2 = zw(L[SC_(C_WP2)]);
+        const float zw_m1 = zw(LCOL(C_WM1)), zw_p1 = zw(LCOL(C_WP1)), zw_p2 = zw(LCOL(C_WP2));
         acc = P4F(1) * M.f2_c[q] * Zw[2];
         acc = acc + P4F(0) * M.f2_c[q - 1] * zw_m1;
         acc = acc + P4F(2) * M.f2_c[q + 1] * zw_p1;
@@ -839,6 +853,7 @@ __global__ __launch_bounds__(64 * ST_BJ) void topo_str_kernel(const TopoStrArgs<
     }
   }
 #undef SC_
+#undef LCOL
 }
 
 // The relaxation texture of the Q pass: the per-cell arrays (TOPO_TEX_ARRAYS),
@@ -1074,6 +1089,30 @@ bool metrics_present(const TMetrics& M, const char* where) {
   return true;
 }
 
+// Chunks of the curvilinear stress kernel's k lane map for a column of nz
+// levels: nseg = min(topo_nseg, ceil(nz / 64)) waves of 64 levels per chunk,
+// so a column of up to 64 * topo_nseg levels is one chunk (nz = 192: three
+// waves, where one-wave segments of 58 outputs need four, one of them with
+// 12); rj j rows per workgroup keep it at 4-8 waves.  The velocity kernel
+// measured slower with chunks (1.06 against 0.99 ms at 384x384x192; its top
+// wave's one-sided rows hold the chunk's other waves at the plane barrier)
+// and keeps one-wave segments.
+void topo_tiles(int nz, VelTiles* V) {
+  int nseg = awp_cdiv(nz, 64);
+  const int cap = awp_tuning().topo_nseg;
+  if (nseg > cap) nseg = cap;
+  if (nseg > VT_MAXW) nseg = VT_MAXW;
+  if (nseg < 1) nseg = 1;
+  V->nseg = nseg;
+  V->ch = 64 * nseg;
+  V->step = V->ch - 6;
+  V->rj = nseg == 1 ? 4 : (VT_MAXW / nseg < 4 ? VT_MAXW / nseg : 4);
+  if (V->rj < 1) V->rj = 1;
+  V->ktop0 = nz - V->ch;
+  V->wtop0 = V->ktop0 + 3 > 3 ? V->ktop0 + 3 : 3;
+  V->nint = V->wtop0 > 3 ? awp_cdiv(V->wtop0 - 3, V->step) : 0;
+}
+
 template <class ST>
 void topo_velocity_launch(topo_t* T, const WaveScales& W, const char* where) {
   TGeom G;
@@ -1215,14 +1254,17 @@ void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float
   if (G.ei <= G.bi || G.ej <= G.bj) return;
   hipStream_t st = T->stream_i;
   VelTiles V;
-  V.ktop0 = T->nz - 64;
-  V.wtop0 = V.ktop0 + 3 > 3 ? V.ktop0 + 3 : 3;
-  V.nint = V.wtop0 > 3 ? awp_cdiv(V.wtop0 - 3, 58) : 0;
+  topo_tiles(T->nz, &V);
   V.ichunk = awp_tuning().topo_str_ichunk;
-  dim3 block(64, ST_BJ, 1);
-  dim3 grid(V.nint + 1, awp_cdiv(G.ej - G.bj, ST_BJ), awp_cdiv(G.ei - G.bi, V.ichunk));
+  dim3 block(64, V.nseg * V.rj, 1);
+  dim3 grid(V.nint + 1, awp_cdiv(G.ej - G.bj, V.rj), awp_cdiv(G.ei - G.bi, V.ichunk));
+  // LDS of a launch of rate group g: 2 parities x rj rows x its columns
+  auto lds = [&](int g) {
+    return (size_t)2 * V.rj * st_group_ncols(g) * V.ch * sizeof(float);
+  };
   if (!rates) {
-    hipLaunchKernelGGL((topo_str_kernel<ST, false>), grid, block, 0, st, A, G, V, nullptr, 0ll);
+    hipLaunchKernelGGL((topo_str_kernel<ST, false>), grid, block, lds(0), st, A, G, V, nullptr,
+                       0ll);
     awp_check_launch(where);
     return;
   }
@@ -1230,19 +1272,19 @@ void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float
   // variable update streaming them back
   const long long rstride = (long long)G.sl * (T->nx + 4 + AWP_NGSL2);
   if (awp_tuning().topo_str_split == 2) {
-    hipLaunchKernelGGL((topo_str_kernel<ST, true, 1>), grid, block, 0, st, A, G, V, rates,
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 1>), grid, block, lds(1), st, A, G, V, rates,
                        rstride);
-    hipLaunchKernelGGL((topo_str_kernel<ST, true, 3>), grid, block, 0, st, A, G, V, rates,
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 3>), grid, block, lds(3), st, A, G, V, rates,
                        rstride);
-    hipLaunchKernelGGL((topo_str_kernel<ST, true, 4>), grid, block, 0, st, A, G, V, rates,
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 4>), grid, block, lds(4), st, A, G, V, rates,
                        rstride);
   } else if (awp_tuning().topo_str_split) {
-    hipLaunchKernelGGL((topo_str_kernel<ST, true, 1>), grid, block, 0, st, A, G, V, rates,
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 1>), grid, block, lds(1), st, A, G, V, rates,
                        rstride);
-    hipLaunchKernelGGL((topo_str_kernel<ST, true, 2>), grid, block, 0, st, A, G, V, rates,
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 2>), grid, block, lds(2), st, A, G, V, rates,
                        rstride);
   } else {
-    hipLaunchKernelGGL((topo_str_kernel<ST, true, 0>), grid, block, 0, st, A, G, V, rates,
+    hipLaunchKernelGGL((topo_str_kernel<ST, true, 0>), grid, block, lds(0), st, A, G, V, rates,
                        rstride);
   }
   const int qchunk = awp_tuning().topo_q_ichunk;

@@ -22,6 +22,7 @@ struct AwpTuning {
   int adj_jgroup;       // adjoint kernels' block order: 0 plane by plane, else j tiles per group
   int topo_str_split;   // strain-rate pass: 0 one launch, 1 two (normal, shear), 2 three (normal, xy, xz+yz)
   int kw;               // k-run width of the Cartesian kernels: 0 by the idle-lane rule, 32, 64
+  int topo_nseg;        // curvilinear stress kernel: waves per k chunk at most (1: one-wave segments)
 };
 
 AwpTuning& awp_tuning();

@@ -280,3 +280,37 @@ def test_gpu_topography_workload_against_fp64(gpu, storage, tol):
     err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
     print(f"{storage}: receiver rel-L2 vs FP64 oracle {err:.3e}")
     assert err < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nseg", [8, 2, 1])
+def test_gpu_topography_stress_chunks_bit_exact(gpu, oracle_lib, nseg):
+    """The curvilinear stress kernel's k chunks on a 200-level column:
+    awp_set_tuning("topo_nseg") 8 (the default: one chunk of four waves
+    sharing an LDS column, 56 lanes below level 0), 2 (an interior chunk of
+    two waves and the top chunk, overlapping by six levels) and 1 (one-wave
+    segments of 58 outputs).  24 steps with the two-pass (three rate groups)
+    and one-pass updates, bit-exact against the oracle."""
+    torch, _lib = gpu
+    from awp_amd.solver import FIELDS, Solver
+    from oracle.oracle import OracleSolver
+    lib = _lib.load()
+    w = configs.topography_hills(nx=24, ny=20, nz=200, nt=24, DH=100.0, DT=0.005, nhills=3,
+                                 hmax=500.0, wmin=600.0, wmax=1500.0, nrec=6)
+    med = w.medium()
+    topo = w.params["topography"]
+    o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    o.run(w.nt)
+    assert lib.awp_set_tuning(b"topo_nseg", nseg) == 0
+    try:
+        for two_pass in (True, False):
+            g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+            if not two_pass:
+                g.topo.rates = None
+            g.run(w.nt, check_every=8)
+            torch.cuda.synchronize()
+            assert np.array_equal(g.traces(), o.traces()), two_pass
+            for n in FIELDS:
+                assert np.array_equal(g.field(n), o.f[n]), (two_pass, n)
+    finally:
+        assert lib.awp_set_tuning(b"topo_nseg", 8) == 0
