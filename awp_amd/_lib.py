@@ -48,6 +48,7 @@ SIGNATURES: dict[str, list] = {
     "swap_H": [_fp] * 18 + [_i, _i] + [_fp] * 4 + [_i, _s, _i],
     "awp_dvelcx_s16": [_fp] * 13 + [_i, _i, _s, _i, _i, _i, _i, _i, _fp],
     "awp_dstrqc_s16": [_fp] * 23 + [_i, _i, _s] + [_fp] * 5 + [_i] * 9 + [_i, _fp],
+    "awp_dstrqc_tex_s16": [_fp] * 23 + [_i, _i, _s] + [_fp] * 3 + [_i] * 9 + [_i, _fp],
     "awp_fstr_s16": [_fp] * 3 + [_s, _i, _i, _i, _i, _i, _fp],
     "awp_addsrc_s16": [_i, _i, _i, _fp, _i, _s] + [_fp] * 12 + [_i, _i, _fp],
     "awp_receivers_gather_s16": [_fp] * 4 + [_i, _fp, _i, _s, _i, _i, _fp],

@@ -559,6 +559,54 @@ void awp_dstrqc_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint
                                        rankx, ranky, s_i, e_i, s_j, e_j, d_i, SC);
 }
 
+void awp_dstrqc_tex_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* xz,
+                        uint16_t* yz, uint16_t* r1, uint16_t* r2, uint16_t* r3, uint16_t* r4,
+                        uint16_t* r5, uint16_t* r6, uint16_t* u1, uint16_t* v1, uint16_t* w1,
+                        float* lam, float* mu, float* qp, float* coeff, float* qs, float* dcrjx,
+                        float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St,
+                        float* lam_mu, const float* tex, int* ww, int NX, int NPC, int rankx,
+                        int ranky, int s_i, int e_i, int s_j, int e_j, int d_i, int fmt,
+                        const float* scale) {
+  const char* W = "awp_dstrqc_tex_s16";
+  if (!tex) {
+    awp_set_error(AWP_ERR_ARG, "%s: the texture table is NULL", W);
+    return;
+  }
+  if (!awp_grid_ok(d_i, W)) return;
+  WaveScales SC;
+  if (!awp_scales_from_host(W, fmt, scale, &SC)) return;
+  const AwpGrid& g = awp_state().grid[d_i];
+  TexTable TT;
+  for (int m = 0; m < 8; ++m) {
+    TT.vx1[m] = tex[m];
+    TT.vx2[m] = tex[8 + m];
+    TT.wwo[m] = tex[16 + m];
+  }
+  // init_texture's region on one rank: the interior columns, all levels
+  TT.xls = 2 + AWP_NGSL;
+  TT.yls = 2 + AWP_NGSL;
+  TT.nj = g.nyt;
+  TT.nzt = g.nzt;
+  void* f15[15] = {xx, yy, zz, xy, xz, yz, r1, r2, r3, r4, r5, r6, u1, v1, w1};
+#define AWP_TEX16(STT, TEXM, WWP)                                                             \
+  dstrqc_common<false, false, STT, TEXM>(W, f15, CStress6{}, lam, mu, qp, coeff, qs, dcrjx,   \
+                                         dcrjy, dcrjz, nyt, nzt, St, lam_mu, nullptr, nullptr, \
+                                         WWP, nullptr, NX, NPC, rankx, ranky, s_i, e_i, s_j,   \
+                                         e_j, d_i, SC, TT)
+  if (fmt == AWP_FMT_BF16) {
+    if (ww)
+      AWP_TEX16(StBF16, TEX_TABLE, ww);
+    else
+      AWP_TEX16(StBF16, TEX_RULE, nullptr);
+  } else {
+    if (ww)
+      AWP_TEX16(StF16, TEX_TABLE, ww);
+    else
+      AWP_TEX16(StF16, TEX_RULE, nullptr);
+  }
+#undef AWP_TEX16
+}
+
 void awp_dstrqc_oop(float* xx, float* yy, float* zz, float* xy, float* xz, float* yz,
                     const float* const* s0, float* r1, float* r2, float* r3, float* r4,
                     float* r5, float* r6, float* u1, float* v1, float* w1, float* lam, float* mu,

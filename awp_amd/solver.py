@@ -495,12 +495,16 @@ class Solver16(ScaledStorage, Solver):
     def __init__(self, medium: Medium, sources: PointSources | None = None,
                  receivers: Receivers | None = None, nt_record: int = 0, device: str = "cuda",
                  storage: str = "bf16", check_every: int | None = None, topography=None,
-                 scaling: str = "auto"):
+                 scaling: str = "auto", texture: str = "arrays"):
         if isinstance(medium, (list, tuple)):
             raise ValueError("16-bit storage runs one grid")
         if storage not in FORMATS:
             raise ValueError(f"storage must be one of {sorted(FORMATS)}")
-        super().__init__(medium, sources, receivers, nt_record, device, topography=topography)
+        # texture: as for Solver; "table" / "rule" take the Cartesian stress
+        # update through awp_dstrqc_tex_s16 (the curvilinear one reads the
+        # per-cell arrays)
+        super().__init__(medium, sources, receivers, nt_record, device, topography=topography,
+                         texture=texture)
         self.storage = storage
         self.fmt = FORMATS[storage]
         self.scaling = scaling
@@ -564,6 +568,16 @@ class Solver16(ScaledStorage, Solver):
         L = self.levels[p]
         d, f, m = L.dims, L.f, L.m
         if s_i > e_i or s_j > e_j:
+            return
+        if L.texmode != "arrays":
+            self.lib.awp_dstrqc_tex_s16(
+                *(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",
+                                       "r5", "r6", "u1", "v1", "w1")),
+                _ptr(m["lam"]), _ptr(m["mu"]), _ptr(m["qp"]), _ptr(m["coeff"]), _ptr(m["qs"]),
+                _ptr(m["dcrjx"]), _ptr(m["dcrjy"]), _ptr(m["dcrjz"]), d.nyt, d.nzt, self.stream,
+                _ptr(m["lam_mu"]), C.addressof(L.tex),
+                _ptr(m["ww"]) if L.texmode == "table" else None,
+                d.nxt, self.NPC, 0, 0, s_i, e_i, s_j, e_j, p, self.fmt, self._sc)
             return
         self.lib.awp_dstrqc_s16(
             *(_ptr(f[n]) for n in ("xx", "yy", "zz", "xy", "xz", "yz", "r1", "r2", "r3", "r4",

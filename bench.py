@@ -50,6 +50,14 @@ BYTES16_VEL = 28.0
 BYTES16_STR = 86.0
 
 
+def _stress_entry(args) -> str:
+    """The stress launcher the anelastic workload's step calls."""
+    s16 = args.storage != "f32"
+    if args.texture == "arrays":
+        return "awp_dstrqc_s16" if s16 else "dstrqc_H"
+    return "awp_dstrqc_tex_s16" if s16 else "awp_dstrqc_tex"
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,11 +108,13 @@ def parse():
     ap.add_argument("--storage", choices=("f32", "bf16", "fp16"), default="f32",
                     help="wavefield storage of the anelastic workload (16-bit: per-field "
                          "power-of-two scales, binary32 arithmetic)")
-    ap.add_argument("--texture", choices=("arrays", "table", "rule"), default="rule",
-                    help="relaxation texture of the FP32 stress update: per-cell vx1/vx2/wwo "
+    ap.add_argument("--texture", choices=("arrays", "table", "rule"), default=None,
+                    help="relaxation texture of the stress update: per-cell vx1/vx2/wwo "
                          "arrays (dstrqc_H), or awp_dstrqc_tex's 8-entry table with the ww "
                          "array (-12 B/cell) or with init_texture's parity rule (-16 B/cell); "
-                         "the dm workload takes rule per grid where ww follows it, else the table")
+                         "the dm workload takes rule per grid where ww follows it, else the "
+                         "table.  Default: rule, except arrays for FP16 storage, whose table "
+                         "form measured slower (2.06 against 1.72 ms per stress update)")
     ap.add_argument("--fused", action="store_true",
                     help="anelastic workload: the whole step in one sweep (awp_step, state A -> "
                          "state B, texture table; 144 B/cell, 140 B with --texture rule)")
@@ -223,6 +233,8 @@ def _size(args, nx: int, ny: int, nz: int) -> tuple:
 
 def main():
     args = parse()
+    if args.texture is None:
+        args.texture = "arrays" if args.storage == "fp16" else "rule"
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of "
                           "it (the path would shard as 2-D x/y slabs with an 8-cell velocity "
@@ -286,8 +298,9 @@ def main():
     else:
         from awp_amd.solver import Solver16
         sol = Solver16(med, w.sources, w.receivers, nt_record=args.steps + args.warmup + 1,
-                       storage=args.storage)
-        b_vel, b_str = BYTES16_VEL, BYTES16_STR
+                       storage=args.storage, texture=args.texture)
+        # the binary32 medium's vx1, vx2, wwo (and ww) as for FP32 storage
+        b_vel, b_str = BYTES16_VEL, BYTES16_STR - TEXTURE_SAVES[args.texture]
     check_every = getattr(sol, "check_every", 0)
     del med
     cells = w.dims.ncells
@@ -400,14 +413,13 @@ def main():
                    "texture": args.texture,
                    "step": (("awp_step (velocity, FS ghosts and stress in one sweep)"
                              if args.fused else
-                             "dvelcx_H_opt, " + ("dstrqc_H" if args.texture == "arrays" else
-                                                 f"awp_dstrqc_tex ({args.texture})")
+                             "dvelcx_H_opt, " + (_stress_entry(args) + (
+                                 "" if args.texture == "arrays" else f" ({args.texture})"))
                              + " (+FS ghosts)") + ", addsrc_H, fstr_H, receivers")
                            + (f", free-surface volume output every {args.volume_output} steps"
                               if args.volume_output else "")},
         "roofline": {"bound": "hbm",
-                     "kernel": ("awp_step" if args.fused else
-                                "dstrqc_H" if args.texture == "arrays" else "awp_dstrqc_tex"),
+                     "kernel": ("awp_step" if args.fused else _stress_entry(args)),
                      "achieved": str_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": str_gbs / HBM_PEAK_GBS,
                      "traffic": traffic,
@@ -423,7 +435,7 @@ def main():
                      "step_events_avg_ms": float(np.mean(step_ms))} if args.fused else
                     {"dvelcx_H_opt": {"avg_ms": vel_avg, "GBps": vel_gbs,
                                       "frac": vel_gbs / HBM_PEAK_GBS},
-                     "dstrqc_H" if args.texture == "arrays" else "awp_dstrqc_tex":
+                     _stress_entry(args):
                          {"avg_ms": str_avg, "GBps": str_gbs, "frac": str_gbs / HBM_PEAK_GBS},
                      "step_events_avg_ms": float(np.mean(step_ms))}),
         "step_roofline": {"bytes_per_cell": b_vel + b_str, "achieved_GBps": step_gbs,

@@ -255,6 +255,19 @@ void awp_dstrqc_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint
                     float* vx1, float* vx2, int* ww, float* wwo, int NX, int NPC, int rankx,
                     int ranky, int s_i, int e_i, int s_j, int e_j, int d_i, int fmt,
                     const float* scale);
+/* awp_dstrqc_s16 with the relaxation texture of awp_dstrqc_tex: tex (host,
+ * 24 floats: vx1[8], vx2[8], wwo[8]) indexed by ww - 1, ww read, or with ww
+ * NULL from init_texture's parity rule (16 B/cell less than the 86 B of
+ * awp_dstrqc_s16).  The same results whenever the arrays awp_dstrqc_s16
+ * would read hold tex[ww - 1]. */
+void awp_dstrqc_tex_s16(uint16_t* xx, uint16_t* yy, uint16_t* zz, uint16_t* xy, uint16_t* xz,
+                        uint16_t* yz, uint16_t* r1, uint16_t* r2, uint16_t* r3, uint16_t* r4,
+                        uint16_t* r5, uint16_t* r6, uint16_t* u1, uint16_t* v1, uint16_t* w1,
+                        float* lam, float* mu, float* qp, float* coeff, float* qs, float* dcrjx,
+                        float* dcrjy, float* dcrjz, int nyt, int nzt, hipStream_t St,
+                        float* lam_mu, const float* tex, int* ww, int NX, int NPC, int rankx,
+                        int ranky, int s_i, int e_i, int s_j, int e_j, int d_i, int fmt,
+                        const float* scale);
 void awp_fstr_s16(uint16_t* zz, uint16_t* xz, uint16_t* yz, hipStream_t St, int s_i, int e_i,
                   int s_j, int e_j, int fmt, const float* scale);
 void awp_addsrc_s16(int i, int READ_STEP, int dim, int* psrc, int npsrc, hipStream_t St,

@@ -175,19 +175,26 @@ def test_gpu_codecs(gpu, oracle_lib, fmt):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("storage,scaling", [("bf16", "auto"), ("bf16", "scaled"),
-                                             ("fp16", "auto")])
-def test_gpu_16bit_run_bit_exact(gpu, oracle_lib, storage, scaling):
+@pytest.mark.parametrize("storage,scaling,texture", [("bf16", "auto", "arrays"),
+                                                     ("bf16", "scaled", "arrays"),
+                                                     ("fp16", "auto", "arrays"),
+                                                     ("bf16", "auto", "rule"),
+                                                     ("fp16", "auto", "table")])
+def test_gpu_16bit_run_bit_exact(gpu, oracle_lib, storage, scaling, texture):
     """A 50-step anelastic run with 16-bit storage (velocity, stress with FS
     ghosts, source, fstr, receivers; with scaling, rescaling every 8 steps):
     stored values and traces bit-exact against the oracle's 16-bit run.
     bfloat16 "auto" keeps unit scales and no checks, "scaled" follows the
-    binary16 policy."""
+    binary16 policy.  texture "rule" / "table": the stress update with the
+    8-entry relaxation-texture table (awp_dstrqc_tex_s16), ww from
+    init_texture's parity rule or read."""
     from awp_amd.solver import Solver16
     from oracle.oracle import OracleSolver16
     w = _small()
     med = w.medium()
-    g = Solver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage, scaling=scaling)
+    g = Solver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage, scaling=scaling,
+                 texture=texture)
+    assert g.levels[0].texmode == texture
     g.run(w.nt, check_every=10)
     o = OracleSolver16(med, w.sources, w.receivers, nt_record=w.nt, storage=storage,
                        scaling=scaling)
