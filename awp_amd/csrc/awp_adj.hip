@@ -356,33 +356,14 @@ __device__ __forceinline__ Vec3 strain_gather(const CPtr6& E, long long c, long 
   return r;
 }
 
-// stress^T, gather part on levels [k0, k1] (interior columns).
-__global__ __launch_bounds__(256) void adj_strain_gather_kernel(float* __restrict__ ub,
-                                                                float* __restrict__ vb,
-                                                                float* __restrict__ wb,
-                                                                CPtr6 E, const AdjGrid G,
-                                                                int k0, int k1) {
-  const BandCell b = band_cell(G, k0, k1);
-  if (!b.ok) return;
-  const long long S = G.slice, Y = G.yline;
-  const long long c = aidx(G, b.i, b.j, b.k);
-  const Vec3 g = strain_gather(E, c, S, Y, G.c1, G.c2);
-  ub[c] += g.u;
-  vb[c] += g.v;
-  wb[c] += g.w;
-}
-
-// ghost^T, part 1 (per column): the adjoints of ug, vg and of the bracket of
-// the w ghost, and the lam_mu gradient.
-__global__ __launch_bounds__(256) void adj_ghost1_kernel(const float* __restrict__ ub,
-                                                         const float* __restrict__ vb,
-                                                         const float* __restrict__ wb,
-                                                         CPtr3 V, const float* __restrict__ lam_mu,
-                                                         float* __restrict__ col,
-                                                         float* __restrict__ glm, const AdjGrid G) {
-  const int j = 2 + AWP_NGSL + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int i = 2 + AWP_NGSL + (int)(blockIdx.y * 4 + threadIdx.y);
-  if (j >= G.ny + 2 + AWP_NGSL || i >= G.nx + 2 + AWP_NGSL) return;
+// ghost^T, part 1 (column (i, j)): the adjoints of ug, vg and of the bracket
+// of the w ghost, and the lam_mu gradient, from the ghost-level adjoints
+// ubg, vbg, wbg (ub, vb, wb at top+1).
+__device__ __forceinline__ void ghost1_column(int i, int j, float ubg, float vbg, float wbg,
+                                              const CPtr3& V,
+                                              const float* __restrict__ lam_mu,
+                                              float* __restrict__ col,
+                                              float* __restrict__ glm, const AdjGrid& G) {
   const long long S = G.slice, Y = G.yline;
   const long long p = aidx(G, i, j, G.top);
   const long long q2 = (long long)i * G.lam_mu_ld + j;
@@ -395,25 +376,23 @@ __global__ __launch_bounds__(256) void adj_ghost1_kernel(const float* __restrict
   const float vs1 = cx ? u[p + S] - (w[p + S] - w[p]) : 0.0f;
   const float vs2 = cy ? v[p - Y] - (w[p] - w[p - Y]) : 0.0f;
   const float Bk = (vs1 - ug) + (u[p + S] - u[p]) + (vg - vs2) + (v[p] - v[p - Y]);
-  const float wgb = wb[p + 1];
+  const float wgb = wbg;
   const float s = -lam_mu[q2] * wgb;
   glm[q2] += -wgb * Bk;
   const long long n2 = (long long)(G.nx + 4 + AWP_NGSL2) * G.lam_mu_ld;
-  col[q2] = ub[p + 1] - s;       // adjoint of ug
-  col[n2 + q2] = vb[p + 1] + s;  // adjoint of vg
-  col[2 * n2 + q2] = s;          // adjoint of the bracket
+  col[q2] = ubg - s;       // adjoint of ug
+  col[n2 + q2] = vbg + s;  // adjoint of vg
+  col[2 * n2 + q2] = s;    // adjoint of the bracket
 }
 
-// ghost^T, part 2 (per column): gather the ghost adjoints onto the cells the
-// ghosts were computed from, then clear the ghost adjoints.
-__global__ __launch_bounds__(256) void adj_ghost2_kernel(float* __restrict__ ub,
-                                                         float* __restrict__ vb,
-                                                         float* __restrict__ wb,
-                                                         const float* __restrict__ col,
-                                                         const AdjGrid G) {
-  const int j = 2 + AWP_NGSL + (int)(blockIdx.x * 64 + threadIdx.x);
-  const int i = 2 + AWP_NGSL + (int)(blockIdx.y * 4 + threadIdx.y);
-  if (j >= G.ny + 2 + AWP_NGSL || i >= G.nx + 2 + AWP_NGSL) return;
+// ghost^T, part 2 (column (i, j)): gather the ghost adjoints onto the cells
+// the ghosts were computed from, then clear the ghost adjoints.  The w
+// ghost's own term, wb[top-1] += wb[top+1], is left to the caller (the lane
+// of level top-1).
+__device__ __forceinline__ void ghost2_column(int i, int j, float* __restrict__ ub,
+                                              float* __restrict__ vb, float* __restrict__ wb,
+                                              const float* __restrict__ col,
+                                              const AdjGrid& G) {
   const long long p = aidx(G, i, j, G.top);
   const int L = G.lam_mu_ld;
   const long long n2 = (long long)(G.nx + 4 + AWP_NGSL2) * L;
@@ -443,40 +422,23 @@ __global__ __launch_bounds__(256) void adj_ghost2_kernel(float* __restrict__ ub,
   ub[p] += du;
   vb[p] += dv;
   wb[p] += dw;
-  wb[p - 1] += wb[p + 1];                  // w[top-1] of the w ghost
   ub[p + 1] = 0.0f;
   vb[p + 1] = 0.0f;
   wb[p + 1] = 0.0f;
 }
 
-// velocity^T, local part: A = dcrj * f_d * adjoint (the weights the stress
-// adjoints gather), the velocity adjoint scaled by dcrj, and the f_d
-// gradients with the stress differences of the state before the step.
-// GATHER: the strain -> velocity gather of the same cell first (levels that
-// the free-surface ghosts do not touch), so the velocity adjoints make one
-// trip through HBM instead of two; one column's levels per wave.  Without
-// GATHER the launch covers the free-surface band (band_cell's lane map).
+// velocity^T, local part at cell (i, j, k): A = dcrj * f_d * adjoint (the
+// weights the stress adjoints gather), the velocity adjoint scaled by dcrj,
+// and the f_d gradients with the stress differences of the state before the
+// step.  GATHER: the strain -> velocity gather of the same cell first (levels
+// that the free-surface ghosts do not touch), so the velocity adjoints make
+// one trip through HBM instead of two.
 template <bool GATHER>
-__global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ ub,
-                                                            float* __restrict__ vb,
-                                                            float* __restrict__ wb, Ptr6 Aw,
-                                                            CPtr6 Sg, Medium M, Grad8 Gr,
-                                                            const AdjGrid G, CPtr6 E, int k0,
-                                                            int k1) {
-  int i, j, k;
-  if constexpr (GATHER) {
-    const Blk3 B = adj_block(G);
-    k = k0 + (int)(B.x * 64 + threadIdx.x);
-    j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
-    i = 2 + AWP_NGSL + B.z;
-    if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
-  } else {
-    const BandCell b = band_cell(G, k0, k1);
-    if (!b.ok) return;
-    i = b.i;
-    j = b.j;
-    k = b.k;
-  }
+__device__ __forceinline__ void vel_local_cell(int i, int j, int k, float* __restrict__ ub,
+                                               float* __restrict__ vb, float* __restrict__ wb,
+                                               const Ptr6& Aw, const CPtr6& Sg, const Medium& M,
+                                               const Grad8& Gr, const AdjGrid& G,
+                                               const CPtr6& E) {
   const long long S = G.slice, Y = G.yline;
   const long long p = aidx(G, i, j, k);
   const float* d1 = M.d1;
@@ -512,6 +474,67 @@ __global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ 
   ub[p] = dcrj * a;
   vb[p] = dcrj * b;
   wb[p] = dcrj * c;
+}
+
+// The strain gather and velocity^T below the free-surface band, levels
+// [k0, k1], one column's levels per wave.
+__global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ ub,
+                                                            float* __restrict__ vb,
+                                                            float* __restrict__ wb, Ptr6 Aw,
+                                                            CPtr6 Sg, Medium M, Grad8 Gr,
+                                                            const AdjGrid G, CPtr6 E, int k0,
+                                                            int k1) {
+  const Blk3 B = adj_block(G);
+  const int k = k0 + (int)(B.x * 64 + threadIdx.x);
+  const int j = 2 + AWP_NGSL + (int)(B.y * 4 + threadIdx.y);
+  const int i = 2 + AWP_NGSL + B.z;
+  if (k > k1 || j >= G.ny + 2 + AWP_NGSL) return;
+  vel_local_cell<true>(i, j, k, ub, vb, wb, Aw, Sg, M, Gr, G, E);
+}
+
+// The free-surface band, first part (levels top-1 .. top+1, band_cell's lane
+// map): the strain -> velocity gather of each cell, then, on the ghost level
+// top+1, ghost^T part 1 of the cell's column from the values just gathered.
+// Part 1 reads only its own column's ghost level, so it needs no barrier
+// after the gathers of the other columns.
+__global__ __launch_bounds__(256) void adj_band_gather_kernel(
+    float* __restrict__ ub, float* __restrict__ vb, float* __restrict__ wb, CPtr6 E, CPtr3 V,
+    const float* __restrict__ lam_mu, float* __restrict__ col, float* __restrict__ glm,
+    const AdjGrid G) {
+  const BandCell b = band_cell(G, G.top - 1, G.top + 1);
+  if (!b.ok) return;
+  const long long S = G.slice, Y = G.yline;
+  const long long c = aidx(G, b.i, b.j, b.k);
+  const Vec3 g = strain_gather(E, c, S, Y, G.c1, G.c2);
+  const float un = ub[c] + g.u, vn = vb[c] + g.v, wn = wb[c] + g.w;
+  ub[c] = un;
+  vb[c] = vn;
+  wb[c] = wn;
+  if (b.k == G.top + 1) ghost1_column(b.i, b.j, un, vn, wn, V, lam_mu, col, glm, G);
+}
+
+// The free-surface band, second part (levels top-1 and top, band_cell's lane
+// map: the two lanes of a column are neighbours in one wave).  The lane of
+// level top runs ghost^T part 2 of the column (its ghost adjoints gathered
+// from the neighbours' part 1) and hands the w ghost's adjoint wb[top+1],
+// read before part 2 clears it, to the lane of level top-1, which adds it to
+// its own cell; then each lane runs velocity^T on its cell, which reads only
+// that cell's adjoints.
+__global__ __launch_bounds__(256) void adj_band_ghost_kernel(
+    float* __restrict__ ub, float* __restrict__ vb, float* __restrict__ wb,
+    const float* __restrict__ col, Ptr6 Aw, CPtr6 Sg, Medium M, Grad8 Gr, const AdjGrid G) {
+  const BandCell b = band_cell(G, G.top - 1, G.top);
+  if (!b.ok) return;  // per column: both lanes of a column return or neither
+  const bool at_top = b.k == G.top;
+  const long long p = aidx(G, b.i, b.j, G.top);
+  const float wg = at_top ? wb[p + 1] : 0.0f;
+  const float wg_top = __shfl_down(wg, 1);  // the top lane's wb[top+1], on the top-1 lane
+  if (at_top)
+    ghost2_column(b.i, b.j, ub, vb, wb, col, G);
+  else
+    wb[p - 1] += wg_top;  // w[top-1] of the w ghost
+  const CPtr6 none = {};
+  vel_local_cell<false>(b.i, b.j, b.k, ub, vb, wb, Aw, Sg, M, Gr, G, none);
 }
 
 // velocity^T, gather part: stress adjoints from the weighted velocity
@@ -771,19 +794,17 @@ void awp_adjoint_step(float* const* adj, const float* const* fwd_v, const float*
   // free-surface ghosts' transpose reads and writes only top-1 .. top+1)
   const int kf1 = G.top - 2;
   const int nkf = kf1 - AWP_ALIGN + 1;
-  hipLaunchKernelGGL((adj_vel_local_kernel<true>),
-                     dim3(awp_cdiv(nkf, 64), awp_cdiv(G.ny, 4), G.nx), dim3(64, 4), 0, St,
-                     A.f[12], A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec, AWP_ALIGN, kf1);
-  // levels top-1 .. top+1: gather, ghosts^T, then velocity^T (band launches:
-  // (cdiv(nk * ny, 256), 1, nx) blocks for nk levels)
+  hipLaunchKernelGGL(adj_vel_local_kernel, dim3(awp_cdiv(nkf, 64), awp_cdiv(G.ny, 4), G.nx),
+                     dim3(64, 4), 0, St, A.f[12], A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec,
+                     AWP_ALIGN, kf1);
+  // levels top-1 .. top+1: gather and ghost^T part 1; then per column ghost^T
+  // part 2 and velocity^T on top-1 .. top (band launches: (cdiv(nk * ny,
+  // 256), 1, nx) blocks for nk levels)
   auto band = [&](int nk) { return dim3(awp_cdiv(nk * G.ny, 256), 1, G.nx); };
-  hipLaunchKernelGGL(adj_strain_gather_kernel, band(3), dim3(64, 4), 0, St, A.f[12], A.f[13],
-                     A.f[14], Ec, G, G.top - 1, G.top + 1);
-  hipLaunchKernelGGL(adj_ghost1_kernel, g2, b2, 0, St, A.f[12], A.f[13], A.f[14], V, M.lam_mu,
-                     col, grad_lam_mu, G);
-  hipLaunchKernelGGL(adj_ghost2_kernel, g2, b2, 0, St, A.f[12], A.f[13], A.f[14], col, G);
-  hipLaunchKernelGGL((adj_vel_local_kernel<false>), band(2), dim3(64, 4), 0, St, A.f[12],
-                     A.f[13], A.f[14], Aw, Sg, M, Gr, G, Ec, G.top - 1, G.top);
+  hipLaunchKernelGGL(adj_band_gather_kernel, band(3), dim3(64, 4), 0, St, A.f[12], A.f[13],
+                     A.f[14], Ec, V, M.lam_mu, col, grad_lam_mu, G);
+  hipLaunchKernelGGL(adj_band_ghost_kernel, band(2), dim3(64, 4), 0, St, A.f[12], A.f[13],
+                     A.f[14], col, Aw, Sg, M, Gr, G);
   // the free-surface band now; the levels below stay pending for the next
   // step (or awp_adjoint_settle)
   hipLaunchKernelGGL(adj_vel_gather_kernel, band(5), dim3(64, 4), 0, St, A, Awc, G, G.top - 2,
