@@ -13,9 +13,10 @@
 // curvilinear velocity update as three launches (0.985 against 1.043 ms for
 // one launch at 384x384x192), its strain-rate pass as three (normal rates,
 // xy, xz and yz: 2.056 -> 2.006 -> 1.948 ms per stress update for one, two,
-// three launches); the adjoint kernels' blocks in groups of 8 j tiles over
+// three launches), the velocity waves ordering their LDS at wave scope
+// (1.002 -> 0.971 ms); the adjoint kernels' blocks in groups of 8 j tiles over
 // all planes (adj_stress_local 551 -> 533 us at 256x256x128).
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 2, 0, 8};
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 2, 0, 8, 1};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -214,6 +215,8 @@ int awp_set_tuning(const char* key, int value) {
   } else if (!strcmp(key, "topo_str_split")) {
     if (value < 0 || value > 2) return -1;
     t.topo_str_split = value;
+  } else if (!strcmp(key, "topo_wsync")) {
+    t.topo_wsync = value ? 1 : 0;
   } else if (!strcmp(key, "topo_nseg")) {
     if (value < 1 || value > 8) return -1;
     t.topo_nseg = value;

@@ -86,6 +86,16 @@ struct TopoVelArgs {
 // the reference's nu * 1.0 / A with binary32 nu and A (x 1.0 is exact)
 __device__ __forceinline__ float dbl_quot(float num, float den) { return awp_quot(num, den); }
 
+// Orders one wave's LDS stores before its lanes read each other's values
+// (release, then acquire, at wavefront scope).  Enough where a wave reads
+// only the LDS it writes itself; a chunk of several waves sharing LDS needs
+// the workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // Velocity: dtopo_vel_111_unroll (velocity_unroll.cu:1-461) on levels
 // 3 <= k < nz-6 and dtopo_vel_112 (optimized_velocity.cu:1040-1627) on the top
@@ -116,6 +126,7 @@ struct VelTiles {
   int ch;     // levels per chunk, 64 * nseg
   int step;   // ch - 6
   int rj;     // j rows per workgroup (nseg * rj waves)
+  int wsync;  // 1: a wave that reads only its own LDS syncs at wave scope, not the workgroup's
 };
 
 // COMP: 0 updates u1, u2 and u3; 1, 2, 3 update only u1, u2 or u3 (the
@@ -234,7 +245,12 @@ __global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<
     // density at plane i
     const float r_k1 = A.rho[p - 1], r_k = A.rho[p], r_j = A.rho[p - yl],
                 r_jk1 = A.rho[p - yl - 1];
-    __syncthreads();
+    // each wave reads only its own LDS rows: with wsync the waves of a
+    // workgroup do not wait for each other
+    if (V.wsync)
+      wave_lds_sync();
+    else
+      __syncthreads();
 
     if (out) {
       const int q = (j + AWP_ALIGN + AWP_NGSL + 2) + (i + AWP_NGSL + 2) * fr;
@@ -652,7 +668,11 @@ __global__ __launch_bounds__(64 * VT_MAXW) void topo_str_kernel(const TopoStrArg
     if constexpr (GN) LCOL(C_UN)[kl] = ux[q + 1];
     if constexpr (GXY) LCOL(C_VN)[kl] = vx[q + 1];
     if constexpr (GZ) LCOL(C_WN)[kl] = wx[q + 1];
-    __syncthreads();
+    // one-wave chunks read only their own LDS rows (see topo_vel_kernel)
+    if (V.wsync && V.nseg == 1)
+      wave_lds_sync();
+    else
+      __syncthreads();
     if constexpr (GN) Zu[q + 1] = zu_x(LCOL(C_UN));
     if constexpr (GXY) Zv[q + 1] = zv_x(LCOL(C_VN));
     if constexpr (GZ) Zw[q + 1] = zw(LCOL(C_WN));
@@ -716,7 +736,10 @@ __global__ __launch_bounds__(64 * VT_MAXW) void topo_str_kernel(const TopoStrArg
     stage(C_WM1, wm1);
     stage(C_WP1, wp1);
     stage(C_WP2, wp2);
-    __syncthreads();
+    if (V.wsync && V.nseg == 1)
+      wave_lds_sync();
+    else
+      __syncthreads();
     if constexpr (GN) Zu[3] = zu_x(LCOL(C_UN));
     if constexpr (GXY) Zv[3] = zv_x(LCOL(C_VN));
     if constexpr (GZ) Zw[3] = zw(LCOL(C_WN));
@@ -1163,6 +1186,7 @@ void topo_velocity_launch(topo_t* T, const WaveScales& W, const char* where) {
   V.wtop0 = V.ktop0 + 3 > 3 ? V.ktop0 + 3 : 3;
   V.nint = V.wtop0 > 3 ? awp_cdiv(V.wtop0 - 3, 58) : 0;
   V.ichunk = awp_tuning().topo_vel_ichunk;
+  V.wsync = awp_tuning().topo_wsync;
   dim3 block(64, VT_BJ, 1);
   dim3 grid(V.nint + 1, awp_cdiv(T->ny, VT_BJ), awp_cdiv(T->nx, V.ichunk));
   if (awp_tuning().topo_vel_split) {
@@ -1256,6 +1280,7 @@ void topo_stress_launch(topo_t* T, const WaveScales& W, const char* where, float
   VelTiles V;
   topo_tiles(T->nz, &V);
   V.ichunk = awp_tuning().topo_str_ichunk;
+  V.wsync = awp_tuning().topo_wsync;
   dim3 block(64, V.nseg * V.rj, 1);
   dim3 grid(V.nint + 1, awp_cdiv(G.ej - G.bj, V.rj), awp_cdiv(G.ei - G.bi, V.ichunk));
   // LDS of a launch of rate group g: 2 parities x rj rows x its columns

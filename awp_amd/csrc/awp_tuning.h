@@ -23,6 +23,7 @@ struct AwpTuning {
   int topo_str_split;   // strain-rate pass: 0 one launch, 1 two (normal, shear), 2 three (normal, xy, xz+yz)
   int kw;               // k-run width of the Cartesian kernels: 0 by the idle-lane rule, 32, 64
   int topo_nseg;        // curvilinear stress kernel: waves per k chunk at most (1: one-wave segments)
+  int topo_wsync;       // curvilinear kernels: 1 wave-scope LDS sync where a wave reads only its own
 };
 
 AwpTuning& awp_tuning();

@@ -314,3 +314,36 @@ def test_gpu_topography_stress_chunks_bit_exact(gpu, oracle_lib, nseg):
                 assert np.array_equal(g.field(n), o.f[n]), (two_pass, n)
     finally:
         assert lib.awp_set_tuning(b"topo_nseg", 8) == 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nseg,wsync", [(1, 1), (8, 1), (1, 0)])
+def test_gpu_topography_wave_sync_bit_exact(gpu, oracle_lib, nseg, wsync):
+    """The curvilinear kernels with wave-scope LDS ordering where a wave
+    reads only its own LDS (awp_set_tuning("topo_wsync", 1), the default:
+    the velocity kernel, and the strain-rate kernels in one-wave segments;
+    chunks of several waves keep the workgroup barrier) and with the
+    workgroup barrier everywhere (0): 24 steps on a 200-level column,
+    bit-exact against the oracle."""
+    torch, _lib = gpu
+    from awp_amd.solver import FIELDS, Solver
+    from oracle.oracle import OracleSolver
+    lib = _lib.load()
+    w = configs.topography_hills(nx=24, ny=20, nz=200, nt=24, DH=100.0, DT=0.005, nhills=3,
+                                 hmax=500.0, wmin=600.0, wmax=1500.0, nrec=6)
+    med = w.medium()
+    topo = w.params["topography"]
+    o = OracleSolver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+    o.run(w.nt)
+    assert lib.awp_set_tuning(b"topo_wsync", wsync) == 0
+    assert lib.awp_set_tuning(b"topo_nseg", nseg) == 0
+    try:
+        g = Solver(med, w.sources, w.receivers, nt_record=w.nt, topography=topo)
+        g.run(w.nt, check_every=8)
+        torch.cuda.synchronize()
+    finally:
+        assert lib.awp_set_tuning(b"topo_wsync", 1) == 0
+        assert lib.awp_set_tuning(b"topo_nseg", 8) == 0
+    assert np.array_equal(g.traces(), o.traces())
+    for n in FIELDS:
+        assert np.array_equal(g.field(n), o.f[n]), n
