@@ -16,7 +16,7 @@
 // three launches), the velocity waves ordering their LDS at wave scope
 // (1.002 -> 0.971 ms); the adjoint kernels' blocks in groups of 8 j tiles over
 // all planes (adj_stress_local 551 -> 533 us at 256x256x128).
-static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 2, 0, 8, 1};
+static AwpTuning g_tuning = {4, 16, 1024, 2048, 0, 0, 1, 1, 8, 16, 1, 32, 2048, 0, 0, 0, 1, 8, 2, 0, 8, 1, 0};
 AwpTuning& awp_tuning() { return g_tuning; }
 
 namespace {
@@ -215,6 +215,9 @@ int awp_set_tuning(const char* key, int value) {
   } else if (!strcmp(key, "topo_str_split")) {
     if (value < 0 || value > 2) return -1;
     t.topo_str_split = value;
+  } else if (!strcmp(key, "topo_rj")) {
+    if (value < 0 || value > 8) return -1;
+    t.topo_rj = value;
   } else if (!strcmp(key, "topo_wsync")) {
     t.topo_wsync = value ? 1 : 0;
   } else if (!strcmp(key, "topo_nseg")) {

@@ -1116,7 +1116,7 @@ bool metrics_present(const TMetrics& M, const char* where) {
 // levels: nseg = min(topo_nseg, ceil(nz / 64)) waves of 64 levels per chunk,
 // so a column of up to 64 * topo_nseg levels is one chunk (nz = 192: three
 // waves, where one-wave segments of 58 outputs need four, one of them with
-// 12); rj j rows per workgroup keep it at 4-8 waves.  The velocity kernel
+// 12).  The velocity kernel
 // measured slower with chunks (1.06 against 0.99 ms at 384x384x192; its top
 // wave's one-sided rows hold the chunk's other waves at the plane barrier)
 // and keeps one-wave segments.
@@ -1129,7 +1129,12 @@ void topo_tiles(int nz, VelTiles* V) {
   V->nseg = nseg;
   V->ch = 64 * nseg;
   V->step = V->ch - 6;
-  V->rj = nseg == 1 ? 4 : (VT_MAXW / nseg < 4 ? VT_MAXW / nseg : 4);
+  // one j row per workgroup for chunks, so each plane's barrier waits only
+  // for the chunk's own waves (1.87 -> 1.78 ms per stress update at
+  // 384x384x192 against two rows); four rows of one-wave segments
+  V->rj = nseg == 1 ? 4 : 1;
+  const int rjf = awp_tuning().topo_rj;  // > 0 forces the rows per workgroup
+  if (rjf > 0) V->rj = rjf * nseg <= VT_MAXW ? rjf : VT_MAXW / nseg;
   if (V->rj < 1) V->rj = 1;
   V->ktop0 = nz - V->ch;
   V->wtop0 = V->ktop0 + 3 > 3 ? V->ktop0 + 3 : 3;

@@ -24,6 +24,7 @@ struct AwpTuning {
   int kw;               // k-run width of the Cartesian kernels: 0 by the idle-lane rule, 32, 64
   int topo_nseg;        // curvilinear stress kernel: waves per k chunk at most (1: one-wave segments)
   int topo_wsync;       // curvilinear kernels: 1 wave-scope LDS sync where a wave reads only its own
+  int topo_rj;          // curvilinear stress kernel: > 0 forces the j rows per workgroup
 };
 
 AwpTuning& awp_tuning();
