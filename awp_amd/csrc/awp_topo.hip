@@ -131,9 +131,12 @@ struct VelTiles {
 
 // COMP: 0 updates u1, u2 and u3; 1, 2, 3 update only u1, u2 or u3 (the
 // other components' loads, LDS rows and arithmetic drop out), so three
-// launches with fewer live registers each give the same results.
+// launches with fewer live registers each give the same results.  The
+// one-component launches are asked for 8 waves per SIMD (64 VGPRs: 58-64
+// from 71-75, no spills in binary32).
 template <class ST, int COMP = 0>
-__global__ __launch_bounds__(64 * VT_BJ) void topo_vel_kernel(const TopoVelArgs<ST> A,
+__global__ __launch_bounds__(64 * VT_BJ) __attribute__((amdgpu_waves_per_eu(COMP == 0 ? 1 : 8)))
+void topo_vel_kernel(const TopoVelArgs<ST> A,
                                                               const TGeom G, const VelTiles V) {
   constexpr bool DU = COMP == 0 || COMP == 1, DV = COMP == 0 || COMP == 2,
                  DW = COMP == 0 || COMP == 3;
