@@ -477,8 +477,10 @@ __device__ __forceinline__ void vel_local_cell(int i, int j, int k, float* __res
 }
 
 // The strain gather and velocity^T below the free-surface band, levels
-// [k0, k1], one column's levels per wave.
-__global__ __launch_bounds__(256) void adj_vel_local_kernel(float* __restrict__ ub,
+// [k0, k1], one column's levels per wave; asked for 8 waves per SIMD (63
+// VGPRs from 72, no spills).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+void adj_vel_local_kernel(float* __restrict__ ub,
                                                             float* __restrict__ vb,
                                                             float* __restrict__ wb, Ptr6 Aw,
                                                             CPtr6 Sg, Medium M, Grad8 Gr,
